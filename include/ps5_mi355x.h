@@ -1,0 +1,159 @@
+/*
+ * ps5_mi355x.h -- C ABI of the MI355X pS5 string sample sort.
+ *
+ * Reference interface being replaced (paths relative to the reference
+ * checkout, bingmann/parallel-string-sorting):
+ *   - Contest plug-in signatures, src/tools/contest.hpp:88-91
+ *       typedef void (*func_type)(unsigned char** strings, size_t n);
+ *       typedef void (*func_lcp_type)(unsigned char** strings, uintptr_t* lcp, size_t n);
+ *   - pS5-Unroll contestants, src/parallel/bingmann-parallel_sample_sort.cpp:31-42
+ *     (parallel_sample_sortBTCUI, formerly parallel_sample_sortBTCU2) and
+ *     :97-108 (parallel_sample_sortBTCUI_lcp, which discards its LCPs); the
+ *     LCP-returning entry is parallel_sample_sort_lcp_base,
+ *     src/parallel/bingmann-parallel_sample_sort.hpp:1733-1748.
+ *
+ * Output contract (all entry points): strings are ordered by unsigned
+ * byte-wise comparison up to the NUL terminator; byte-identical strings keep
+ * their input order; lcp[i] (i >= 1) is the length of the longest common
+ * prefix of output strings i-1 and i; lcp[0] is never written.
+ *
+ * No HIP or torch types appear here: device pointers are plain pointers and
+ * streams are passed as void* (a hipStream_t, or NULL for the context's own).
+ */
+#ifndef PS5_MI355X_H
+#define PS5_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    PS5_OK = 0,
+    PS5_ERR_ARG = 1,        /* invalid argument (size limits, NULL, alignment) */
+    PS5_ERR_HIP = 2,        /* a HIP runtime call failed */
+    PS5_ERR_CAPACITY = 3,   /* an internal work list overflowed */
+    PS5_ERR_INTERNAL = 4,   /* a device-side invariant check failed */
+    PS5_ERR_NOMEM = 5       /* device allocation failed / over budget */
+};
+
+/* Message for the last error on the calling thread ("" if none). */
+const char* ps5_last_error(void);
+
+/* ---- Contest drop-ins (host pointers) ---------------------------------- */
+
+/* Replaces parallel_sample_sortBTCUI (bingmann-parallel_sample_sort.cpp:31-37),
+ * Contest func_type (contest.hpp:88).  Permutes strings[] in place.  Like the
+ * reference it returns void: on failure it prints the error and abort()s. */
+void mi355x_ps5_sort(unsigned char** strings, size_t n);
+
+/* Replaces parallel_sample_sort_lcp_base (bingmann-parallel_sample_sort.hpp:
+ * 1733-1748) behind the Contest func_lcp_type signature (contest.hpp:89):
+ * sorts and fills lcp[1..n-1]; lcp[0] untouched.  abort()s on failure. */
+void mi355x_ps5_sort_lcp(unsigned char** strings, uintptr_t* lcp, size_t n);
+
+/* Replaces parallel_sample_sortBTCUI_out (bingmann-parallel_sample_sort.cpp:
+ * 44-62, _out_base .hpp:1704-1714): sorted pointers go to output[], strings[]
+ * is left as it was.  abort()s on failure. */
+void mi355x_ps5_sort_out(unsigned char** strings, unsigned char** output, size_t n);
+
+/* Status-returning form of the three above: lcp and output may be NULL. */
+int ps5_sort_strings(unsigned char** strings, size_t n, uintptr_t* lcp,
+                     unsigned char** output);
+
+/* ---- device-resident interface ----------------------------------------- */
+
+typedef struct ps5_ctx ps5_ctx;
+
+/* A context owns a HIP stream and the device work buffers of one sort at a
+ * time; buffers grow on demand and are kept between calls. */
+int ps5_ctx_create(ps5_ctx** out, int device);
+void ps5_ctx_destroy(ps5_ctx* ctx);
+/* Pre-size the work buffers for up to n strings (optional). */
+int ps5_ctx_reserve(ps5_ctx* ctx, uint64_t n);
+/* The context's stream as a hipStream_t cast to void*. */
+void* ps5_ctx_stream(ps5_ctx* ctx);
+
+/* Sort n strings that live in device memory.
+ *   d_arena : characters; 8-byte aligned; every string NUL-terminated; the
+ *             byte at d_arena[arena_bytes-1] must be a NUL.  Never written.
+ *   d_offs  : n uint32 start offsets into the arena (arena_bytes < 2^32).
+ *   d_perm  : n uint32 out: input index of the i-th smallest string.
+ *   d_lcp   : n uint32 out (may be NULL): lcp[i] for i >= 1, lcp[0] untouched.
+ * Runs on the context's stream; returns after the result is complete. */
+int ps5_sort_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
+                    const uint32_t* d_offs, uint64_t n,
+                    uint32_t* d_perm, uint32_t* d_lcp);
+
+/* Per-kernel accounting of the last ps5_sort_device call when profiling is
+ * on (ps5_ctx_set_profile(ctx, 1)): each launch is bracketed by HIP events on
+ * the context's stream. */
+int ps5_ctx_set_profile(ps5_ctx* ctx, int on);
+int ps5_ctx_kernel_count(ps5_ctx* ctx);
+/* name (static string), summed milliseconds, launches, algorithmic bytes */
+int ps5_ctx_kernel_stat(ps5_ctx* ctx, int i, const char** name, double* ms,
+                        uint64_t* launches, uint64_t* alg_bytes);
+/* Work statistics of the last sort: rounds (8-byte depth steps), levels,
+ * sample-sort segments, leaves, and the number of host synchronisations. */
+int ps5_ctx_work_stats(ps5_ctx* ctx, uint64_t* rounds, uint64_t* levels,
+                       uint64_t* segments, uint64_t* leaves, uint64_t* syncs);
+
+/* ---- device memory helpers (so callers need no HIP headers) ------------ */
+int ps5_dev_malloc(void** p, uint64_t bytes);
+int ps5_dev_free(void* p);
+int ps5_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
+int ps5_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
+int ps5_device_synchronize(void);
+/* Milliseconds of one ps5_sort_device call measured with HIP events on the
+ * context's stream (device-resident time). */
+int ps5_sort_device_timed(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
+                          const uint32_t* d_offs, uint64_t n,
+                          uint32_t* d_perm, uint32_t* d_lcp, double* ms);
+
+/* ---- synthetic inputs (seeded, host memory) ---------------------------- */
+
+/* Uniform strings: lengths uniform in [len_lo, len_hi], characters drawn
+ * uniformly from alphabet[0..alpha_n); strings are appended (each followed by
+ * a NUL) until the next would not fit in arena_cap bytes.  PRNG: xoshiro256**
+ * seeded through splitmix64; a uniform integer in [lo,hi] is
+ * lo + (next() >> 11) % (hi - lo + 1).  Writes at most offs_cap offsets;
+ * returns the number of strings and the used arena bytes in *arena_used.
+ * arena must hold arena_cap bytes; the unused tail is zeroed. */
+uint64_t ps5_gen_uniform(uint64_t seed, uint64_t arena_cap, uint32_t len_lo,
+                         uint32_t len_hi, const uint8_t* alphabet, uint32_t alpha_n,
+                         uint8_t* arena, uint32_t* offs, uint64_t offs_cap,
+                         uint64_t* arena_used);
+
+/* High-LCP windows: a text of text_len i.i.d. uniform ACGT characters; copies
+ * of nseg random segments of seg_len characters overwrite destinations drawn
+ * uniformly until at least repeat_frac of the text has been written; then n
+ * windows of win_len characters at uniform offsets become strings (each
+ * followed by a NUL) in arena (n*(win_len+1) bytes). */
+int ps5_gen_windows(uint64_t seed, uint64_t text_len, uint32_t nseg, uint32_t seg_len,
+                    double repeat_frac, uint64_t n, uint32_t win_len,
+                    uint8_t* arena, uint32_t* offs);
+
+/* The reference's own artificial inputs random2/4/10/62/255
+ * (src/tools/input.hpp:425-468): LCGRandom(1234567), string lengths 16..18
+ * including the NUL, letter = letters[(rng()/100) % k].  size bytes are
+ * generated and 9 NULs appended (arena must hold size+9 bytes).  Returns the
+ * string count and fills offs (capacity offs_cap). */
+uint64_t ps5_gen_reference_random(uint64_t size, const uint8_t* letters, uint32_t k,
+                                  uint8_t* arena, uint32_t* offs, uint64_t offs_cap);
+
+/* Prefix-heavy serving content: each string is one of nprefix random host
+ * prefixes (length uniform in [8,40], chars 33..126) chosen Zipf(s) over ranks,
+ * followed by a random path of length uniform in [10,60] over 33..126.
+ * Fills up to arena_cap bytes as ps5_gen_uniform does. */
+uint64_t ps5_gen_prefixed(uint64_t seed, uint64_t arena_cap, uint32_t nprefix,
+                          double zipf_s, uint8_t* arena, uint32_t* offs,
+                          uint64_t offs_cap, uint64_t* arena_used);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PS5_MI355X_H */

@@ -1,0 +1,174 @@
+// ps5_contest.cpp -- the reference's Contest entry points on host string pointers.
+//
+// Replaces (paths relative to the reference checkout):
+//   parallel_sample_sortBTCUI      src/parallel/bingmann-parallel_sample_sort.cpp:31-37
+//   parallel_sample_sortBTCUI_out  src/parallel/bingmann-parallel_sample_sort.cpp:44-62
+//   parallel_sample_sort_lcp_base  src/parallel/bingmann-parallel_sample_sort.hpp:1733-1748
+// behind the signatures of src/tools/contest.hpp:88-91.
+//
+// The strings are staged to the device as one character arena plus uint32
+// offsets.  When the pointers already describe a contiguous NUL-separated
+// arena in increasing order (the layout psstest builds, src/psstest.cpp:479-501)
+// that span is copied as is; otherwise the strings are packed first.  The
+// characters are never written.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/ps5_mi355x.h"
+
+namespace {
+
+std::mutex g_mu;
+ps5_ctx* g_ctx = nullptr;
+
+struct Staging {
+    uint8_t* d_arena = nullptr;
+    uint64_t arena_cap = 0;
+    uint32_t* d_offs = nullptr;
+    uint32_t* d_perm = nullptr;
+    uint32_t* d_lcp = nullptr;
+    uint64_t n_cap = 0;
+} g_st;
+
+unsigned host_threads()
+{
+    unsigned t = std::thread::hardware_concurrency();
+    if (t == 0) t = 1;
+    return std::min(t, 16u);
+}
+
+template <class F>
+void parallel_for(uint64_t n, F f)
+{
+    const unsigned T = host_threads();
+    if (n < (1u << 16) || T == 1) { f(0, n); return; }
+    std::vector<std::thread> th;
+    const uint64_t chunk = (n + T - 1) / T;
+    for (unsigned t = 0; t < T; ++t) {
+        const uint64_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([=] { f(b, e); });
+    }
+    for (auto& x : th) x.join();
+}
+
+int ensure_staging(uint64_t arena_bytes, uint64_t n)
+{
+    if (arena_bytes + 16 > g_st.arena_cap) {
+        if (g_st.d_arena) (void)hipFree(g_st.d_arena);
+        g_st.d_arena = nullptr;
+        g_st.arena_cap = 0;
+        void* p = nullptr;
+        if (ps5_dev_malloc(&p, arena_bytes + 16)) return PS5_ERR_NOMEM;
+        g_st.d_arena = (uint8_t*)p;
+        g_st.arena_cap = arena_bytes + 16;
+    }
+    if (n > g_st.n_cap) {
+        for (uint32_t** q : { &g_st.d_offs, &g_st.d_perm, &g_st.d_lcp }) {
+            if (*q) (void)hipFree(*q);
+            *q = nullptr;
+        }
+        g_st.n_cap = 0;
+        for (uint32_t** q : { &g_st.d_offs, &g_st.d_perm, &g_st.d_lcp }) {
+            void* p = nullptr;
+            if (ps5_dev_malloc(&p, n * sizeof(uint32_t))) return PS5_ERR_NOMEM;
+            *q = (uint32_t*)p;
+        }
+        g_st.n_cap = n;
+    }
+    return PS5_OK;
+}
+
+}  // namespace
+
+extern "C" int ps5_sort_strings(unsigned char** strings, size_t n, uintptr_t* lcp, unsigned char** output)
+{
+    if (n == 0) return PS5_OK;
+    if (!strings) return PS5_ERR_ARG;
+    if (n >= 0xFFFFFF00ull) return PS5_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_mu);
+    int rc;
+    if (!g_ctx && (rc = ps5_ctx_create(&g_ctx, 0))) return rc;
+
+    // lengths and contiguity of the pointer array
+    std::vector<uint64_t> len(n);
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) len[i] = strlen((const char*)strings[i]);
+    });
+    bool contiguous = true;
+    for (uint64_t i = 0; i + 1 < n && contiguous; ++i)
+        contiguous = strings[i + 1] == strings[i] + len[i] + 1;
+    std::vector<uint32_t> offs(n);
+    std::vector<uint8_t> packed;
+    const uint8_t* host_arena;
+    uint64_t alen;
+    if (contiguous) {
+        host_arena = strings[0];
+        alen = (uint64_t)(strings[n - 1] - strings[0]) + len[n - 1] + 1;
+        if (alen > 0xFFFFFFF0ull) return PS5_ERR_ARG;
+        for (uint64_t i = 0; i < n; ++i) offs[i] = (uint32_t)(strings[i] - strings[0]);
+    } else {
+        uint64_t tot = 0;
+        for (uint64_t i = 0; i < n; ++i) {
+            if (tot > 0xFFFFFFF0ull) return PS5_ERR_ARG;
+            offs[i] = (uint32_t)tot;
+            tot += len[i] + 1;
+        }
+        alen = tot;
+        if (alen > 0xFFFFFFF0ull) return PS5_ERR_ARG;
+        packed.resize(alen);
+        parallel_for(n, [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = b; i < e; ++i) memcpy(&packed[offs[i]], strings[i], len[i] + 1);
+        });
+        host_arena = packed.data();
+    }
+    if ((rc = ensure_staging(alen, n))) return rc;
+    if ((rc = ps5_memcpy_h2d(g_st.d_arena, host_arena, alen))) return rc;
+    if ((rc = ps5_memcpy_h2d(g_st.d_offs, offs.data(), n * sizeof(uint32_t)))) return rc;
+    if ((rc = ps5_sort_device(g_ctx, g_st.d_arena, alen, g_st.d_offs, n, g_st.d_perm, lcp ? g_st.d_lcp : nullptr)))
+        return rc;
+    std::vector<uint32_t> perm(n);
+    if ((rc = ps5_memcpy_d2h(perm.data(), g_st.d_perm, n * sizeof(uint32_t)))) return rc;
+    if (lcp) {
+        std::vector<uint32_t> l(n);
+        if ((rc = ps5_memcpy_d2h(l.data(), g_st.d_lcp, n * sizeof(uint32_t)))) return rc;
+        for (uint64_t i = 1; i < n; ++i) lcp[i] = l[i];
+    }
+    unsigned char** out = output ? output : strings;
+    if (out == strings) {
+        std::vector<unsigned char*> orig(strings, strings + n);
+        for (uint64_t i = 0; i < n; ++i) out[i] = orig[perm[i]];
+    } else {
+        for (uint64_t i = 0; i < n; ++i) out[i] = strings[perm[i]];
+    }
+    return PS5_OK;
+}
+
+static void die_on(int rc, const char* who)
+{
+    if (rc == PS5_OK) return;
+    fprintf(stderr, "%s: error %d: %s\n", who, rc, ps5_last_error());
+    abort();
+}
+
+extern "C" void mi355x_ps5_sort(unsigned char** strings, size_t n)
+{
+    die_on(ps5_sort_strings(strings, n, nullptr, nullptr), "mi355x_ps5_sort");
+}
+
+extern "C" void mi355x_ps5_sort_lcp(unsigned char** strings, uintptr_t* lcp, size_t n)
+{
+    die_on(ps5_sort_strings(strings, n, lcp, nullptr), "mi355x_ps5_sort_lcp");
+}
+
+extern "C" void mi355x_ps5_sort_out(unsigned char** strings, unsigned char** output, size_t n)
+{
+    die_on(ps5_sort_strings(strings, n, nullptr, output), "mi355x_ps5_sort_out");
+}

@@ -1,0 +1,93 @@
+// ps5_kernels.hpp -- shared definitions of the gfx950 pS5 kernels.
+//
+// Device data layout (one sort, n strings, positions 0..n-1):
+//   arena      characters as given by the caller (never written)
+//   offs[n]    uint32 start offset of input string i
+//   key[2][n]  uint64 ping-pong: 8 characters at the current depth,
+//              big-endian, zero from the first NUL on
+//   idx[2][n]  uint32 ping-pong: input index of the string at a position
+//   bktrank[n] uint32 scratch: (bucket << 21) | rank inside its tile or wave
+//   bkey[n]    uint64 scratch: depth key of the first/last position of a leaf
+//   perm[n], lcp[n]  results (caller memory)
+//
+// A sort proceeds in rounds; round r compares the 8 characters at depth 8r.
+// Inside a round, segments (contiguous position ranges whose strings share
+// their first 8r characters) are split by splitter-tree sample-sort steps
+// until every piece is a leaf: a small unsorted bucket or an equal-key bucket.
+// Leaf kernels finish a leaf at this depth (order, LCPs, permutation) and
+// turn every run of equal keys without a NUL into a segment of round r+1.
+#pragma once
+#include <cstdint>
+
+namespace ps5 {
+
+typedef uint64_t key64;
+
+constexpr int kWG = 256;                     // threads per workgroup
+constexpr int kWaves = kWG / 64;             // wave64
+constexpr int kMaxTreebits = 10;             // reference DefaultTreebits
+constexpr int kMaxSpl = (1 << kMaxTreebits) - 1;    // 1023 splitters
+constexpr int kMaxBkt = 2 * kMaxSpl + 1;            // 2047 buckets
+constexpr int kWaveChunk = 8192;             // elements ranked by one wave (multi-tile step)
+constexpr int kTile = kWaveChunk * kWaves;   // elements per workgroup (multi-tile step)
+constexpr uint32_t kTinyT = 64;              // leaves sorted by one wave
+constexpr uint32_t kSmallT = 1024;           // leaves sorted by one workgroup
+constexpr uint32_t kMedT = 65536;            // segments split by one workgroup
+constexpr uint32_t kRunChunk = 4096;         // equal-key leaves are processed in chunks
+constexpr int kRankBits = 21;
+constexpr uint32_t kRankMask = (1u << kRankBits) - 1;
+constexpr uint32_t kNoBucket = 0x7FF;        // bucket id of inactive lanes
+
+struct Item {            // a segment or a leaf
+    uint32_t begin;      // first position
+    uint32_t size;       // number of positions
+    uint32_t rbeg;       // first position of the round segment it belongs to
+    uint32_t flags;      // F_BUFB | F_SORTED | chunk index << 8
+};
+constexpr uint32_t F_BUFB = 1u;              // data lives in key[1]/idx[1]
+constexpr uint32_t F_SORTED = 2u;            // all keys equal (equal bucket)
+
+// device counters
+enum : int {
+    C_LARGE = 0,   // [2] level lists (parity)
+    C_MED = 2,     // [2]
+    C_TINY = 4,    // [2] leaf lists (round parity)
+    C_SMALL = 6,   // [2]
+    C_RUN = 8,     // [2]
+    C_ERR = 10,    // error flags
+    C_EL_MED = 11, C_EL_TINY = 12, C_EL_SMALL = 13, C_EL_RUN = 14,  // elements processed
+    C_EL_GATHER = 15,                                               // keys re-gathered
+    C_NCNT = 16
+};
+constexpr uint32_t ERR_CAP = 1u, ERR_LCP = 2u, ERR_BOUNDS = 4u;
+
+struct Sink {            // where child items are appended
+    Item* large; Item* med; Item* tiny; Item* small; Item* run;
+    uint32_t* cnt;       // counter base
+    int c_large, c_med, c_tiny, c_small, c_run;   // counter indices
+    uint32_t cap_large, cap_med, cap_tiny, cap_small, cap_run;
+};
+
+struct Bufs {
+    uint32_t n;          // number of strings (bound of every position and index)
+    uint32_t* cnt;       // counters (error flags)
+    const uint8_t* arena;
+    uint64_t alen;
+    const uint32_t* offs;
+    key64* key0; key64* key1;
+    uint32_t* idx0; uint32_t* idx1;
+    uint32_t* perm;
+    uint32_t* lcp;       // may be null
+    key64* bkey;
+    uint32_t* bktrank;
+};
+
+// treebits used for a segment of m strings: about 16 strings per bucket
+__host__ __device__ inline int treebits_for(uint32_t m, int shift)
+{
+    int lg = 31 - __builtin_clz(m | 1u);
+    int k = lg - shift;
+    return k < 1 ? 1 : (k > kMaxTreebits ? kMaxTreebits : k);
+}
+
+}  // namespace ps5

@@ -1,0 +1,286 @@
+"""ctypes binding of libps5_mi355x.so, the C ABI declared in include/ps5_mi355x.h.
+
+The library is the product: HIP kernels for gfx950 plus their host
+orchestration.  This module only marshals numpy arrays to it.  It does not
+import torch (torch ships its own HIP runtime with the same soname) and has no
+CPU fallback: when the library is missing every call raises Ps5Error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libps5_mi355x.so")
+
+PS5_OK = 0
+
+# every symbol include/ps5_mi355x.h declares
+EXPORTS = [
+    "ps5_last_error",
+    "mi355x_ps5_sort", "mi355x_ps5_sort_lcp", "mi355x_ps5_sort_out", "ps5_sort_strings",
+    "ps5_ctx_create", "ps5_ctx_destroy", "ps5_ctx_reserve", "ps5_ctx_stream",
+    "ps5_sort_device", "ps5_ctx_set_profile", "ps5_ctx_kernel_count", "ps5_ctx_kernel_stat",
+    "ps5_ctx_work_stats",
+    "ps5_dev_malloc", "ps5_dev_free", "ps5_memcpy_h2d", "ps5_memcpy_d2h", "ps5_device_synchronize",
+    "ps5_sort_device_timed",
+    "ps5_gen_uniform", "ps5_gen_windows", "ps5_gen_reference_random", "ps5_gen_prefixed",
+]
+
+
+class Ps5Error(RuntimeError):
+    pass
+
+
+_lib = None
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_U32 = ctypes.c_uint32
+_I = ctypes.c_int
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def load():
+    """Load the library (raises Ps5Error if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Ps5Error(f"{LIB_PATH} is missing: build it first (__graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    _sig(lib, "ps5_last_error", ctypes.c_char_p, [])
+    _sig(lib, "mi355x_ps5_sort", None, [_P, ctypes.c_size_t])
+    _sig(lib, "mi355x_ps5_sort_lcp", None, [_P, _P, ctypes.c_size_t])
+    _sig(lib, "mi355x_ps5_sort_out", None, [_P, _P, ctypes.c_size_t])
+    _sig(lib, "ps5_sort_strings", _I, [_P, ctypes.c_size_t, _P, _P])
+    _sig(lib, "ps5_ctx_create", _I, [ctypes.POINTER(_P), _I])
+    _sig(lib, "ps5_ctx_destroy", None, [_P])
+    _sig(lib, "ps5_ctx_reserve", _I, [_P, _U64])
+    _sig(lib, "ps5_ctx_stream", _P, [_P])
+    _sig(lib, "ps5_sort_device", _I, [_P, _P, _U64, _P, _U64, _P, _P])
+    _sig(lib, "ps5_sort_device_timed", _I, [_P, _P, _U64, _P, _U64, _P, _P, ctypes.POINTER(ctypes.c_double)])
+    _sig(lib, "ps5_ctx_set_profile", _I, [_P, _I])
+    _sig(lib, "ps5_ctx_kernel_count", _I, [_P])
+    _sig(lib, "ps5_ctx_kernel_stat", _I, [_P, _I, ctypes.POINTER(ctypes.c_char_p),
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_U64),
+                                          ctypes.POINTER(_U64)])
+    _sig(lib, "ps5_ctx_work_stats", _I, [_P] + [ctypes.POINTER(_U64)] * 5)
+    _sig(lib, "ps5_dev_malloc", _I, [ctypes.POINTER(_P), _U64])
+    _sig(lib, "ps5_dev_free", _I, [_P])
+    _sig(lib, "ps5_memcpy_h2d", _I, [_P, _P, _U64])
+    _sig(lib, "ps5_memcpy_d2h", _I, [_P, _P, _U64])
+    _sig(lib, "ps5_device_synchronize", _I, [])
+    _sig(lib, "ps5_gen_uniform", _U64, [_U64, _U64, _U32, _U32, _P, _U32, _P, _P, _U64,
+                                        ctypes.POINTER(_U64)])
+    _sig(lib, "ps5_gen_windows", _I, [_U64, _U64, _U32, _U32, ctypes.c_double, _U64, _U32, _P, _P])
+    _sig(lib, "ps5_gen_reference_random", _U64, [_U64, _P, _U32, _P, _P, _U64])
+    _sig(lib, "ps5_gen_prefixed", _U64, [_U64, _U64, _U32, ctypes.c_double, _P, _P, _U64,
+                                         ctypes.POINTER(_U64)])
+    _lib = lib
+    return lib
+
+
+def _check(rc, what):
+    if rc != PS5_OK:
+        msg = load().ps5_last_error().decode(errors="replace")
+        raise Ps5Error(f"{what} failed with status {rc}: {msg}")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------- inputs
+
+ALPHA_DIGITS = b"0123456789"
+ALPHA_ACGT = b"ACGT"
+ALPHA_ASCII94 = bytes(range(33, 127))
+ALPHA_ALNUM = b"0123456789ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz"
+
+
+def gen_uniform(seed, arena_bytes, len_lo, len_hi, alphabet):
+    """Strings with uniform lengths in [len_lo, len_hi] over `alphabet`,
+    appended until `arena_bytes` (NULs included) is full.  Returns
+    (arena uint8 with 16 zero bytes of padding, offs uint32)."""
+    lib = load()
+    alpha = np.frombuffer(bytes(alphabet), np.uint8).copy()
+    arena = np.zeros(arena_bytes + 16, np.uint8)
+    cap = arena_bytes // (len_lo + 1) + 1
+    offs = np.zeros(cap, np.uint32)
+    used = _U64(0)
+    n = lib.ps5_gen_uniform(seed, arena_bytes, len_lo, len_hi, _ptr(alpha), len(alpha),
+                            _ptr(arena), _ptr(offs), cap, ctypes.byref(used))
+    return arena[: used.value + 16], offs[:n].copy()
+
+
+def gen_windows(seed, text_len, nseg, seg_len, repeat_frac, n, win_len):
+    lib = load()
+    arena = np.zeros(n * (win_len + 1) + 16, np.uint8)
+    offs = np.zeros(n, np.uint32)
+    _check(lib.ps5_gen_windows(seed, text_len, nseg, seg_len, repeat_frac, n, win_len,
+                               _ptr(arena), _ptr(offs)), "ps5_gen_windows")
+    return arena, offs
+
+
+def gen_reference_random(size, letters):
+    """The reference's random2/4/10/62/255 inputs (src/tools/input.hpp:425-468)."""
+    lib = load()
+    let = np.frombuffer(bytes(letters), np.uint8).copy()
+    arena = np.zeros(size + 9 + 16, np.uint8)
+    cap = size // 16 + 2
+    offs = np.zeros(cap, np.uint32)
+    n = lib.ps5_gen_reference_random(size, _ptr(let), len(let), _ptr(arena), _ptr(offs), cap)
+    return arena, offs[:n].copy()
+
+
+def gen_prefixed(seed, arena_bytes, nprefix=1000, zipf_s=1.1):
+    lib = load()
+    arena = np.zeros(arena_bytes + 16, np.uint8)
+    cap = arena_bytes // 19 + 1
+    offs = np.zeros(cap, np.uint32)
+    used = _U64(0)
+    n = lib.ps5_gen_prefixed(seed, arena_bytes, nprefix, zipf_s, _ptr(arena), _ptr(offs), cap,
+                             ctypes.byref(used))
+    return arena, offs[:n].copy()
+
+
+def workload(name, scale=1.0):
+    """Named workloads; `scale` shrinks the byte/string budget for tests."""
+    if name == "random10mb":        # 10 MiB, L in [8,32], '0'..'9'
+        return gen_uniform(0x5EED0001, int(10 * 2**20 * scale), 8, 32, ALPHA_DIGITS)
+    if name == "ascii2gb":          # 2 GiB, L in [1,64], bytes 33..126
+        return gen_uniform(0x5EED0002, int(2 * 2**30 * scale), 1, 64, ALPHA_ASCII94)
+    if name == "highlcp":           # 20M windows of 100 over a 2^29 ACGT text, 5% repeats
+        n = max(1, int(20_000_000 * scale))
+        text = max(4096, int(2**29 * scale))
+        return gen_windows(0x5EED0003, text, 10, 2048, 0.05, n, 100)
+    raise ValueError(name)
+
+
+# ---------------------------------------------------------------- device API
+
+class DeviceBuffer:
+    """Device memory owned through the library (no HIP headers needed)."""
+
+    def __init__(self, nbytes):
+        lib = load()
+        p = _P()
+        _check(lib.ps5_dev_malloc(ctypes.byref(p), max(1, int(nbytes))), "ps5_dev_malloc")
+        self.ptr = p
+        self.nbytes = int(nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(load().ps5_memcpy_h2d(self.ptr, _ptr(arr), arr.nbytes), "ps5_memcpy_h2d")
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(load().ps5_memcpy_d2h(_ptr(out), self.ptr, out.nbytes), "ps5_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().ps5_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """A sort context (stream + work buffers) on one device."""
+
+    def __init__(self, device=0):
+        lib = load()
+        p = _P()
+        _check(lib.ps5_ctx_create(ctypes.byref(p), device), "ps5_ctx_create")
+        self.h = p
+
+    def close(self):
+        if self.h:
+            load().ps5_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sort_device(self, d_arena, arena_bytes, d_offs, n, d_perm, d_lcp=None, timed=False):
+        lib = load()
+        lcp = d_lcp.ptr if d_lcp is not None else None
+        if timed:
+            ms = ctypes.c_double(0)
+            _check(lib.ps5_sort_device_timed(self.h, d_arena.ptr, arena_bytes, d_offs.ptr, n, d_perm.ptr, lcp,
+                                             ctypes.byref(ms)), "ps5_sort_device_timed")
+            return ms.value
+        _check(lib.ps5_sort_device(self.h, d_arena.ptr, arena_bytes, d_offs.ptr, n, d_perm.ptr, lcp),
+               "ps5_sort_device")
+        return None
+
+    def sort(self, arena, offs, with_lcp=True):
+        """Upload, sort on the device, download: returns (perm, lcp)."""
+        n = len(offs)
+        arena_bytes = int(arena.size)
+        # the last byte of the arena must be a NUL (padding is zero)
+        da = DeviceBuffer(arena_bytes)
+        do = DeviceBuffer(max(4, n * 4))
+        dp = DeviceBuffer(max(4, n * 4))
+        dl = DeviceBuffer(max(4, n * 4)) if with_lcp else None
+        try:
+            da.upload(arena)
+            do.upload(np.ascontiguousarray(offs, np.uint32))
+            self.sort_device(da, arena_bytes, do, n, dp, dl)
+            perm = dp.download(np.uint32, n)
+            lcp = dl.download(np.uint32, n) if with_lcp else None
+        finally:
+            for b in (da, do, dp, dl):
+                if b is not None:
+                    b.free()
+        return perm, lcp
+
+    def set_profile(self, on):
+        _check(load().ps5_ctx_set_profile(self.h, 1 if on else 0), "ps5_ctx_set_profile")
+
+    def kernel_stats(self):
+        lib = load()
+        out = {}
+        for i in range(lib.ps5_ctx_kernel_count(self.h)):
+            name = ctypes.c_char_p()
+            ms = ctypes.c_double()
+            la = _U64()
+            by = _U64()
+            _check(lib.ps5_ctx_kernel_stat(self.h, i, ctypes.byref(name), ctypes.byref(ms), ctypes.byref(la),
+                                           ctypes.byref(by)), "ps5_ctx_kernel_stat")
+            out[name.value.decode()] = {"ms": ms.value, "launches": la.value, "bytes": by.value}
+        return out
+
+    def work_stats(self):
+        vals = [_U64() for _ in range(5)]
+        _check(load().ps5_ctx_work_stats(self.h, *[ctypes.byref(v) for v in vals]), "ps5_ctx_work_stats")
+        return dict(zip(["rounds", "levels", "segments", "leaves", "syncs"], [v.value for v in vals]))
+
+
+def sort_strings(strings, with_lcp=False):
+    """Sort a list of bytes objects through the Contest-shaped host entry
+    (ps5_sort_strings): returns (sorted list, lcp array or None)."""
+    lib = load()
+    n = len(strings)
+    bufs = [ctypes.create_string_buffer(s, len(s) + 1) for s in strings]
+    arr = (ctypes.c_void_p * max(1, n))(*[ctypes.addressof(b) for b in bufs])
+    lcp = np.full(max(1, n), 42, np.uint64) if with_lcp else None
+    _check(lib.ps5_sort_strings(arr, n, _ptr(lcp) if with_lcp else None, None), "ps5_sort_strings")
+    addr = {ctypes.addressof(b): i for i, b in enumerate(bufs)}
+    order = [addr[arr[i]] for i in range(n)]
+    return [strings[i] for i in order], order, (lcp[:n] if with_lcp else None)
