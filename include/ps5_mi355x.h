@@ -107,6 +107,8 @@ int ps5_dev_free(void* p);
 int ps5_memcpy_h2d(void* dst, const void* src, uint64_t bytes);
 int ps5_memcpy_d2h(void* dst, const void* src, uint64_t bytes);
 int ps5_device_synchronize(void);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned). */
+int ps5_dev_copy(void* dst, const void* src, uint64_t bytes);
 /* Milliseconds of one ps5_sort_device call measured with HIP events on the
  * context's stream (device-resident time). */
 int ps5_sort_device_timed(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,

@@ -1,0 +1,199 @@
+// ps5_device.hpp -- device helpers of the gfx950 pS5 kernels (wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ps5_kernels.hpp"
+
+namespace ps5 {
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// 8 characters of the arena at byte offset `off`, big-endian, zeroed from the
+// first NUL on (the reference's get_char_uint64_simple, stringset.hpp:201-231).
+// Two aligned 8-byte loads and a funnel shift; near the arena end the bytes
+// are fetched one by one so nothing outside arena[0, alen) is read.
+__device__ __forceinline__ key64 load_key(const uint8_t* arena, uint64_t alen, uint64_t off)
+{
+    const uint64_t base = (uint64_t)(uintptr_t)arena;
+    const uint64_t a = base + off;
+    const uint64_t a8 = a & ~7ull;
+    uint64_t lo, hi;
+    if (a8 + 16 <= base + alen) {
+        lo = *(const uint64_t*)(uintptr_t)a8;
+        hi = *(const uint64_t*)(uintptr_t)(a8 + 8);
+    } else {
+        lo = 0;
+        hi = 0;
+        for (int i = 0; i < 16; ++i) {
+            uint64_t p = a8 + i;
+            if (p >= base && p < base + alen) {
+                uint64_t c = *(const uint8_t*)(uintptr_t)p;
+                if (i < 8) lo |= c << (8 * i);
+                else hi |= c << (8 * (i - 8));
+            }
+        }
+    }
+    const uint32_t sh = (uint32_t)(a & 7) * 8;
+    uint64_t v = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+    v = __builtin_bswap64(v);
+    // high bit of each byte of t is set iff that byte is non-zero (no carries)
+    const uint64_t t = ((v & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | v;
+    const uint64_t z = ~t & 0x8080808080808080ull;
+    if (z) {
+        const uint32_t p = (uint32_t)__clzll((long long)z);   // 8 * index of the first NUL
+        v = p ? (v & (~0ull << (64 - p))) : 0ull;
+    }
+    return v;
+}
+
+// characters the two keys share (lcpKeyType, bingmann-parallel_sample_sort.hpp:225-230)
+__device__ __forceinline__ uint32_t lcp_key(key64 a, key64 b)
+{
+    key64 x = a ^ b;
+    return x ? ((uint32_t)__clzll((long long)x) >> 3) : 8u;
+}
+// non-NUL characters of a key (lcpKeyDepth, bingmann-parallel_sample_sort.hpp:232-237)
+__device__ __forceinline__ uint32_t key_depth(key64 a)
+{
+    return a ? 8u - ((uint32_t)__builtin_ctzll(a) >> 3) : 0u;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+__device__ __forceinline__ uint32_t wave_id() { return threadIdx.x >> 6; }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+// lanes holding the same 11-bit bucket id (all 64 lanes must execute this)
+__device__ __forceinline__ uint64_t match_bucket(uint32_t b)
+{
+    uint64_t peers = __ballot(1);
+#pragma unroll
+    for (int i = 0; i < 11; ++i) {
+        const bool bit = (b >> i) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    return peers;
+}
+
+__device__ __forceinline__ void flag_error(uint32_t* cnt, uint32_t e) { atomicOr(&cnt[C_ERR], e); }
+
+// an item must describe a non-empty range inside [0, n)
+__device__ __forceinline__ bool item_ok(const Item& it, const Bufs& B)
+{
+    return it.size != 0 && it.begin < B.n && it.size <= B.n - it.begin;
+}
+
+__device__ __forceinline__ void put_item(Item* list, uint32_t cap, uint32_t slot, Item it, uint32_t* cnt)
+{
+    if (slot < cap) list[slot] = it;
+    else flag_error(cnt, ERR_CAP);
+}
+
+// Append for every lane with pred: returns its slot (wave-aggregated atomic).
+__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
+{
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return 0xFFFFFFFFu;
+    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, (int)leader, 64);
+    return base + (uint32_t)__popcll(m & lanemask_lt());
+}
+
+// Append an unsorted segment (any size) to the list of its size class.
+__device__ __forceinline__ void wave_append_segment(bool pred, Item it, const Sink& s)
+{
+    const uint32_t m = it.size;
+    const bool t = pred && m <= kTinyT;
+    const bool bl = pred && m > kTinyT && m <= kBlkT;
+    const bool lg = pred && m > kBlkT;
+    uint32_t slot;
+    slot = wave_append(t, &s.cnt[s.c_tiny]);
+    if (t) put_item(s.tiny, s.cap_tiny, slot, it, s.cnt);
+    slot = wave_append(bl, &s.cnt[s.c_block]);
+    if (bl) put_item(s.block, s.cap_block, slot, it, s.cnt);
+    slot = wave_append(lg, &s.cnt[s.c_large]);
+    if (lg) put_item(s.large, s.cap_large, slot, it, s.cnt);
+}
+
+// Exclusive block scan of one uint32 per thread (NW waves); *total = sum.
+template <int NW>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total, uint32_t* s_tmp)
+{
+    const uint32_t lane = lane_id(), w = wave_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    if (lane == 63) s_tmp[w] = x;
+    __syncthreads();
+    uint32_t wpre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        uint32_t t = s_tmp[i];
+        wpre += (i < (int)w) ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return wpre + x - v;
+}
+
+// Bitonic sort of P (power of two) uint64 in LDS by a workgroup of NT threads.
+template <int NT>
+__device__ void bitonic_sort_u64(key64* s, int P)
+{
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < (P >> 1); i += NT) {
+                // i-th compare-exchange pair: low index with bit j cleared
+                const int lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                const int hi = lo | j;
+                const key64 a = s[lo], b = s[hi];
+                const bool up = (lo & k) == 0;
+                if ((a > b) == up) { s[lo] = b; s[hi] = a; }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// In-order index of level-order node t of a complete tree with k levels.
+__device__ __forceinline__ int inorder_of(int t, int k)
+{
+    const int l = 31 - __clz(t);
+    const int j = t - (1 << l);
+    return ((2 * j + 1) << (k - l - 1)) - 1;
+}
+
+// Bucket of a key: b = 2 * lower_bound + [splitter == key]
+// (ClassifyTreeSimple::find_bkt, bingmann-sample_sortBTC.hpp:45-72).  The
+// smallest splitter >= key is the node of the last left turn of the descent,
+// so the equality test needs no separate in-order splitter array.
+__device__ __forceinline__ uint32_t classify_key(key64 key, const key64* s_tree, int k)
+{
+    uint32_t i = 1;
+    key64 cand = 0;
+    bool left = false;
+    for (int l = 0; l < k; ++l) {
+        const key64 t = s_tree[i];
+        const bool gt = key > t;
+        if (!gt) { cand = t; left = true; }
+        i = 2 * i + (gt ? 1u : 0u);
+    }
+    i -= (1u << k);
+    return 2 * i + ((left && cand == key) ? 1u : 0u);
+}
+
+}  // namespace ps5

@@ -6,16 +6,18 @@
 //   key[2][n]  uint64 ping-pong: 8 characters at the current depth,
 //              big-endian, zero from the first NUL on
 //   idx[2][n]  uint32 ping-pong: input index of the string at a position
-//   bktrank[n] uint32 scratch: (bucket << 21) | rank inside its tile or wave
+//   bktrank[n] uint32 scratch: (bucket << 21) | rank inside its wave chunk
 //   bkey[n]    uint64 scratch: depth key of the first/last position of a leaf
 //   perm[n], lcp[n]  results (caller memory)
 //
 // A sort proceeds in rounds; round r compares the 8 characters at depth 8r.
 // Inside a round, segments (contiguous position ranges whose strings share
-// their first 8r characters) are split by splitter-tree sample-sort steps
-// until every piece is a leaf: a small unsorted bucket or an equal-key bucket.
-// Leaf kernels finish a leaf at this depth (order, LCPs, permutation) and
-// turn every run of equal keys without a NUL into a segment of round r+1.
+// their first 8r characters) are split by multi-workgroup splitter-tree steps
+// until every piece is a leaf: a block (at most kBlkT strings, sorted
+// completely in LDS by one workgroup), a tiny leaf (at most 64 strings, one
+// wave) or an equal-key bucket.  Leaves finish their strings at this depth
+// (order, LCPs, permutation) and turn every run of equal keys without a NUL
+// into a segment of round r+1.
 #pragma once
 #include <cstdint>
 
@@ -23,16 +25,18 @@ namespace ps5 {
 
 typedef uint64_t key64;
 
-constexpr int kWG = 256;                     // threads per workgroup
+constexpr int kWG = 256;                     // threads per workgroup (small kernels)
 constexpr int kWaves = kWG / 64;             // wave64
+constexpr int kBigWG = 1024;                 // threads of the LDS-heavy kernels
+constexpr int kBigWaves = kBigWG / 64;
 constexpr int kMaxTreebits = 10;             // reference DefaultTreebits
 constexpr int kMaxSpl = (1 << kMaxTreebits) - 1;    // 1023 splitters
 constexpr int kMaxBkt = 2 * kMaxSpl + 1;            // 2047 buckets
-constexpr int kWaveChunk = 8192;             // elements ranked by one wave (multi-tile step)
-constexpr int kTile = kWaveChunk * kWaves;   // elements per workgroup (multi-tile step)
+constexpr int kMaxSamples = 4096;            // sample keys of one splitter tree
+constexpr int kWaveChunk = 8192;             // keys ranked by one wave (multi-workgroup step)
+constexpr int kTile = kWaveChunk * kWaves;   // keys per count workgroup
 constexpr uint32_t kTinyT = 64;              // leaves sorted by one wave
-constexpr uint32_t kSmallT = 1024;           // leaves sorted by one workgroup
-constexpr uint32_t kMedT = 65536;            // segments split by one workgroup
+constexpr uint32_t kBlkT = 1024;             // leaves sorted on chip by one workgroup
 constexpr uint32_t kRunChunk = 4096;         // equal-key leaves are processed in chunks
 constexpr int kRankBits = 21;
 constexpr uint32_t kRankMask = (1u << kRankBits) - 1;
@@ -49,23 +53,34 @@ constexpr uint32_t F_SORTED = 2u;            // all keys equal (equal bucket)
 
 // device counters
 enum : int {
-    C_LARGE = 0,   // [2] level lists (parity)
-    C_MED = 2,     // [2]
+    C_LROUND = 0,  // [2] large segments starting a round (round parity)
+    C_LLVL = 2,    // [2] large segments of the next level (level parity)
     C_TINY = 4,    // [2] leaf lists (round parity)
-    C_SMALL = 6,   // [2]
+    C_BLOCK = 6,   // [2]
     C_RUN = 8,     // [2]
     C_ERR = 10,    // error flags
-    C_EL_MED = 11, C_EL_TINY = 12, C_EL_SMALL = 13, C_EL_RUN = 14,  // elements processed
-    C_EL_GATHER = 15,                                               // keys re-gathered
     C_NCNT = 16
 };
 constexpr uint32_t ERR_CAP = 1u, ERR_LCP = 2u, ERR_BOUNDS = 4u;
 
-struct Sink {            // where child items are appended
-    Item* large; Item* med; Item* tiny; Item* small; Item* run;
+struct LargeDesc {       // one segment of the multi-workgroup sample-sort step
+    Item seg;
+    uint32_t T;          // tiles of kTile keys
+    uint32_t hist_base;  // first histogram entry of this segment
+    uint32_t k;          // treebits
+    uint32_t oversample; // sample keys per splitter
+};
+
+struct TileRef {         // one workgroup of the multi-workgroup step
+    uint32_t seg;        // index into the LargeDesc array
+    uint32_t t;          // tile inside the segment
+};
+
+struct Sink {            // where new items are appended
+    Item* large; Item* block; Item* tiny; Item* run;
     uint32_t* cnt;       // counter base
-    int c_large, c_med, c_tiny, c_small, c_run;   // counter indices
-    uint32_t cap_large, cap_med, cap_tiny, cap_small, cap_run;
+    int c_large, c_block, c_tiny, c_run;     // counter indices
+    uint32_t cap_large, cap_block, cap_tiny, cap_run;
 };
 
 struct Bufs {
@@ -81,13 +96,5 @@ struct Bufs {
     key64* bkey;
     uint32_t* bktrank;
 };
-
-// treebits used for a segment of m strings: about 16 strings per bucket
-__host__ __device__ inline int treebits_for(uint32_t m, int shift)
-{
-    int lg = 31 - __builtin_clz(m | 1u);
-    int k = lg - shift;
-    return k < 1 ? 1 : (k > kMaxTreebits ? kMaxTreebits : k);
-}
 
 }  // namespace ps5

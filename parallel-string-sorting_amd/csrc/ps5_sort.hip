@@ -1,4 +1,4 @@
-// ps5_sort.hip -- pS5 string sample sort on MI355X (gfx950), HIP kernels and
+// ps5_sort.hip -- pS5 string sample sort on MI355X (gfx950): HIP kernels and
 // the host orchestration behind the C ABI in include/ps5_mi355x.h.
 //
 // Reference algorithm (paths relative to the reference checkout):
@@ -6,320 +6,28 @@
 //   splitter tree classify                   src/sequential/bingmann-sample_sortBTC.hpp:45-72,297-399
 //   bucket recursion rules                   src/parallel/bingmann-parallel_sample_sort.hpp:1492-1584
 //   bucket-boundary LCPs                     src/parallel/bingmann-parallel_sample_sort.hpp:311-396
+//   LCP insertion sort base case             src/sequential/bingmann-lcp_inssort.hpp:36-187
 //   key extraction                           src/tools/stringset.hpp:201-231
 // The GPU formulation keeps the reference's outputs but reorganises the work
-// breadth-first: every round handles all segments at one 8-byte depth, and
-// every pass is a kernel over a device-side work list.
+// breadth-first: every round handles all segments at one 8-byte depth, every
+// pass is a kernel over a device-side work list, and buckets small enough for
+// LDS are finished by one workgroup without another trip through HBM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <mutex>
 #include <string>
 #include <vector>
 
+#include "ps5_device.hpp"
 #include "ps5_kernels.hpp"
 #include "../../include/ps5_mi355x.h"
 
 using namespace ps5;
-
-// ===========================================================================
-// device helpers
-// ===========================================================================
-
-__device__ __forceinline__ uint64_t mix64(uint64_t z)
-{
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-// 8 characters of the arena at byte offset `off`, big-endian, zeroed from the
-// first NUL on (the reference's get_char_uint64_simple, stringset.hpp:201-231).
-// Two aligned 8-byte loads and a funnel shift; near the arena end the bytes
-// are fetched one by one so nothing past arena[alen-1] is read.
-__device__ __forceinline__ key64 load_key(const uint8_t* arena, uint64_t alen, uint64_t off)
-{
-    const uint64_t base = (uint64_t)(uintptr_t)arena;
-    const uint64_t a = base + off;
-    const uint64_t a8 = a & ~7ull;
-    uint64_t lo, hi;
-    if (a8 + 16 <= base + alen) {
-        lo = *(const uint64_t*)(uintptr_t)a8;
-        hi = *(const uint64_t*)(uintptr_t)(a8 + 8);
-    } else {
-        lo = 0;
-        hi = 0;
-        for (int i = 0; i < 16; ++i) {
-            uint64_t p = a8 + i;
-            if (p >= base && p < base + alen) {
-                uint64_t c = *(const uint8_t*)(uintptr_t)p;
-                if (i < 8) lo |= c << (8 * i);
-                else hi |= c << (8 * (i - 8));
-            }
-        }
-    }
-    const uint32_t sh = (uint32_t)(a & 7) * 8;
-    uint64_t v = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
-    v = __builtin_bswap64(v);
-    // high bit of each byte of t is set iff that byte is non-zero (no carries)
-    const uint64_t t = ((v & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | v;
-    const uint64_t z = ~t & 0x8080808080808080ull;
-    if (z) {
-        const uint32_t p = (uint32_t)__clzll((long long)z);   // 8 * index of first NUL
-        v = p ? (v & (~0ull << (64 - p))) : 0ull;
-    }
-    return v;
-}
-
-// number of leading characters the two keys share (lcpKeyType, .hpp:225-230)
-__device__ __forceinline__ uint32_t lcp_key(key64 a, key64 b)
-{
-    key64 x = a ^ b;
-    return x ? ((uint32_t)__clzll((long long)x) >> 3) : 8u;
-}
-// number of non-NUL characters of a key (lcpKeyDepth, .hpp:232-237)
-__device__ __forceinline__ uint32_t key_depth(key64 a)
-{
-    return a ? 8u - ((uint32_t)__builtin_ctzll(a) >> 3) : 0u;
-}
-
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
-__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
-
-__device__ __forceinline__ key64 readlane64(key64 v, uint32_t l)
-{
-    uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)l);
-    uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)l);
-    return ((key64)hi << 32) | lo;
-}
-
-// lanes holding the same 11-bit bucket id (all 64 lanes must execute this)
-__device__ __forceinline__ uint64_t match_bucket(uint32_t b)
-{
-    uint64_t peers = __ballot(1);
-#pragma unroll
-    for (int i = 0; i < 11; ++i) {
-        const bool bit = (b >> i) & 1u;
-        const uint64_t bal = __ballot(bit);
-        peers &= bit ? bal : ~bal;
-    }
-    return peers;
-}
-
-// Append for every lane with pred: returns its slot (wave-aggregated atomic).
-__device__ __forceinline__ uint32_t wave_append(bool pred, uint32_t* counter)
-{
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return 0xFFFFFFFFu;
-    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
-    uint32_t base = 0;
-    if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, (int)leader, 64);
-    return base + (uint32_t)__popcll(m & lanemask_lt());
-}
-
-__device__ __forceinline__ void flag_error(uint32_t* cnt, uint32_t e) { atomicOr(&cnt[C_ERR], e); }
-
-// an item must describe a non-empty range inside [0, n)
-__device__ __forceinline__ bool item_ok(const Item& it, const Bufs& B)
-{
-    return it.size != 0 && it.begin < B.n && it.size <= B.n - it.begin;
-}
-
-__device__ __forceinline__ void put_item(Item* list, uint32_t cap, uint32_t slot, Item it, uint32_t* cnt)
-{
-    if (slot < cap) list[slot] = it;
-    else atomicOr(&cnt[C_ERR], ERR_CAP);
-}
-
-// append one next-round segment (keys already at the new depth, buffer 0)
-__device__ __forceinline__ void wave_append_segment(bool pred, Item it, const Sink& s)
-{
-    const uint32_t m = it.size;
-    const bool t = pred && m <= kTinyT;
-    const bool sm = pred && m > kTinyT && m <= kSmallT;
-    const bool md = pred && m > kSmallT && m <= kMedT;
-    const bool lg = pred && m > kMedT;
-    uint32_t slot;
-    slot = wave_append(t, &s.cnt[s.c_tiny]);
-    if (t) put_item(s.tiny, s.cap_tiny, slot, it, s.cnt);
-    slot = wave_append(sm, &s.cnt[s.c_small]);
-    if (sm) put_item(s.small, s.cap_small, slot, it, s.cnt);
-    slot = wave_append(md, &s.cnt[s.c_med]);
-    if (md) put_item(s.med, s.cap_med, slot, it, s.cnt);
-    slot = wave_append(lg, &s.cnt[s.c_large]);
-    if (lg) put_item(s.large, s.cap_large, slot, it, s.cnt);
-}
-
-// Exclusive block scan of one uint32 per thread; *total gets the sum.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total, uint32_t* s_tmp)
-{
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    if (lane == 63) s_tmp[w] = x;
-    __syncthreads();
-    uint32_t wpre = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < kWaves; ++i) {
-        uint32_t t = s_tmp[i];
-        wpre += (i < (int)w) ? t : 0u;
-        tot += t;
-    }
-    __syncthreads();
-    *total = tot;
-    return wpre + x - v;
-}
-
-// Bitonic sort of P (power of two) uint64 in LDS by the whole workgroup.
-__device__ void bitonic_sort_u64(key64* s, int P)
-{
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < P; i += kWG) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const key64 a = s[i], b = s[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) { s[i] = b; s[ixj] = a; }
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-// Splitters from 2*nspl random keys of the segment (oversample factor 2,
-// bingmann-sample_sort.hpp:57): sort the sample, take every second key
-// (splitter i = sample 2i+1) and lay the tree out in level order so that
-// descending i = 2i + (key > tree[i]) computes the lower bound.
-__device__ void build_tree(const key64* K, uint32_t beg, uint32_t m, uint64_t seed, int k,
-                           key64* s_samp, key64* s_tree, key64* s_spl)
-{
-    const int nspl = (1 << k) - 1;
-    const int ss = 2 * nspl;
-    int P = 2;
-    while (P < ss) P <<= 1;
-    for (int j = threadIdx.x; j < P; j += kWG) {
-        key64 v = ~0ull;
-        if (j < ss) {
-            const uint64_t h = mix64(seed ^ ((uint64_t)beg << 21) ^ (uint64_t)j);
-            const uint32_t pos = (uint32_t)(((h >> 32) * (uint64_t)m) >> 32);
-            v = K[beg + pos];
-        }
-        s_samp[j] = v;
-    }
-    __syncthreads();
-    bitonic_sort_u64(s_samp, P);
-    for (int t = threadIdx.x + 1; t <= nspl; t += kWG) {
-        const int l = 31 - __clz(t);
-        const int j = t - (1 << l);
-        const int inorder = ((2 * j + 1) << (k - l - 1)) - 1;
-        s_tree[t] = s_samp[2 * inorder + 1];
-    }
-    for (int i = threadIdx.x; i < nspl; i += kWG) s_spl[i] = s_samp[2 * i + 1];
-    __syncthreads();
-}
-
-// bucket of a key: b = 2 * lower_bound + [splitter == key]
-// (ClassifyTreeSimple::find_bkt, bingmann-sample_sortBTC.hpp:45-72)
-__device__ __forceinline__ uint32_t classify_key(key64 key, const key64* s_tree, const key64* s_spl,
-                                                 int k, int nspl)
-{
-    uint32_t i = 1;
-    for (int l = 0; l < k; ++l) i = 2 * i + (key > s_tree[i] ? 1u : 0u);
-    i -= (1u << k);
-    uint32_t b = 2 * i;
-    if ((int)i < nspl && s_spl[i] == key) b += 1;
-    return b;
-}
-
-// Emit the children buckets of a sample-sort step.  start(b) = bstart[b*stride],
-// relative to seg.begin; the last bucket ends at seg.size.  Equal buckets
-// (odd b) are sorted leaves, the rest by size: tiny / small leaves, medium /
-// large segments of the next level (distribute_finished, .hpp:1492-1584).
-__device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart, uint32_t stride,
-                              const Sink& s, uint32_t* s_tmp)
-{
-    constexpr int kPer = (kMaxBkt + kWG - 1) / kWG;   // 8
-    const uint32_t dstbuf = (seg.flags & F_BUFB) ^ F_BUFB;
-    uint32_t c_t = 0, c_s = 0, c_m = 0, c_l = 0, c_r = 0;
-    uint32_t st[kPer], sz[kPer];
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        const int b = threadIdx.x * kPer + j;
-        st[j] = 0;
-        sz[j] = 0;
-        if (b < nbkt) {
-            st[j] = bstart[(uint32_t)b * stride];
-            const uint32_t en = (b + 1 < nbkt) ? bstart[(uint32_t)(b + 1) * stride] : seg.size;
-            sz[j] = en - st[j];
-            const uint32_t m = sz[j];
-            if (m == 0) continue;
-            if ((b & 1) && m > kTinyT) c_r += (m + kRunChunk - 1) / kRunChunk;
-            else if (m <= kTinyT) c_t++;
-            else if (m <= kSmallT) c_s++;
-            else if (m <= kMedT) c_m++;
-            else c_l++;
-        }
-    }
-    uint32_t tt, ts, tm, tl, tr;
-    uint32_t o_t = block_excl_scan(c_t, &tt, s_tmp);
-    uint32_t o_s = block_excl_scan(c_s, &ts, s_tmp);
-    uint32_t o_m = block_excl_scan(c_m, &tm, s_tmp);
-    uint32_t o_l = block_excl_scan(c_l, &tl, s_tmp);
-    uint32_t o_r = block_excl_scan(c_r, &tr, s_tmp);
-    if (threadIdx.x == 0) {
-        s_tmp[8] = tt ? atomicAdd(&s.cnt[s.c_tiny], tt) : 0;
-        s_tmp[9] = ts ? atomicAdd(&s.cnt[s.c_small], ts) : 0;
-        s_tmp[10] = tm ? atomicAdd(&s.cnt[s.c_med], tm) : 0;
-        s_tmp[11] = tl ? atomicAdd(&s.cnt[s.c_large], tl) : 0;
-        s_tmp[12] = tr ? atomicAdd(&s.cnt[s.c_run], tr) : 0;
-    }
-    __syncthreads();
-    o_t += s_tmp[8];
-    o_s += s_tmp[9];
-    o_m += s_tmp[10];
-    o_l += s_tmp[11];
-    o_r += s_tmp[12];
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) {
-        const int b = threadIdx.x * kPer + j;
-        const uint32_t m = sz[j];
-        if (b >= nbkt || m == 0) continue;
-        Item it;
-        it.begin = seg.begin + st[j];
-        it.size = m;
-        it.rbeg = seg.rbeg;
-        it.flags = dstbuf;
-        if ((b & 1) && m > kTinyT) {
-            const uint32_t nch = (m + kRunChunk - 1) / kRunChunk;
-            for (uint32_t c = 0; c < nch; ++c) {
-                Item r = it;
-                r.flags = dstbuf | F_SORTED | (c << 8);
-                put_item(s.run, s.cap_run, o_r++, r, s.cnt);
-            }
-        } else if (m <= kTinyT) {
-            put_item(s.tiny, s.cap_tiny, o_t++, it, s.cnt);
-        } else if (m <= kSmallT) {
-            put_item(s.small, s.cap_small, o_s++, it, s.cnt);
-        } else if (m <= kMedT) {
-            put_item(s.med, s.cap_med, o_m++, it, s.cnt);
-        } else {
-            put_item(s.large, s.cap_large, o_l++, it, s.cnt);
-        }
-    }
-}
 
 // ===========================================================================
 // kernels
@@ -356,52 +64,71 @@ __global__ void k_reset(uint32_t* cnt, int a, int b, int c, int d, int e)
     if (t == 4 && e >= 0) cnt[e] = 0;
 }
 
-// ---- multi-workgroup sample-sort step (segments > kMedT) -------------------
+// ---- multi-workgroup sample-sort step (segments > kBlkT) -------------------
 
-// Splitter tree of one large segment -> tree[0..1023] level order,
-// tree[1024..2047] in order.
-__global__ __launch_bounds__(kWG) void k_tree(Item seg, Bufs B, uint64_t seed, key64* tree)
+// Splitter trees of the large segments of a level, one workgroup each.
+// D.oversample * nspl random keys are sorted; splitter i is sample
+// oversample/2 + oversample*i (the reference's choice for oversample factor 2,
+// bingmann-sample_sort_tree_builder.hpp:44-58) and the tree is stored in
+// level order: trees[seg*1024 + t], t = 1..nspl.
+__global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, uint64_t seed, key64* trees)
 {
-    __shared__ key64 s_samp[2048];
-    __shared__ key64 s_tree[1024];
-    __shared__ key64 s_spl[1024];
-    const key64* K = (seg.flags & F_BUFB) ? B.key1 : B.key0;
-    build_tree(K, seg.begin, seg.size, seed, kMaxTreebits, s_samp, s_tree, s_spl);
-    for (int i = threadIdx.x; i < 1024; i += kWG) {
-        tree[i] = s_tree[i];
-        tree[1024 + i] = s_spl[i];
+    __shared__ key64 s_samp[kMaxSamples];
+    const LargeDesc D = desc[blockIdx.x];
+    const Item seg = D.seg;
+    if (!item_ok(seg, B)) {
+        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
+        return;
     }
+    const key64* K = (seg.flags & F_BUFB) ? B.key1 : B.key0;
+    const int k = (int)D.k, nspl = (1 << k) - 1, a = (int)D.oversample;
+    const int ss = a * nspl;
+    int P = 2;
+    while (P < ss) P <<= 1;
+    for (int j = threadIdx.x; j < P; j += kBigWG) {
+        key64 v = ~0ull;
+        if (j < ss) {
+            const uint64_t h = mix64(seed ^ ((uint64_t)seg.begin << 21) ^ (uint64_t)j);
+            v = K[seg.begin + (uint32_t)(((h >> 32) * (uint64_t)seg.size) >> 32)];
+        }
+        s_samp[j] = v;
+    }
+    __syncthreads();
+    bitonic_sort_u64<kBigWG>(s_samp, P);
+    key64* tree = trees + (uint64_t)blockIdx.x * 1024;
+    for (int t = threadIdx.x + 1; t <= nspl; t += kBigWG) tree[t] = s_samp[a / 2 + a * inorder_of(t, k)];
 }
 
 // Classify one tile of kTile keys.  Each wave ranks its own chunk of
 // kWaveChunk keys stably (ballot matching, ranks follow input order) and the
-// chunk counts go to hist[(b * T + t) * kWaves + w]: bucket-major, then tile,
-// then wave -- the order of the reference's prefix sum (.hpp:1437-1460) with
-// wave chunks as the parts -- so an exclusive scan of hist gives every chunk's
-// base offset per bucket.  bktrank gets bucket and rank inside the chunk.
-__global__ __launch_bounds__(kWG) void k_count(Item seg, Bufs B, const key64* tree, uint32_t T,
-                                                uint32_t* hist)
+// chunk counts go to hist[hist_base + (b * T + t) * kWaves + w]: bucket-major,
+// then tile, then wave -- the order of the reference's prefix sum
+// (.hpp:1437-1460) with wave chunks as the parts -- so an exclusive scan of
+// hist gives every chunk's base offset per bucket.  bktrank gets bucket and
+// rank inside the chunk.
+__global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const TileRef* tiles, Bufs B,
+                                                const key64* trees, uint32_t* hist)
 {
     __shared__ key64 s_tree[1024];
-    __shared__ key64 s_spl[1024];
     __shared__ uint32_t s_hist[kWaves][2048];
-    const int k = kMaxTreebits, nspl = kMaxSpl, nbkt = kMaxBkt;
-    for (int i = threadIdx.x; i < 1024; i += kWG) {
-        s_tree[i] = tree[i];
-        s_spl[i] = tree[1024 + i];
-    }
+    const TileRef tr = tiles[blockIdx.x];
+    const LargeDesc D = desc[tr.seg];
+    const Item seg = D.seg;
+    const int k = (int)D.k, nbkt = (2 << k) - 1;
+    const key64* tree = trees + (uint64_t)tr.seg * 1024;
+    for (int i = threadIdx.x; i < 1024; i += kWG) s_tree[i] = tree[i];
     for (int i = threadIdx.x; i < kWaves * 2048; i += kWG) (&s_hist[0][0])[i] = 0;
     __syncthreads();
     const key64* K = (seg.flags & F_BUFB) ? B.key1 : B.key0;
-    const uint32_t t = blockIdx.x;
-    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t t = tr.t;
+    const uint32_t w = wave_id(), lane = lane_id();
     const uint32_t cb = t * kTile + w * kWaveChunk;
-    const uint32_t ce = item_ok(seg, B) ? min(seg.size, cb + kWaveChunk) : 0u;
+    const uint32_t ce = (item_ok(seg, B) && t < D.T) ? min(seg.size, cb + kWaveChunk) : 0u;
     for (uint32_t g = cb; g < ce; g += 64) {
         const uint32_t p = g + lane;
         const bool act = p < ce;
         uint32_t b = kNoBucket;
-        if (act) b = classify_key(K[seg.begin + p], s_tree, s_spl, k, nspl);
+        if (act) b = classify_key(K[seg.begin + p], s_tree, k);
         const uint64_t peers = match_bucket(b);
         const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
         if (act) {
@@ -411,9 +138,10 @@ __global__ __launch_bounds__(kWG) void k_count(Item seg, Bufs B, const key64* tr
         }
     }
     __syncthreads();
+    uint32_t* h = hist + D.hist_base;
     for (int i = threadIdx.x; i < nbkt * kWaves; i += kWG) {
         const uint32_t b = (uint32_t)i / kWaves, ww = (uint32_t)i % kWaves;
-        hist[((uint32_t)b * T + t) * kWaves + ww] = s_hist[ww][b];
+        h[((uint32_t)b * D.T + t) * kWaves + ww] = s_hist[ww][b];
     }
 }
 
@@ -432,7 +160,7 @@ __global__ __launch_bounds__(kWG) void k_scan_reduce(const uint32_t* in, uint32_
         if (i < len) sum += in[i];
     }
     uint32_t tot;
-    block_excl_scan(sum, &tot, s_tmp);
+    block_excl_scan<kWaves>(sum, &tot, s_tmp);
     if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
@@ -445,7 +173,7 @@ __global__ __launch_bounds__(kWG) void k_scan_top(uint32_t* part, uint32_t npart
         const uint32_t i = base + threadIdx.x;
         const uint32_t v = i < nparts ? part[i] : 0u;
         uint32_t tot;
-        const uint32_t ex = block_excl_scan(v, &tot, s_tmp);
+        const uint32_t ex = block_excl_scan<kWaves>(v, &tot, s_tmp);
         if (i < nparts) part[i] = carry + ex;
         carry += tot;
     }
@@ -465,7 +193,7 @@ __global__ __launch_bounds__(kWG) void k_scan_down(uint32_t* data, uint32_t len,
         sum += v[j];
     }
     uint32_t tot;
-    uint32_t run = block_excl_scan(sum, &tot, s_tmp) + part[blockIdx.x];
+    uint32_t run = block_excl_scan<kWaves>(sum, &tot, s_tmp) + part[blockIdx.x];
 #pragma unroll
     for (int j = 0; j < kScanPer; ++j) {
         const uint32_t i = base + j;
@@ -474,143 +202,186 @@ __global__ __launch_bounds__(kWG) void k_scan_down(uint32_t* data, uint32_t len,
     }
 }
 
-// Move each key/index of a tile to seg.begin + base(bucket, tile, wave) + rank
-// (distribute, .hpp:1464-1490, but stable: ranks follow input order).
-__global__ __launch_bounds__(kWG) void k_scatter(Item seg, Bufs B, uint32_t T, const uint32_t* hscan)
+// LDS layout of k_scatter (dynamic shared memory)
+constexpr uint32_t kScatKeyOff = 0;
+constexpr uint32_t kScatIdxOff = kScatKeyOff + kWaveChunk * 8;
+constexpr uint32_t kScatBktOff = kScatIdxOff + kWaveChunk * 4;
+constexpr uint32_t kScatStartOff = kScatBktOff + kWaveChunk * 2;
+constexpr uint32_t kScatBaseOff = kScatStartOff + 2048 * 4;
+constexpr uint32_t kScatTmpOff = kScatBaseOff + 2048 * 4;
+constexpr uint32_t kScatLds = kScatTmpOff + 64 * 4;
+
+// Move one wave chunk (kWaveChunk keys) to its buckets (distribute,
+// .hpp:1464-1490, but stable).  The chunk is first reordered by bucket in
+// LDS, then written out position by position, so consecutive lanes store
+// consecutive addresses inside each bucket: full cache lines instead of the
+// scattered 8-byte stores of a direct scatter.
+__global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* desc, const TileRef* tiles, Bufs B,
+                                                     const uint32_t* hscan)
 {
-    __shared__ uint32_t s_off[kWaves][2048];
-    const uint32_t t = blockIdx.x;
-    for (int i = threadIdx.x; i < kMaxBkt * kWaves; i += kWG) {
-        const uint32_t b = (uint32_t)i / kWaves, ww = (uint32_t)i % kWaves;
-        s_off[ww][b] = hscan[((uint32_t)b * T + t) * kWaves + ww];
-    }
-    __syncthreads();
-    const int src = seg.flags & F_BUFB;
-    const key64* K = src ? B.key1 : B.key0;
-    const uint32_t* I = src ? B.idx1 : B.idx0;
-    key64* KD = src ? B.key0 : B.key1;
-    uint32_t* ID = src ? B.idx0 : B.idx1;
-    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    extern __shared__ __align__(16) unsigned char smem[];
+    key64* s_key = reinterpret_cast<key64*>(smem + kScatKeyOff);
+    uint32_t* s_idx = reinterpret_cast<uint32_t*>(smem + kScatIdxOff);
+    uint16_t* s_bkt = reinterpret_cast<uint16_t*>(smem + kScatBktOff);
+    uint32_t* s_start = reinterpret_cast<uint32_t*>(smem + kScatStartOff);
+    uint32_t* s_base = reinterpret_cast<uint32_t*>(smem + kScatBaseOff);
+    uint32_t* s_tmp = reinterpret_cast<uint32_t*>(smem + kScatTmpOff);
+
+    const TileRef tr = tiles[blockIdx.x / kWaves];
+    const uint32_t w = blockIdx.x % kWaves;
+    const LargeDesc D = desc[tr.seg];
+    const Item seg = D.seg;
+    const int k = (int)D.k, nbkt = (2 << k) - 1;
+    const uint32_t t = tr.t;
     const uint32_t cb = t * kTile + w * kWaveChunk;
-    const uint32_t ce = item_ok(seg, B) ? min(seg.size, cb + kWaveChunk) : 0u;
-    for (uint32_t p = cb + lane; p < ce; p += 64) {
-        const uint32_t q = seg.begin + p;
-        const uint32_t br = B.bktrank[q];
-        const uint32_t b = br >> kRankBits;
-        const uint32_t rel = b < (uint32_t)kMaxBkt ? s_off[w][b] + (br & kRankMask) : 0xFFFFFFFFu;
-        if (rel >= seg.size) { flag_error(B.cnt, ERR_BOUNDS); continue; }
-        KD[seg.begin + rel] = K[q];
-        ID[seg.begin + rel] = I[q];
+    if (!item_ok(seg, B) || t >= D.T || cb >= seg.size) return;
+    const uint32_t m = min(seg.size - cb, (uint32_t)kWaveChunk);
+    const uint32_t* h = hscan + D.hist_base;
+    const uint32_t segbase = h[0];
+    // counts and global bases of this chunk per bucket; entry e's count is
+    // scan[e + 1] - scan[e] (the entry after the segment's last one holds
+    // segbase + size)
+    const uint32_t last = ((uint32_t)(nbkt - 1) * D.T + (D.T - 1)) * kWaves + (kWaves - 1);
+    uint32_t c0 = 0, c1 = 0;
+    {
+        const int b0 = threadIdx.x, b1 = threadIdx.x + kBigWG;
+        if (b0 < nbkt) {
+            const uint32_t e = ((uint32_t)b0 * D.T + t) * kWaves + w;
+            const uint32_t v = h[e];
+            c0 = (e == last ? segbase + seg.size : h[e + 1]) - v;
+            s_base[b0] = v - segbase;
+        }
+        if (b1 < nbkt) {
+            const uint32_t e = ((uint32_t)b1 * D.T + t) * kWaves + w;
+            const uint32_t v = h[e];
+            c1 = (e == last ? segbase + seg.size : h[e + 1]) - v;
+            s_base[b1] = v - segbase;
+        }
     }
-}
-
-__global__ __launch_bounds__(kWG) void k_children(Item seg, uint32_t T, const uint32_t* hscan, Sink s)
-{
-    __shared__ uint32_t s_tmp[16];
-    emit_children(seg, kMaxBkt, hscan, T * kWaves, s, s_tmp);
-}
-
-// ---- one-workgroup sample-sort step (kSmallT < size <= kMedT) -----------------
-
-__global__ __launch_bounds__(kWG) void k_medium(const Item* items, const uint32_t* cnt_items, Bufs B,
-                                                 Sink s, uint64_t seed, int tb_shift)
-{
-    __shared__ key64 s_tree[1024];
-    __shared__ key64 s_spl[1024];
-    __shared__ uint32_t s_hist[kWaves][2048];
-    __shared__ uint32_t s_tmp[16];
-    if (blockIdx.x >= *cnt_items) return;
-    const Item seg = items[blockIdx.x];
-    if (!item_ok(seg, B) || seg.size > kMedT) {
+    // local bucket starts: buckets 0..1023 first, then 1024..2046
+    uint32_t tot0, tot1;
+    const uint32_t ex0 = block_excl_scan<kBigWaves>(c0, &tot0, s_tmp);
+    const uint32_t ex1 = block_excl_scan<kBigWaves>(c1, &tot1, s_tmp);
+    if (threadIdx.x < (uint32_t)nbkt) s_start[threadIdx.x] = ex0;
+    if (threadIdx.x + kBigWG < (uint32_t)nbkt) s_start[threadIdx.x + kBigWG] = tot0 + ex1;
+    __syncthreads();
+    if (tot0 + tot1 != m) {   // the histogram must account for every key of the chunk
         if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
-    const uint32_t m = seg.size, beg = seg.begin;
     const int src = seg.flags & F_BUFB;
     const key64* K = src ? B.key1 : B.key0;
     const uint32_t* I = src ? B.idx1 : B.idx0;
     key64* KD = src ? B.key0 : B.key1;
     uint32_t* ID = src ? B.idx0 : B.idx1;
-    const int k = treebits_for(m, tb_shift);
-    const int nspl = (1 << k) - 1, nbkt = 2 * nspl + 1;
-
-    build_tree(K, beg, m, seed, k, reinterpret_cast<key64*>(&s_hist[0][0]), s_tree, s_spl);
-    for (int i = threadIdx.x; i < kWaves * 2048; i += kWG) (&s_hist[0][0])[i] = 0;
-    __syncthreads();
-
-    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
-    const uint32_t chunk = (((m + kWaves - 1) / kWaves) + 63u) & ~63u;
-    const uint32_t cb = w * chunk, ce = min(m, cb + chunk);
-    for (uint32_t g = cb; g < cb + chunk; g += 64) {
-        const uint32_t p = g + lane;
-        const bool act = p < ce;
-        uint32_t b = kNoBucket;
-        if (act) b = classify_key(K[beg + p], s_tree, s_spl, k, nspl);
-        const uint64_t peers = match_bucket(b);
-        const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
-        if (act) {
-            const uint32_t old = s_hist[w][b];
-            if (r == 0) s_hist[w][b] = old + (uint32_t)__popcll(peers);
-            B.bktrank[beg + p] = (b << kRankBits) | (old + r);
-        }
+    for (uint32_t p = threadIdx.x; p < m; p += kBigWG) {
+        const uint32_t q = seg.begin + cb + p;
+        const uint32_t br = B.bktrank[q];
+        const uint32_t b = br >> kRankBits;
+        const uint32_t l = b < (uint32_t)nbkt ? s_start[b] + (br & kRankMask) : 0xFFFFFFFFu;
+        if (l >= m) { flag_error(B.cnt, ERR_BOUNDS); continue; }
+        s_key[l] = K[q];
+        s_idx[l] = I[q];
+        s_bkt[l] = (uint16_t)b;
     }
     __syncthreads();
-    // bucket offsets (exclusive over buckets) and per-wave bases
-    constexpr int kPer = 2048 / kWG;   // 8 buckets per thread
-    uint32_t tot[kPer];
-    uint32_t mysum = 0;
+    for (uint32_t j = threadIdx.x; j < m; j += kBigWG) {
+        const uint32_t b = s_bkt[j];
+        const uint32_t rel = s_base[b] + (j - s_start[b]);
+        if (rel >= seg.size) { flag_error(B.cnt, ERR_BOUNDS); continue; }
+        KD[seg.begin + rel] = s_key[j];
+        ID[seg.begin + rel] = s_idx[j];
+    }
+}
+
+// Emit the children buckets of a multi-workgroup step.  start(b) =
+// bstart[b*stride] - base relative to seg.begin; the last bucket ends at
+// seg.size.  Tiny buckets become tiny leaves, other equal buckets (odd b)
+// equal-key leaves in chunks, other less-than buckets blocks or large
+// segments of the next level (distribute_finished, .hpp:1492-1584).
+__device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart, uint32_t stride, uint32_t base,
+                              const Sink& s, uint32_t* s_tmp)
+{
+    constexpr int kPer = (kMaxBkt + kWG - 1) / kWG;   // 8
+    const uint32_t dstbuf = (seg.flags & F_BUFB) ^ F_BUFB;
+    uint32_t c_t = 0, c_b = 0, c_l = 0, c_r = 0;
+    uint32_t st[kPer], sz[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int b = threadIdx.x * kPer + j;
-        uint32_t t = 0;
+        st[j] = 0;
+        sz[j] = 0;
         if (b < nbkt) {
-#pragma unroll
-            for (int ww = 0; ww < kWaves; ++ww) t += s_hist[ww][b];
+            st[j] = bstart[(uint32_t)b * stride] - base;
+            const uint32_t en = (b + 1 < nbkt) ? bstart[(uint32_t)(b + 1) * stride] - base : seg.size;
+            sz[j] = en - st[j];
+            const uint32_t m = sz[j];
+            if (m == 0) continue;
+            if (m <= kTinyT) c_t++;
+            else if (b & 1) c_r += (m + kRunChunk - 1) / kRunChunk;
+            else if (m <= kBlkT) c_b++;
+            else c_l++;
         }
-        tot[j] = t;
-        mysum += t;
     }
-    uint32_t all;
-    uint32_t off = block_excl_scan(mysum, &all, s_tmp);
+    uint32_t tt, tb, tl, tr;
+    uint32_t o_t = block_excl_scan<kWaves>(c_t, &tt, s_tmp);
+    uint32_t o_b = block_excl_scan<kWaves>(c_b, &tb, s_tmp);
+    uint32_t o_l = block_excl_scan<kWaves>(c_l, &tl, s_tmp);
+    uint32_t o_r = block_excl_scan<kWaves>(c_r, &tr, s_tmp);
+    if (threadIdx.x == 0) {
+        s_tmp[8] = tt ? atomicAdd(&s.cnt[s.c_tiny], tt) : 0;
+        s_tmp[9] = tb ? atomicAdd(&s.cnt[s.c_block], tb) : 0;
+        s_tmp[10] = tl ? atomicAdd(&s.cnt[s.c_large], tl) : 0;
+        s_tmp[11] = tr ? atomicAdd(&s.cnt[s.c_run], tr) : 0;
+    }
+    __syncthreads();
+    o_t += s_tmp[8];
+    o_b += s_tmp[9];
+    o_l += s_tmp[10];
+    o_r += s_tmp[11];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int b = threadIdx.x * kPer + j;
-        if (b < nbkt) {
-            uint32_t run = off;
-#pragma unroll
-            for (int ww = 0; ww < kWaves; ++ww) {
-                const uint32_t c = s_hist[ww][b];
-                s_hist[ww][b] = run;
-                run += c;
+        const uint32_t m = sz[j];
+        if (b >= nbkt || m == 0) continue;
+        Item it;
+        it.begin = seg.begin + st[j];
+        it.size = m;
+        it.rbeg = seg.rbeg;
+        it.flags = dstbuf;
+        if (m <= kTinyT) {
+            put_item(s.tiny, s.cap_tiny, o_t++, it, s.cnt);
+        } else if (b & 1) {
+            const uint32_t nch = (m + kRunChunk - 1) / kRunChunk;
+            for (uint32_t c = 0; c < nch; ++c) {
+                Item r = it;
+                r.flags = dstbuf | F_SORTED | (c << 8);
+                put_item(s.run, s.cap_run, o_r++, r, s.cnt);
             }
-        }
-        off += tot[j];
-    }
-    __syncthreads();
-    for (uint32_t g = cb; g < cb + chunk; g += 64) {
-        const uint32_t p = g + lane;
-        if (p < ce) {
-            const uint32_t br = B.bktrank[beg + p];
-            const uint32_t b = br >> kRankBits;
-            const uint32_t rel = b < (uint32_t)nbkt ? s_hist[w][b] + (br & kRankMask) : 0xFFFFFFFFu;
-            if (rel >= m) { flag_error(B.cnt, ERR_BOUNDS); continue; }
-            KD[beg + rel] = K[beg + p];
-            ID[beg + rel] = I[beg + p];
+        } else if (m <= kBlkT) {
+            put_item(s.block, s.cap_block, o_b++, it, s.cnt);
+        } else {
+            put_item(s.large, s.cap_large, o_l++, it, s.cnt);
         }
     }
-    if (threadIdx.x == 0) atomicAdd(&s.cnt[C_EL_MED], m);
-    __syncthreads();
-    emit_children(seg, nbkt, &s_hist[0][0], 1, s, s_tmp);
+}
+
+__global__ __launch_bounds__(kWG) void k_children(const LargeDesc* desc, const uint32_t* hscan, Sink s)
+{
+    __shared__ uint32_t s_tmp[16];
+    const LargeDesc D = desc[blockIdx.x];
+    const uint32_t* h = hscan + D.hist_base;
+    emit_children(D.seg, (2 << D.k) - 1, h, D.T * kWaves, h[0], s, s_tmp);
 }
 
 // ---- leaves --------------------------------------------------------------------
 
-// Finish one sorted position of a leaf at depth d (shared by all leaf kernels):
-// LCP with the predecessor inside the leaf when keys differ, and the fate of
-// the run of equal keys the position belongs to.
-__device__ __forceinline__ void finish_position(const Bufs& B, uint32_t pos, uint32_t id, key64 k,
-                                                bool has_prev, key64 pk, bool run_single,
-                                                uint32_t depth)
+// Finish one sorted position at depth d: LCP with the predecessor when the
+// keys differ; inside a run of equal keys the LCP is known when the key holds
+// a NUL (fill_lcp with lcpKeyDepth, .hpp:1546-1547), otherwise the run moves
+// on to depth d+8 and the string's next key is gathered for round r+1.
+__device__ __forceinline__ void finish_position(const Bufs& B, uint32_t pos, uint32_t id, key64 k, bool has_prev,
+                                                key64 pk, bool run_single, uint32_t depth)
 {
     const bool eqp = has_prev && pk == k;
     if (has_prev && !eqp && B.lcp) B.lcp[pos] = depth + lcp_key(pk, k);
@@ -625,13 +396,14 @@ __device__ __forceinline__ void finish_position(const Bufs& B, uint32_t pos, uin
     }
 }
 
-// One wave per leaf of at most 64 strings: rank sort with broadcast compares.
+// One wave per leaf of at most 64 strings: rank sort with broadcast compares
+// (the position each string takes in an insertion sort, computed directly).
 __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint32_t* cnt_items, Bufs B,
                                                    Sink nxt, uint32_t depth)
 {
     __shared__ key64 s_k[kWaves][64];
     __shared__ uint32_t s_i[kWaves][64];
-    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t w = wave_id(), lane = lane_id();
     const uint32_t it = blockIdx.x * kWaves + w;
     const bool valid = it < *cnt_items;
     Item L = { 0u, 0u, 0u, 0u };
@@ -652,124 +424,186 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
         key = src ? B.key1[L.begin + lane] : B.key0[L.begin + lane];
         id = src ? B.idx1[L.begin + lane] : B.idx0[L.begin + lane];
     }
+    if (act) s_k[w][lane] = key;
+    __syncthreads();
     uint32_t rank = 0;
     for (uint32_t i = 0; i < m; ++i) {
-        const key64 ki = readlane64(key, i);
+        const key64 ki = s_k[w][i];
         rank += (ki < key || (ki == key && i < lane)) ? 1u : 0u;
     }
+    __syncthreads();
     if (act) {
         s_k[w][rank] = key;
         s_i[w][rank] = id;
     }
     __syncthreads();
-    key64 k = 0, pk = 0;
+    key64 k = 0, pk = 0, nk = 0;
     if (act) {
         k = s_k[w][lane];
         id = s_i[w][lane];
         if (lane > 0) pk = s_k[w][lane - 1];
+        if (lane + 1 < m) nk = s_k[w][lane + 1];
     }
     const bool eqp = act && lane > 0 && pk == k;
-    const uint64_t starts = __ballot(act && !eqp);
-    const uint64_t le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-    const uint64_t before = starts & le;
-    const uint32_t rb = before ? 63u - (uint32_t)__clzll((long long)before) : 0u;
-    const uint64_t after = starts & ~le;
-    const uint32_t re = after ? (uint32_t)__ffsll((long long)after) - 1u : m;
-    const uint32_t runlen = re - rb;
+    const bool eqn = act && lane + 1 < m && nk == k;
     if (act) {
         const uint32_t pos = L.begin + lane;
-        finish_position(B, pos, id, k, lane > 0, pk, runlen == 1, depth);
+        finish_position(B, pos, id, k, lane > 0, pk, !eqp && !eqn, depth);
         if (lane == 0 || lane == m - 1) B.bkey[pos] = k;
     }
-    const bool nr = act && lane == rb && runlen >= 2 && (k & 0xFFull) != 0;
-    Item ni = { L.begin + rb, runlen, L.begin + rb, 0u };
+    // a run of equal keys without a NUL continues at depth + 8
+    const bool nr = act && !eqp && eqn && (k & 0xFFull) != 0;
+    uint32_t len = 0;
+    if (nr) {
+        len = 1;
+        while (lane + len < m && s_k[w][lane + len] == k) ++len;
+    }
+    Item ni = { L.begin + lane, len, L.begin + lane, 0u };
     wave_append_segment(nr, ni, nxt);
-    if (lane == 0 && valid) atomicAdd(&nxt.cnt[C_EL_TINY], m);
 }
 
-// One workgroup per leaf of 65..1024 strings: bitonic sort of (key, position).
-__global__ __launch_bounds__(kWG) void k_leaf_small(const Item* items, const uint32_t* cnt_items, Bufs B,
-                                                    Sink nxt, uint32_t depth)
+// (key, position) order: positions are distinct, so the order is total and
+// equal keys keep their input order (stable).
+__device__ __forceinline__ bool kp_less(key64 ka, uint32_t pa, key64 kb, uint32_t pb)
 {
-    __shared__ key64 s_k[kSmallT];
-    __shared__ uint32_t s_p[kSmallT];
-    __shared__ uint32_t s_rb[kSmallT];
-    __shared__ uint32_t s_len[kSmallT];
-    if (blockIdx.x >= *cnt_items) return;
-    const Item L = items[blockIdx.x];
-    if (!item_ok(L, B) || L.size > kSmallT) {
-        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
-        return;
+    return ka < kb || (ka == kb && pa < pb);
+}
+
+__device__ __forceinline__ key64 shfl_xor64(key64 v, int m)
+{
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+    return ((key64)hi << 32) | lo;
+}
+
+// Bitonic sort of P = kWG*E (key, position) pairs held in registers: element
+// i = e*kWG + threadIdx.x lives in k[e], p[e] of that thread.  Partners at
+// distance j >= kWG are in the same thread, 64 <= j < kWG in another wave
+// (exchanged through LDS), j < 64 in the same wave (lane shuffles).
+template <int E>
+__device__ __forceinline__ void block_bitonic(key64 (&k)[E], uint32_t (&p)[E], key64* s_k, uint32_t* s_p)
+{
+    constexpr int P = kWG * E;
+    const uint32_t tid = threadIdx.x;
+#pragma unroll 1
+    for (int kk = 2; kk <= P; kk <<= 1) {
+#pragma unroll 1
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            if (j >= kWG) {
+                const int d = j / kWG;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int f = e ^ d;
+                    if (f > e) {
+                        const uint32_t i = (uint32_t)e * kWG + tid;
+                        const bool up = (i & (uint32_t)kk) == 0;
+                        if (kp_less(k[f], p[f], k[e], p[e]) == up) {
+                            const key64 tk = k[e]; k[e] = k[f]; k[f] = tk;
+                            const uint32_t tp = p[e]; p[e] = p[f]; p[f] = tp;
+                        }
+                    }
+                }
+            } else if (j >= 64) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    s_k[e * kWG + tid] = k[e];
+                    s_p[e * kWG + tid] = p[e];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const uint32_t i = (uint32_t)e * kWG + tid;
+                    const key64 ok = s_k[i ^ (uint32_t)j];
+                    const uint32_t op = s_p[i ^ (uint32_t)j];
+                    const bool keep_min = ((i & (uint32_t)j) == 0) == ((i & (uint32_t)kk) == 0);
+                    const bool other_less = kp_less(ok, op, k[e], p[e]);
+                    if (other_less == keep_min) { k[e] = ok; p[e] = op; }
+                }
+                __syncthreads();
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const uint32_t i = (uint32_t)e * kWG + tid;
+                    const key64 ok = shfl_xor64(k[e], j);
+                    const uint32_t op = (uint32_t)__shfl_xor((int)p[e], j, 64);
+                    const bool keep_min = ((i & (uint32_t)j) == 0) == ((i & (uint32_t)kk) == 0);
+                    const bool other_less = kp_less(ok, op, k[e], p[e]);
+                    if (other_less == keep_min) { k[e] = ok; p[e] = op; }
+                }
+            }
+        }
     }
-    const uint32_t m = L.size;
+}
+
+// Sort a block of m <= kWG*E strings and finish every position at this depth.
+template <int E>
+__device__ __forceinline__ void block_sort_finish(const Item& L, const Bufs& B, const Sink& nxt, uint32_t depth,
+                                                  key64* s_k, uint32_t* s_p, uint32_t* s_id)
+{
+    const uint32_t m = L.size, beg = L.begin, tid = threadIdx.x;
     const int src = L.flags & F_BUFB;
     const key64* K = src ? B.key1 : B.key0;
     const uint32_t* I = src ? B.idx1 : B.idx0;
-    uint32_t P = 2;
-    while (P < m) P <<= 1;
-    for (uint32_t j = threadIdx.x; j < P; j += kWG) {
-        s_k[j] = j < m ? K[L.begin + j] : ~0ull;
-        s_p[j] = j;
+    key64 k[E];
+    uint32_t p[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = (uint32_t)e * kWG + tid;
+        k[e] = i < m ? K[beg + i] : ~0ull;   // padding sorts last (positions >= m)
+        p[e] = i;
+        if (i < m) s_id[i] = I[beg + i];
     }
+    block_bitonic<E>(k, p, s_k, s_p);
+#pragma unroll
+    for (int e = 0; e < E; ++e) s_k[e * kWG + tid] = k[e];
     __syncthreads();
-    for (uint32_t kk = 2; kk <= P; kk <<= 1) {
-        for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
-            for (uint32_t i = threadIdx.x; i < P; i += kWG) {
-                const uint32_t ixj = i ^ jj;
-                if (ixj > i) {
-                    const key64 a = s_k[i], b = s_k[ixj];
-                    const uint32_t pa = s_p[i], pb = s_p[ixj];
-                    const bool gt = a > b || (a == b && pa > pb);
-                    const bool up = (i & kk) == 0;
-                    if (gt == up) {
-                        s_k[i] = b; s_k[ixj] = a;
-                        s_p[i] = pb; s_p[ixj] = pa;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    // input indices in sorted order (read before any position is rewritten)
-    for (uint32_t j = threadIdx.x; j < m; j += kWG) {
-        s_len[j] = I[L.begin + s_p[j]];
-        s_rb[j] = (j == 0 || s_k[j - 1] != s_k[j]) ? j : 0u;
-    }
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < m; j += kWG) s_p[j] = s_len[j];   // s_p := ids
-    __syncthreads();
-    // run begins: inclusive max scan
-    for (uint32_t d = 1; d < m; d <<= 1) {
-        uint32_t v[kSmallT / kWG];
-        int c = 0;
-        for (uint32_t j = threadIdx.x; j < m; j += kWG, ++c) v[c] = (j >= d) ? max(s_rb[j], s_rb[j - d]) : s_rb[j];
-        __syncthreads();
-        c = 0;
-        for (uint32_t j = threadIdx.x; j < m; j += kWG, ++c) s_rb[j] = v[c];
-        __syncthreads();
-    }
-    for (uint32_t j = threadIdx.x; j < m; j += kWG)
-        if (j == m - 1 || s_k[j + 1] != s_k[j]) s_len[s_rb[j]] = j + 1 - s_rb[j];
-    __syncthreads();
-    for (uint32_t j0 = 0; j0 < m; j0 += kWG) {   // uniform loop for the wave appends
-        const uint32_t j = j0 + threadIdx.x;
-        const bool act = j < m;
-        bool nr = false;
-        Item ni = { 0u, 0u, 0u, 0u };
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = (uint32_t)e * kWG + tid;   // sorted position
+        const bool act = i < m;
+        bool eqp = false, eqn = false;
+        key64 pk = 0;
         if (act) {
-            const key64 k = s_k[j];
-            const uint32_t rb = s_rb[j], runlen = s_len[rb];
-            const uint32_t pos = L.begin + j;
-            finish_position(B, pos, s_p[j], k, j > 0, j > 0 ? s_k[j - 1] : 0ull, runlen == 1, depth);
-            if (j == 0 || j == m - 1) B.bkey[pos] = k;
-            nr = j == rb && runlen >= 2 && (k & 0xFFull) != 0;
-            ni.begin = L.begin + rb;
-            ni.size = runlen;
-            ni.rbeg = L.begin + rb;
+            if (i > 0) pk = s_k[i - 1];
+            eqp = i > 0 && pk == k[e];
+            eqn = i + 1 < m && s_k[i + 1] == k[e];
+            finish_position(B, beg + i, s_id[p[e]], k[e], i > 0, pk, !eqp && !eqn, depth);
+            if (i == 0 || i == m - 1) B.bkey[beg + i] = k[e];
         }
+        // a run of equal keys without a NUL continues at depth + 8
+        const bool nr = act && !eqp && eqn && (k[e] & 0xFFull) != 0;
+        uint32_t rl = 0;
+        if (nr) {
+            rl = 1;
+            while (i + rl < m && s_k[i + rl] == k[e]) ++rl;
+        }
+        Item ni = { beg + i, rl, beg + i, 0u };
         wave_append_segment(nr, ni, nxt);
     }
-    if (threadIdx.x == 0) atomicAdd(&nxt.cnt[C_EL_SMALL], m);
+}
+
+// One workgroup sorts a block of 65..kBlkT strings completely on chip: a
+// bitonic network over (key, position) pairs in registers, then every sorted
+// position is finished -- LCP to its neighbour, identical strings done, runs
+// of equal keys without a NUL handed to round r+1 (the reference's
+// insertion-sort base case, lcp_inssort.hpp:36-187, and the equal-bucket
+// rules, .hpp:1531-1557, on sorted keys).
+__global__ __launch_bounds__(kWG) void k_block(const Item* items, uint32_t first, const uint32_t* cnt_items,
+                                                Bufs B, Sink nxt, uint32_t depth)
+{
+    __shared__ key64 s_k[kBlkT];
+    __shared__ uint32_t s_p[kBlkT];
+    __shared__ uint32_t s_id[kBlkT];
+    const uint32_t idx = first + blockIdx.x;
+    if (idx >= *cnt_items) return;
+    const Item L = items[idx];
+    if (!item_ok(L, B) || L.size > kBlkT) {
+        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
+        return;
+    }
+    if (L.size <= kWG) block_sort_finish<1>(L, B, nxt, depth, s_k, s_p, s_id);
+    else if (L.size <= 2 * kWG) block_sort_finish<2>(L, B, nxt, depth, s_k, s_p, s_id);
+    else block_sort_finish<4>(L, B, nxt, depth, s_k, s_p, s_id);
 }
 
 // One workgroup per chunk of an equal-key leaf (all keys equal).
@@ -811,7 +645,6 @@ __global__ __launch_bounds__(kWG) void k_leaf_run(const Item* items, const uint3
         if (threadIdx.x == 0) {
             if (c == 0) B.bkey[L.begin] = key;
             if (ce == rend) B.bkey[rend - 1] = key;
-            atomicAdd(&nxt.cnt[C_EL_RUN], ce - cb);
         }
     }
 }
@@ -819,23 +652,50 @@ __global__ __launch_bounds__(kWG) void k_leaf_run(const Item* items, const uint3
 // LCP between neighbouring leaves of one round segment:
 // depth + lcp_key(last key of the left leaf, first key of the right leaf)
 // (sample_sort_lcp, .hpp:311-396).
-__global__ __launch_bounds__(kWG) void k_leafbound(const Item* t, const Item* s, const Item* r,
-                                                   uint32_t nt, uint32_t ns, uint32_t nr, Bufs B,
-                                                   uint32_t* cnt, uint32_t depth)
+__global__ __launch_bounds__(kWG) void k_leafbound(const Item* t, const Item* b, const Item* r, uint32_t nt,
+                                                   uint32_t nb, uint32_t nr, Bufs B, uint32_t depth)
 {
     const uint32_t i = blockIdx.x * kWG + threadIdx.x;
     Item L;
     if (i < nt) L = t[i];
-    else if (i < nt + ns) L = s[i - nt];
-    else if (i < nt + ns + nr) {
-        L = r[i - nt - ns];
+    else if (i < nt + nb) L = b[i - nt];
+    else if (i < nt + nb + nr) {
+        L = r[i - nt - nb];
         if ((L.flags >> 8) != 0) return;
     } else return;
     if (L.begin <= L.rbeg || !B.lcp) return;
-    if (L.begin >= B.n) { flag_error(cnt, ERR_BOUNDS); return; }
-    const key64 a = B.bkey[L.begin - 1], b = B.bkey[L.begin];
-    if (a == b) { atomicOr(&cnt[C_ERR], ERR_LCP); return; }
-    B.lcp[L.begin] = depth + lcp_key(a, b);
+    if (L.begin >= B.n) { flag_error(B.cnt, ERR_BOUNDS); return; }
+    const key64 a = B.bkey[L.begin - 1], c = B.bkey[L.begin];
+    if (a == c) { flag_error(B.cnt, ERR_LCP); return; }
+    B.lcp[L.begin] = depth + lcp_key(a, c);
+}
+
+// plain copy of 16-byte words (calibrates the HBM byte counters)
+__global__ __launch_bounds__(kWG) void k_copy16(const uint4* src, uint4* dst, uint64_t n16)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * kWG)
+        dst[i] = src[i];
+}
+
+// sum of the sizes of a work list range (profiling only)
+__global__ __launch_bounds__(kWG) void k_sum_sizes(const Item* items, uint32_t first, const uint32_t* cnt_items,
+                                                    unsigned long long* out)
+{
+    __shared__ uint32_t s_tmp[16];
+    const uint32_t n = *cnt_items;
+    uint32_t v = 0;
+    for (uint32_t i = first + blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
+        const Item it = items[i];
+        if (it.flags & F_SORTED) {   // a chunk of an equal-key leaf
+            const uint32_t c = it.flags >> 8, cb = c * kRunChunk;
+            v += cb < it.size ? min(kRunChunk, it.size - cb) : 0u;
+        } else {
+            v += it.size;
+        }
+    }
+    uint32_t tot;
+    block_excl_scan<kWaves>(v, &tot, s_tmp);
+    if (threadIdx.x == 0 && tot) atomicAdd(out, (unsigned long long)tot);
 }
 
 // ===========================================================================
@@ -866,13 +726,15 @@ int set_err(int code, const char* fmt, ...)
     } while (0)
 
 enum KernelId {
-    K_INIT, K_TREE, K_COUNT, K_SCAN, K_SCATTER, K_CHILDREN, K_MEDIUM,
-    K_LEAF_TINY, K_LEAF_SMALL, K_LEAF_RUN, K_LEAFBOUND, K_MISC, K_NUM
+    K_INIT, K_TREE, K_COUNT, K_SCAN, K_SCATTER, K_CHILDREN, K_BLOCK,
+    K_LEAF_TINY, K_LEAF_RUN, K_LEAFBOUND, K_MISC, K_NUM
 };
 const char* kKernelNames[K_NUM] = {
-    "k_init", "k_tree", "k_count", "k_scan", "k_scatter", "k_children", "k_medium",
-    "k_leaf_tiny", "k_leaf_small", "k_leaf_run", "k_leafbound", "k_misc"
+    "k_init", "k_tree", "k_count", "k_scan", "k_scatter", "k_children", "k_block",
+    "k_leaf_tiny", "k_leaf_run", "k_leafbound", "k_misc"
 };
+
+enum { S_BLOCK, S_TINY, S_RUN, S_NUM };
 
 }  // namespace
 
@@ -884,24 +746,32 @@ struct ps5_ctx {
     uint32_t* idx[2] = { nullptr, nullptr };
     key64* bkey = nullptr;
     uint32_t* bktrank = nullptr;
-    // lists: level lists [2] x {large, med}; leaf lists [2] x {tiny, small, run}
-    Item* l_large[2] = { nullptr, nullptr };
-    Item* l_med[2] = { nullptr, nullptr };
+    // large segments: first level of a round [round parity], later levels [level parity]
+    Item* l_round[2] = { nullptr, nullptr };
+    Item* l_lvl[2] = { nullptr, nullptr };
+    // leaves of a round [round parity]
     Item* l_tiny[2] = { nullptr, nullptr };
-    Item* l_small[2] = { nullptr, nullptr };
+    Item* l_block[2] = { nullptr, nullptr };
     Item* l_run[2] = { nullptr, nullptr };
-    uint32_t cap_large = 0, cap_med = 0, cap_tiny = 0, cap_small = 0, cap_run = 0;
+    uint32_t cap_large = 0, cap_block = 0, cap_tiny = 0, cap_run = 0;
     uint32_t* d_cnt = nullptr;
     uint32_t* h_cnt = nullptr;       // pinned
-    Item* h_large = nullptr;         // pinned
-    uint32_t h_large_cap = 0;
+    Item* h_large = nullptr;         // pinned copy of a large list
+    // multi-workgroup step of a level: descriptors and tile table
+    LargeDesc* d_desc = nullptr;
+    TileRef* d_tiles = nullptr;
+    LargeDesc* h_desc = nullptr;     // pinned
+    TileRef* h_tiles = nullptr;      // pinned
+    uint64_t tiles_cap = 0;
+    key64* trees = nullptr;
     uint32_t* hist = nullptr;
     uint64_t hist_cap = 0;
     uint32_t* part = nullptr;
     uint64_t part_cap = 0;
-    key64* tree = nullptr;
+    unsigned long long* d_sums = nullptr;   // profiling: elements per kernel class
+    unsigned long long* h_sums = nullptr;
     // tuning
-    int tb_shift = 4;
+    uint32_t leaf_target = 512;      // strings per block the multi-workgroup levels aim for
     uint64_t seed = 0x5EED5EEDull;
     // profiling
     bool profile = false;
@@ -919,16 +789,22 @@ void free_buffers(ps5_ctx* c)
 {
     auto f = [](void* p) { if (p) (void)hipFree(p); };
     for (int i = 0; i < 2; ++i) {
-        f(c->key[i]); f(c->idx[i]); f(c->l_large[i]); f(c->l_med[i]);
-        f(c->l_tiny[i]); f(c->l_small[i]); f(c->l_run[i]);
-        c->key[i] = nullptr; c->idx[i] = nullptr; c->l_large[i] = nullptr; c->l_med[i] = nullptr;
-        c->l_tiny[i] = nullptr; c->l_small[i] = nullptr; c->l_run[i] = nullptr;
+        f(c->key[i]); f(c->idx[i]); f(c->l_round[i]); f(c->l_lvl[i]);
+        f(c->l_tiny[i]); f(c->l_block[i]); f(c->l_run[i]);
+        c->key[i] = nullptr; c->idx[i] = nullptr; c->l_round[i] = nullptr; c->l_lvl[i] = nullptr;
+        c->l_tiny[i] = nullptr; c->l_block[i] = nullptr; c->l_run[i] = nullptr;
     }
-    f(c->bkey); f(c->bktrank); f(c->hist); f(c->part);
-    c->bkey = nullptr; c->bktrank = nullptr; c->hist = nullptr; c->part = nullptr;
+    f(c->bkey); f(c->bktrank); f(c->hist); f(c->part); f(c->trees); f(c->d_desc); f(c->d_tiles);
+    c->bkey = nullptr; c->bktrank = nullptr; c->hist = nullptr; c->part = nullptr; c->trees = nullptr;
+    c->d_desc = nullptr; c->d_tiles = nullptr;
+    if (c->h_large) (void)hipHostFree(c->h_large);
+    if (c->h_desc) (void)hipHostFree(c->h_desc);
+    if (c->h_tiles) (void)hipHostFree(c->h_tiles);
+    c->h_large = nullptr; c->h_desc = nullptr; c->h_tiles = nullptr;
     c->cap = 0;
     c->hist_cap = 0;
     c->part_cap = 0;
+    c->tiles_cap = 0;
 }
 
 int ensure_capacity(ps5_ctx* c, uint64_t n)
@@ -942,33 +818,34 @@ int ensure_capacity(ps5_ctx* c, uint64_t n)
     }
     HIPCHK(hipMalloc(&c->bkey, cap * sizeof(key64)));
     HIPCHK(hipMalloc(&c->bktrank, cap * sizeof(uint32_t)));
-    c->cap_large = (uint32_t)(cap / kMedT + 16);
-    c->cap_med = (uint32_t)(cap / kSmallT + 16);
+    c->cap_large = (uint32_t)(cap / kBlkT + 16);
+    c->cap_block = (uint32_t)(cap / kTinyT + 16);
     c->cap_tiny = (uint32_t)(cap + 16);
-    c->cap_small = (uint32_t)(cap / kTinyT + 16);
     c->cap_run = (uint32_t)(cap + 16);
     for (int i = 0; i < 2; ++i) {
-        HIPCHK(hipMalloc(&c->l_large[i], (uint64_t)c->cap_large * sizeof(Item)));
-        HIPCHK(hipMalloc(&c->l_med[i], (uint64_t)c->cap_med * sizeof(Item)));
+        HIPCHK(hipMalloc(&c->l_round[i], (uint64_t)c->cap_large * sizeof(Item)));
+        HIPCHK(hipMalloc(&c->l_lvl[i], (uint64_t)c->cap_large * sizeof(Item)));
         HIPCHK(hipMalloc(&c->l_tiny[i], (uint64_t)c->cap_tiny * sizeof(Item)));
-        HIPCHK(hipMalloc(&c->l_small[i], (uint64_t)c->cap_small * sizeof(Item)));
+        HIPCHK(hipMalloc(&c->l_block[i], (uint64_t)c->cap_block * sizeof(Item)));
         HIPCHK(hipMalloc(&c->l_run[i], (uint64_t)c->cap_run * sizeof(Item)));
     }
-    c->hist_cap = (uint64_t)kMaxBkt * kWaves * (cap / kTile + 2);
+    // every large segment has at most size/kTile + 1 tiles
+    c->tiles_cap = cap / kTile + c->cap_large + 1;
+    c->hist_cap = (uint64_t)kMaxBkt * kWaves * c->tiles_cap;
     HIPCHK(hipMalloc(&c->hist, c->hist_cap * sizeof(uint32_t)));
     c->part_cap = c->hist_cap / kScanBlock + 2;
     HIPCHK(hipMalloc(&c->part, c->part_cap * sizeof(uint32_t)));
-    if (c->h_large_cap < c->cap_large) {
-        if (c->h_large) (void)hipHostFree(c->h_large);
-        c->h_large = nullptr;
-        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_large), (uint64_t)c->cap_large * sizeof(Item)));
-        c->h_large_cap = c->cap_large;
-    }
+    HIPCHK(hipMalloc(&c->trees, (uint64_t)c->cap_large * 1024 * sizeof(key64)));
+    HIPCHK(hipMalloc(&c->d_desc, (uint64_t)c->cap_large * sizeof(LargeDesc)));
+    HIPCHK(hipMalloc(&c->d_tiles, c->tiles_cap * sizeof(TileRef)));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_large), (uint64_t)c->cap_large * sizeof(Item)));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_desc), (uint64_t)c->cap_large * sizeof(LargeDesc)));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_tiles), c->tiles_cap * sizeof(TileRef)));
     c->cap = cap;
     return PS5_OK;
 }
 
-struct Launch {   // brackets a launch with events when profiling
+struct Launch {   // counts a launch; brackets it with events when profiling
     ps5_ctx* c;
     int id;
     hipEvent_t a = nullptr, b = nullptr;
@@ -1011,30 +888,28 @@ void collect_events(ps5_ctx* c)
     c->ev_used.clear();
 }
 
-Sink make_sink(ps5_ctx* c, int lvl, int leafpar)
+// sink: large -> `large` list, leaves -> lists of round parity `rp`
+Sink make_sink(ps5_ctx* c, Item* large, int c_large, int rp)
 {
     Sink s;
-    s.large = c->l_large[lvl];
-    s.med = c->l_med[lvl];
-    s.tiny = c->l_tiny[leafpar];
-    s.small = c->l_small[leafpar];
-    s.run = c->l_run[leafpar];
+    s.large = large;
+    s.block = c->l_block[rp];
+    s.tiny = c->l_tiny[rp];
+    s.run = c->l_run[rp];
     s.cnt = c->d_cnt;
-    s.c_large = C_LARGE + lvl;
-    s.c_med = C_MED + lvl;
-    s.c_tiny = C_TINY + leafpar;
-    s.c_small = C_SMALL + leafpar;
-    s.c_run = C_RUN + leafpar;
+    s.c_large = c_large;
+    s.c_block = C_BLOCK + rp;
+    s.c_tiny = C_TINY + rp;
+    s.c_run = C_RUN + rp;
     s.cap_large = c->cap_large;
-    s.cap_med = c->cap_med;
+    s.cap_block = c->cap_block;
     s.cap_tiny = c->cap_tiny;
-    s.cap_small = c->cap_small;
     s.cap_run = c->cap_run;
     return s;
 }
 
-// copy counters (and up to the first kPeek large items of list `lvl`) to the host
-int readback(ps5_ctx* c, int lvl, bool want_large)
+// copy the counters (and the large list `list` with counter `ci`) to the host
+int readback(ps5_ctx* c, const Item* list, int ci)
 {
     HIPCHK(hipMemcpyAsync(c->h_cnt, c->d_cnt, C_NCNT * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1042,11 +917,11 @@ int readback(ps5_ctx* c, int lvl, bool want_large)
     if (c->h_cnt[C_ERR] & ERR_CAP) return set_err(PS5_ERR_CAPACITY, "work list capacity exceeded");
     if (c->h_cnt[C_ERR] & ERR_LCP) return set_err(PS5_ERR_INTERNAL, "equal keys across a leaf boundary");
     if (c->h_cnt[C_ERR] & ERR_BOUNDS) return set_err(PS5_ERR_INTERNAL, "device bounds check failed");
-    if (want_large) {
-        const uint32_t nl = c->h_cnt[C_LARGE + lvl];
+    if (list) {
+        const uint32_t nl = c->h_cnt[ci];
         if (nl > c->cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
         if (nl) {
-            HIPCHK(hipMemcpyAsync(c->h_large, c->l_large[lvl], nl * sizeof(Item), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(c->h_large, list, nl * sizeof(Item), hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
             c->st_syncs++;
         }
@@ -1056,6 +931,78 @@ int readback(ps5_ctx* c, int lvl, bool want_large)
 
 uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+void sum_sizes(ps5_ctx* c, const Item* list, uint32_t first, const uint32_t* cnt, uint32_t n, int slot)
+{
+    if (!c->profile || n == 0) return;
+    hipLaunchKernelGGL(k_sum_sizes, dim3(std::min<uint32_t>(div_up(n, kWG), 1024)), dim3(kWG), 0, c->stream,
+                       list, first, cnt, c->d_sums + slot);
+}
+
+// Treebits of a multi-workgroup step on m strings: the fewest levels of at
+// most 1023 splitters that bring buckets down to about leaf_target strings,
+// with the fan-out spread evenly over those levels.
+int large_treebits(uint32_t m, uint32_t target)
+{
+    const double need = std::log2(std::max(2.0, (double)m / target));
+    const int levels = std::max(1, (int)std::ceil(need / kMaxTreebits));
+    const int k = (int)std::ceil(need / levels);
+    return std::max(1, std::min(kMaxTreebits, k));
+}
+
+// One multi-workgroup sample-sort step over all large segments of a level.
+int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uint64_t seed)
+{
+    hipStream_t st = c->stream;
+    uint64_t ntiles = 0, hl = 0, elems = 0;
+    for (uint32_t i = 0; i < nlarge; ++i) {
+        const Item seg = c->h_large[i];
+        const uint32_t T = div_up(seg.size, kTile);
+        LargeDesc& D = c->h_desc[i];
+        D.seg = seg;
+        D.T = T;
+        D.hist_base = (uint32_t)hl;
+        D.k = (uint32_t)large_treebits(seg.size, c->leaf_target);
+        const uint32_t nspl = (1u << D.k) - 1;
+        D.oversample = std::max<uint32_t>(2, std::min<uint32_t>(32, (kMaxSamples / nspl) & ~1u));
+        if (ntiles + T > c->tiles_cap) return set_err(PS5_ERR_CAPACITY, "tile table capacity");
+        for (uint32_t t = 0; t < T; ++t) c->h_tiles[ntiles + t] = TileRef{ i, t };
+        ntiles += T;
+        hl += (uint64_t)((2u << D.k) - 1) * T * kWaves;
+        elems += seg.size;
+        if (hl > c->hist_cap || hl > 0xFFFFFFFFull) return set_err(PS5_ERR_CAPACITY, "histogram capacity");
+    }
+    HIPCHK(hipMemcpyAsync(c->d_desc, c->h_desc, nlarge * sizeof(LargeDesc), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->d_tiles, c->h_tiles, ntiles * sizeof(TileRef), hipMemcpyHostToDevice, st));
+    const uint32_t hl32 = (uint32_t)hl;
+    {
+        Launch L(c, K_TREE, (uint64_t)nlarge * kMaxSamples * 8);
+        hipLaunchKernelGGL(k_tree, dim3(nlarge), dim3(kBigWG), 0, st, c->d_desc, B, seed, c->trees);
+    }
+    {
+        Launch L(c, K_COUNT, elems * 12 + hl * 4);
+        hipLaunchKernelGGL(k_count, dim3((uint32_t)ntiles), dim3(kWG), 0, st, c->d_desc, c->d_tiles, B, c->trees,
+                           c->hist);
+    }
+    {
+        const uint32_t nb = div_up(hl32, kScanBlock);
+        Launch L(c, K_SCAN, hl * 12);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kWG), 0, st, c->hist, hl32, c->part);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kWG), 0, st, c->part, nb);
+        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, c->hist, hl32, c->part);
+    }
+    {
+        Launch L(c, K_SCATTER, elems * 28 + hl * 8);
+        hipLaunchKernelGGL(k_scatter, dim3((uint32_t)ntiles * kWaves), dim3(kBigWG), kScatLds, st, c->d_desc,
+                           c->d_tiles, B, c->hist);
+    }
+    {
+        Launch L(c, K_CHILDREN, hl * 4 / kWaves);
+        hipLaunchKernelGGL(k_children, dim3(nlarge), dim3(kWG), 0, st, c->d_desc, c->hist, sink);
+    }
+    c->st_segments += nlarge;
+    return PS5_OK;
+}
+
 int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* offs, uint32_t n,
              uint32_t* perm, uint32_t* lcp)
 {
@@ -1063,6 +1010,8 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     if (rc) return rc;
     for (int i = 0; i < K_NUM; ++i) { c->k_ms[i] = 0; c->k_launch[i] = 0; c->k_bytes[i] = 0; }
     c->st_rounds = c->st_levels = c->st_segments = c->st_leaves = c->st_syncs = 0;
+    hipStream_t st = c->stream;
+    if (c->profile) HIPCHK(hipMemsetAsync(c->d_sums, 0, S_NUM * sizeof(unsigned long long), st));
 
     Bufs B;
     B.n = n;
@@ -1076,7 +1025,6 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     B.lcp = lcp;
     B.bkey = c->bkey;
     B.bktrank = c->bktrank;
-    hipStream_t st = c->stream;
 
     {
         Launch L(c, K_INIT, (uint64_t)n * 24);
@@ -1085,121 +1033,93 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     {
         Item* list; int ci;
         if (n <= kTinyT) { list = c->l_tiny[0]; ci = C_TINY; }
-        else if (n <= kSmallT) { list = c->l_small[0]; ci = C_SMALL; }
-        else if (n <= kMedT) { list = c->l_med[0]; ci = C_MED; }
-        else { list = c->l_large[0]; ci = C_LARGE; }
+        else if (n <= kBlkT) { list = c->l_block[0]; ci = C_BLOCK; }
+        else { list = c->l_round[0]; ci = C_LROUND; }
         Launch L(c, K_MISC, 0);
         hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, st, c->d_cnt, list, ci, n);
     }
     HIPCHK(hipGetLastError());
-    if ((rc = readback(c, 0, true))) return rc;
+    if ((rc = readback(c, c->l_round[0], C_LROUND))) return rc;
 
     for (uint32_t round = 0;; ++round) {
         const uint32_t depth = round * 8;
-        const int lp = round & 1;           // leaf list parity of this round
+        const int rp = round & 1, np = rp ^ 1;
         c->st_rounds++;
-        int lv = 0;
-        for (uint32_t level = 0;; ++level) {   // levels
+        {   // the lists of round r+1 are empty again (consumed in round r-1)
+            Launch L(c, K_MISC, 0);
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, C_LROUND + np, C_TINY + np,
+                               C_BLOCK + np, C_RUN + np, -1);
+        }
+        const Sink nsink = make_sink(c, c->l_round[np], C_LROUND + np, np);
+        uint32_t blk_done = 0;
+        int cur_ci = C_LROUND + rp;
+        for (uint32_t level = 0;; ++level) {
             if (level > 4096) return set_err(PS5_ERR_INTERNAL, "sample-sort levels do not terminate");
-            const uint32_t nlarge = c->h_cnt[C_LARGE + lv];
-            const uint32_t nmed = c->h_cnt[C_MED + lv];
-            if (nlarge == 0 && nmed == 0) break;
+            const uint32_t nlarge = c->h_cnt[cur_ci];
+            const uint32_t nblk = c->h_cnt[C_BLOCK + rp] - blk_done;
+            if (nlarge == 0 && nblk == 0) break;
             c->st_levels++;
-            c->st_segments += nlarge + nmed;
-            const int nx = lv ^ 1;
+            Item* nx_large = c->l_lvl[level & 1];
+            const int nx_ci = C_LLVL + (level & 1);
             {
                 Launch L(c, K_MISC, 0);
-                hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, C_LARGE + nx, C_MED + nx, -1, -1, -1);
+                hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, nx_ci, -1, -1, -1, -1);
             }
-            Sink sink = make_sink(c, nx, lp);
-            std::vector<Item> large(c->h_large, c->h_large + nlarge);
-            for (const Item& seg : large) {
-                const uint32_t T = div_up(seg.size, kTile);
-                const uint64_t hl64 = (uint64_t)kMaxBkt * T * kWaves;
-                if (hl64 > 0xFFFFFFFFull) return set_err(PS5_ERR_ARG, "segment too large");
-                const uint32_t hl = (uint32_t)hl64;
-                if (hl > c->hist_cap) return set_err(PS5_ERR_CAPACITY, "histogram capacity");
-                const uint64_t seed = c->seed ^ ((uint64_t)round << 40);
+            const Sink sink = make_sink(c, nx_large, nx_ci, rp);
+            const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
+            if (nlarge && (rc = large_step(c, B, nlarge, sink, seed))) return rc;
+            if (nblk) {
                 {
-                    Launch L(c, K_TREE, 2046ull * 8);
-                    hipLaunchKernelGGL(k_tree, dim3(1), dim3(kWG), 0, st, seg, B, seed, c->tree);
+                    Launch L(c, K_BLOCK, 0);
+                    hipLaunchKernelGGL(k_block, dim3(nblk), dim3(kWG), 0, st, c->l_block[rp], blk_done,
+                                       c->d_cnt + C_BLOCK + rp, B, nsink, depth);
                 }
-                {
-                    Launch L(c, K_COUNT, (uint64_t)seg.size * 12 + (uint64_t)hl * 4);
-                    hipLaunchKernelGGL(k_count, dim3(T), dim3(kWG), 0, st, seg, B, c->tree, T, c->hist);
-                }
-                {
-                    const uint32_t nb = div_up(hl, kScanBlock);
-                    Launch L(c, K_SCAN, (uint64_t)hl * 12);
-                    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kWG), 0, st, c->hist, hl, c->part);
-                    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kWG), 0, st, c->part, nb);
-                    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, c->hist, hl, c->part);
-                }
-                {
-                    Launch L(c, K_SCATTER, (uint64_t)seg.size * 28 + (uint64_t)hl * 4);
-                    hipLaunchKernelGGL(k_scatter, dim3(T), dim3(kWG), 0, st, seg, B, T, c->hist);
-                }
-                {
-                    Launch L(c, K_CHILDREN, (uint64_t)kMaxBkt * 4);
-                    hipLaunchKernelGGL(k_children, dim3(1), dim3(kWG), 0, st, seg, T, c->hist, sink);
-                }
-            }
-            if (nmed) {
-                Launch L(c, K_MEDIUM, 0);
-                hipLaunchKernelGGL(k_medium, dim3(nmed), dim3(kWG), 0, st, c->l_med[lv], c->d_cnt + C_MED + lv,
-                                   B, sink, c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)c->st_levels << 32),
-                                   c->tb_shift);
+                sum_sizes(c, c->l_block[rp], blk_done, c->d_cnt + C_BLOCK + rp, blk_done + nblk, S_BLOCK);
+                blk_done += nblk;
             }
             HIPCHK(hipGetLastError());
-            if ((rc = readback(c, nx, true))) return rc;
-            lv = nx;
+            if ((rc = readback(c, nx_large, nx_ci))) return rc;
+            cur_ci = nx_ci;
         }
         // leaves of this round
-        const uint32_t nt = c->h_cnt[C_TINY + lp], ns = c->h_cnt[C_SMALL + lp], nr = c->h_cnt[C_RUN + lp];
-        c->st_leaves += nt + ns + nr;
-        const int np = lp ^ 1;
-        {
-            Launch L(c, K_MISC, 0);
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, C_LARGE + 0, C_MED + 0,
-                               C_TINY + np, C_SMALL + np, C_RUN + np);
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, C_LARGE + 1, C_MED + 1, -1, -1, -1);
-        }
-        Sink nsink = make_sink(c, 0, np);
+        const uint32_t nt = c->h_cnt[C_TINY + rp], nb = c->h_cnt[C_BLOCK + rp], nr = c->h_cnt[C_RUN + rp];
+        c->st_leaves += nt + nb + nr;
         if (nt) {
-            Launch L(c, K_LEAF_TINY, 0);
-            hipLaunchKernelGGL(k_leaf_tiny, dim3(div_up(nt, kWaves)), dim3(kWG), 0, st, c->l_tiny[lp],
-                               c->d_cnt + C_TINY + lp, B, nsink, depth);
-        }
-        if (ns) {
-            Launch L(c, K_LEAF_SMALL, 0);
-            hipLaunchKernelGGL(k_leaf_small, dim3(ns), dim3(kWG), 0, st, c->l_small[lp],
-                               c->d_cnt + C_SMALL + lp, B, nsink, depth);
+            {
+                Launch L(c, K_LEAF_TINY, 0);
+                hipLaunchKernelGGL(k_leaf_tiny, dim3(div_up(nt, kWaves)), dim3(kWG), 0, st, c->l_tiny[rp],
+                                   c->d_cnt + C_TINY + rp, B, nsink, depth);
+            }
+            sum_sizes(c, c->l_tiny[rp], 0, c->d_cnt + C_TINY + rp, nt, S_TINY);
         }
         if (nr) {
-            Launch L(c, K_LEAF_RUN, 0);
-            hipLaunchKernelGGL(k_leaf_run, dim3(nr), dim3(kWG), 0, st, c->l_run[lp],
-                               c->d_cnt + C_RUN + lp, B, nsink, depth);
+            {
+                Launch L(c, K_LEAF_RUN, 0);
+                hipLaunchKernelGGL(k_leaf_run, dim3(nr), dim3(kWG), 0, st, c->l_run[rp], c->d_cnt + C_RUN + rp, B,
+                                   nsink, depth);
+            }
+            sum_sizes(c, c->l_run[rp], 0, c->d_cnt + C_RUN + rp, nr, S_RUN);
         }
-        if (nt + ns + nr && lcp) {
-            Launch L(c, K_LEAFBOUND, (uint64_t)(nt + ns + nr) * 36);
-            hipLaunchKernelGGL(k_leafbound, dim3(div_up((uint64_t)nt + ns + nr, kWG)), dim3(kWG), 0, st,
-                               c->l_tiny[lp], c->l_small[lp], c->l_run[lp], nt, ns, nr, B, c->d_cnt, depth);
+        if (nt + nb + nr && lcp) {
+            Launch L(c, K_LEAFBOUND, (uint64_t)(nt + nb + nr) * 36);
+            hipLaunchKernelGGL(k_leafbound, dim3(div_up((uint64_t)nt + nb + nr, kWG)), dim3(kWG), 0, st,
+                               c->l_tiny[rp], c->l_block[rp], c->l_run[rp], nt, nb, nr, B, depth);
         }
         HIPCHK(hipGetLastError());
-        if ((rc = readback(c, 0, true))) return rc;
-        if (c->h_cnt[C_LARGE] + c->h_cnt[C_MED] + c->h_cnt[C_TINY + np] + c->h_cnt[C_SMALL + np] +
-                c->h_cnt[C_RUN + np] == 0)
+        if ((rc = readback(c, c->l_round[np], C_LROUND + np))) return rc;
+        if (c->h_cnt[C_LROUND + np] + c->h_cnt[C_TINY + np] + c->h_cnt[C_BLOCK + np] + c->h_cnt[C_RUN + np] == 0)
             break;
         if (depth > alen + 8) return set_err(PS5_ERR_INTERNAL, "depth exceeds the arena: no progress");
     }
-    // element-proportional byte accounting of the list-driven kernels
-    const uint64_t el_med = c->h_cnt[C_EL_MED], el_tiny = c->h_cnt[C_EL_TINY];
-    const uint64_t el_small = c->h_cnt[C_EL_SMALL], el_run = c->h_cnt[C_EL_RUN];
-    c->k_bytes[K_MEDIUM] += el_med * 40;
-    c->k_bytes[K_LEAF_TINY] += el_tiny * 20;
-    c->k_bytes[K_LEAF_SMALL] += el_small * 20;
-    c->k_bytes[K_LEAF_RUN] += el_run * 12;
-    if (c->profile) collect_events(c);
+    if (c->profile) {
+        HIPCHK(hipMemcpyAsync(c->h_sums, c->d_sums, S_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        // algorithmic bytes per string of the list-driven kernels
+        c->k_bytes[K_BLOCK] += c->h_sums[S_BLOCK] * 20;     // key 8 + idx 4 R, perm 4 + lcp 4 W
+        c->k_bytes[K_LEAF_TINY] += c->h_sums[S_TINY] * 20;
+        c->k_bytes[K_LEAF_RUN] += c->h_sums[S_RUN] * 12;    // idx 4 R, perm 4 + lcp 4 W
+        collect_events(c);
+    }
     return PS5_OK;
 }
 
@@ -1218,12 +1138,14 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     if (!out) return set_err(PS5_ERR_ARG, "out is NULL");
     *out = nullptr;
     HIPCHK(hipSetDevice(device));
+    HIPCHK(hipFuncSetAttribute((const void*)k_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, kScatLds));
     ps5_ctx* c = new ps5_ctx;
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_cnt, C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_cnt), C_NCNT * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&c->tree, 2048 * sizeof(key64));
+    if (e == hipSuccess) e = hipMalloc(&c->d_sums, S_NUM * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_sums), S_NUM * sizeof(unsigned long long));
     if (e != hipSuccess) {
         ps5_ctx_destroy(c);
         return set_err(PS5_ERR_HIP, "context setup failed: %s", hipGetErrorString(e));
@@ -1239,9 +1161,9 @@ void ps5_ctx_destroy(ps5_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_buffers(c);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
-    if (c->tree) (void)hipFree(c->tree);
+    if (c->d_sums) (void)hipFree(c->d_sums);
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
-    if (c->h_large) (void)hipHostFree(c->h_large);
+    if (c->h_sums) (void)hipHostFree(c->h_sums);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto& u : c->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1332,7 +1254,8 @@ int ps5_dev_malloc(void** p, uint64_t bytes)
 {
     if (!p) return set_err(PS5_ERR_ARG, "p is NULL");
     hipError_t e = hipMalloc(p, bytes ? bytes : 1);
-    if (e != hipSuccess) return set_err(PS5_ERR_NOMEM, "hipMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
+    if (e != hipSuccess)
+        return set_err(PS5_ERR_NOMEM, "hipMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
     return PS5_OK;
 }
 
@@ -1351,6 +1274,19 @@ int ps5_memcpy_h2d(void* dst, const void* src, uint64_t bytes)
 int ps5_memcpy_d2h(void* dst, const void* src, uint64_t bytes)
 {
     HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PS5_OK;
+}
+
+int ps5_dev_copy(void* dst, const void* src, uint64_t bytes)
+{
+    if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return set_err(PS5_ERR_ARG, "ps5_dev_copy needs 16-byte aligned pointers and size");
+    if (bytes == 0) return PS5_OK;
+    const uint64_t n16 = bytes / 16;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n16 + kWG - 1) / kWG, 65536);
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(kWG), 0, nullptr, (const uint4*)src, (uint4*)dst, n16);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
     return PS5_OK;
 }
 

@@ -23,6 +23,7 @@ EXPORTS = [
     "ps5_sort_device", "ps5_ctx_set_profile", "ps5_ctx_kernel_count", "ps5_ctx_kernel_stat",
     "ps5_ctx_work_stats",
     "ps5_dev_malloc", "ps5_dev_free", "ps5_memcpy_h2d", "ps5_memcpy_d2h", "ps5_device_synchronize",
+    "ps5_dev_copy",
     "ps5_sort_device_timed",
     "ps5_gen_uniform", "ps5_gen_windows", "ps5_gen_reference_random", "ps5_gen_prefixed",
 ]
@@ -75,6 +76,7 @@ def load():
     _sig(lib, "ps5_memcpy_h2d", _I, [_P, _P, _U64])
     _sig(lib, "ps5_memcpy_d2h", _I, [_P, _P, _U64])
     _sig(lib, "ps5_device_synchronize", _I, [])
+    _sig(lib, "ps5_dev_copy", _I, [_P, _P, _U64])
     _sig(lib, "ps5_gen_uniform", _U64, [_U64, _U64, _U32, _U32, _P, _U32, _P, _P, _U64,
                                         ctypes.POINTER(_U64)])
     _sig(lib, "ps5_gen_windows", _I, [_U64, _U64, _U32, _U32, ctypes.c_double, _U64, _U32, _P, _P])
