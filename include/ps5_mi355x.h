@@ -88,6 +88,18 @@ int ps5_sort_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
                     const uint32_t* d_offs, uint64_t n,
                     uint32_t* d_perm, uint32_t* d_lcp);
 
+/* Batch: sort nsets independent string sets in one pass over the device.
+ * Set s is the strings with input indices [set_first[s], set_first[s+1])
+ * (set_first: host array of nsets+1 non-decreasing values, set_first[0] = 0);
+ * d_offs holds uint64 offsets into one arena (which may exceed 4 GiB).
+ * Results per set, at the set's own positions [set_first[s], set_first[s+1]):
+ * d_perm = index inside the set of the i-th smallest string of the set,
+ * d_lcp (may be NULL) = LCP with its predecessor inside the set; the first
+ * LCP slot of every set is never written. */
+int ps5_sort_batch_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
+                          const uint64_t* d_offs, const uint64_t* set_first, uint32_t nsets,
+                          uint32_t* d_perm, uint32_t* d_lcp);
+
 /* Per-kernel accounting of the last ps5_sort_device call when profiling is
  * on (ps5_ctx_set_profile(ctx, 1)): each launch is bracketed by HIP events on
  * the context's stream. */
@@ -153,6 +165,19 @@ uint64_t ps5_gen_reference_random(uint64_t size, const uint8_t* letters, uint32_
 uint64_t ps5_gen_prefixed(uint64_t seed, uint64_t arena_cap, uint32_t nprefix,
                           double zipf_s, uint8_t* arena, uint32_t* offs,
                           uint64_t offs_cap, uint64_t* arena_used);
+
+/* Batch workload: nsets sets; with s = splitmix64(seed + k), set k draws its
+ * string count floor(exp(U(ln count_lo, ln count_hi))) and its alphabet (ACGT,
+ * '0'..'9' or bytes 33..126, uniformly) from xoshiro256** seeded with s, its
+ * string lengths U[len_lo, len_hi] from a stream seeded with
+ * s ^ 0x5851F42D4C957F2D and its characters from one seeded with s ^ 0xC4A2C7E5.  plan: fills counts[nsets] and
+ * bytes[nsets] (arena bytes of each set, NULs included).  fill: writes set k
+ * at arena + sum(bytes[0..k)) with uint64 offsets at offs + sum(counts[0..k)),
+ * in parallel over sets. */
+int ps5_gen_batch_plan(uint64_t seed, uint32_t nsets, uint64_t count_lo, uint64_t count_hi,
+                       uint32_t len_lo, uint32_t len_hi, uint64_t* counts, uint64_t* bytes);
+int ps5_gen_batch_fill(uint64_t seed, uint32_t nsets, uint64_t count_lo, uint64_t count_hi,
+                       uint32_t len_lo, uint32_t len_hi, uint8_t* arena, uint64_t* offs);
 
 #ifdef __cplusplus
 }

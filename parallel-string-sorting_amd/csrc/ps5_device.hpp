@@ -85,6 +85,12 @@ __device__ __forceinline__ uint64_t match_bucket(uint32_t b)
 
 __device__ __forceinline__ void flag_error(uint32_t* cnt, uint32_t e) { atomicOr(&cnt[C_ERR], e); }
 
+// byte offset of input string i in the arena
+__device__ __forceinline__ uint64_t str_off(const Bufs& B, uint32_t i)
+{
+    return B.offs64 ? B.offs64[i] : (uint64_t)B.offs[i];
+}
+
 // an item must describe a non-empty range inside [0, n)
 __device__ __forceinline__ bool item_ok(const Item& it, const Bufs& B)
 {

@@ -3,9 +3,12 @@
 // The reference's artificial inputs (src/tools/input.hpp:425-468) and the
 // workloads this project measures.  PRNG: xoshiro256** seeded through
 // splitmix64 (Blackman & Vigna, public domain algorithms).
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "../../include/ps5_mi355x.h"
@@ -175,4 +178,90 @@ extern "C" uint64_t ps5_gen_prefixed(uint64_t seed, uint64_t arena_cap, uint32_t
     if (pos < arena_cap) memset(arena + pos, 0, arena_cap - pos);
     if (arena_used) *arena_used = pos;
     return n;
+}
+
+// ---- batch workload -------------------------------------------------------
+
+namespace {
+
+struct BatchSet {
+    uint64_t count;
+    int alpha;          // 0: ACGT, 1: digits, 2: bytes 33..126
+    uint64_t seed;
+};
+
+BatchSet batch_set(uint64_t seed, uint32_t k, uint64_t count_lo, uint64_t count_hi)
+{
+    SplitMix sm{ seed + k };
+    const uint64_t s = sm.next();
+    Xoshiro rng(s);
+    const double u = rng.unit();
+    const double lo = std::log((double)count_lo), hi = std::log((double)count_hi);
+    uint64_t count = (uint64_t)std::floor(std::exp(lo + u * (hi - lo)));
+    count = std::max(count_lo, std::min(count_hi, count));
+    const int alpha = (int)rng.uniform(0, 2);
+    return { count, alpha, s };
+}
+
+template <class F>
+void parallel_sets(uint32_t nsets, F f)
+{
+    unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::atomic<uint32_t> next{ 0 };
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; ++t)
+        th.emplace_back([&] {
+            for (uint32_t k; (k = next.fetch_add(1)) < nsets;) f(k);
+        });
+    for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" int ps5_gen_batch_plan(uint64_t seed, uint32_t nsets, uint64_t count_lo, uint64_t count_hi,
+                                  uint32_t len_lo, uint32_t len_hi, uint64_t* counts, uint64_t* bytes)
+{
+    if (count_lo == 0 || count_hi < count_lo || len_hi < len_lo) return PS5_ERR_ARG;
+    parallel_sets(nsets, [&](uint32_t k) {
+        const BatchSet b = batch_set(seed, k, count_lo, count_hi);
+        Xoshiro lens(b.seed ^ 0x5851F42D4C957F2Dull);
+        uint64_t tot = 0;
+        for (uint64_t i = 0; i < b.count; ++i) tot += lens.uniform(len_lo, len_hi) + 1;
+        counts[k] = b.count;
+        bytes[k] = tot;
+    });
+    return PS5_OK;
+}
+
+extern "C" int ps5_gen_batch_fill(uint64_t seed, uint32_t nsets, uint64_t count_lo, uint64_t count_hi,
+                                  uint32_t len_lo, uint32_t len_hi, uint8_t* arena, uint64_t* offs)
+{
+    if (count_lo == 0 || count_hi < count_lo || len_hi < len_lo) return PS5_ERR_ARG;
+    std::vector<uint64_t> counts(nsets), bytes(nsets);
+    ps5_gen_batch_plan(seed, nsets, count_lo, count_hi, len_lo, len_hi, counts.data(), bytes.data());
+    std::vector<uint64_t> sfirst(nsets + 1, 0), bfirst(nsets + 1, 0);
+    for (uint32_t k = 0; k < nsets; ++k) {
+        sfirst[k + 1] = sfirst[k] + counts[k];
+        bfirst[k + 1] = bfirst[k] + bytes[k];
+    }
+    static const uint8_t acgt[4] = { 'A', 'C', 'G', 'T' };
+    parallel_sets(nsets, [&](uint32_t k) {
+        const BatchSet b = batch_set(seed, k, count_lo, count_hi);
+        Xoshiro lens(b.seed ^ 0x5851F42D4C957F2Dull), chars(b.seed ^ 0xC4A2C7E5ull);
+        uint64_t pos = bfirst[k];
+        for (uint64_t i = 0; i < b.count; ++i) {
+            const uint64_t L = lens.uniform(len_lo, len_hi);
+            offs[sfirst[k] + i] = pos;
+            for (uint64_t j = 0; j < L; ++j) {
+                uint8_t ch;
+                if (b.alpha == 0) ch = acgt[chars.uniform(0, 3)];
+                else if (b.alpha == 1) ch = (uint8_t)('0' + chars.uniform(0, 9));
+                else ch = (uint8_t)chars.uniform(33, 126);
+                arena[pos + j] = ch;
+            }
+            arena[pos + L] = 0;
+            pos += L + 1;
+        }
+    });
+    return PS5_OK;
 }

@@ -88,7 +88,8 @@ struct Bufs {
     uint32_t* cnt;       // counters (error flags)
     const uint8_t* arena;
     uint64_t alen;
-    const uint32_t* offs;
+    const uint32_t* offs;      // string start offsets (uint32), or
+    const uint64_t* offs64;    // uint64 offsets when non-null (batches over 4 GiB)
     key64* key0; key64* key1;
     uint32_t* idx0; uint32_t* idx1;
     uint32_t* perm;

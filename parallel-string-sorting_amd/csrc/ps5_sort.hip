@@ -38,7 +38,7 @@ __global__ __launch_bounds__(kWG) void k_init(Bufs B, uint32_t n)
 {
     const uint32_t i = blockIdx.x * kWG + threadIdx.x;
     if (i >= n) return;
-    B.key0[i] = load_key(B.arena, B.alen, B.offs[i]);
+    B.key0[i] = load_key(B.arena, B.alen, str_off(B, i));
     B.idx0[i] = i;
 }
 
@@ -391,7 +391,7 @@ __device__ __forceinline__ void finish_position(const Bufs& B, uint32_t pos, uin
         if (eqp && B.lcp) B.lcp[pos] = depth + key_depth(k);
     } else {
         B.idx0[pos] = id;
-        if (id < B.n) B.key0[pos] = load_key(B.arena, B.alen, (uint64_t)B.offs[id] + depth + 8);
+        if (id < B.n) B.key0[pos] = load_key(B.arena, B.alen, str_off(B, id) + depth + 8);
         else flag_error(B.cnt, ERR_BOUNDS);
     }
 }
@@ -485,9 +485,9 @@ __device__ __forceinline__ void block_bitonic(key64 (&k)[E], uint32_t (&p)[E], k
 {
     constexpr int P = kWG * E;
     const uint32_t tid = threadIdx.x;
-#pragma unroll 1
+#pragma unroll
     for (int kk = 2; kk <= P; kk <<= 1) {
-#pragma unroll 1
+#pragma unroll
         for (int j = kk >> 1; j > 0; j >>= 1) {
             if (j >= kWG) {
                 const int d = j / kWG;
@@ -634,7 +634,7 @@ __global__ __launch_bounds__(kWG) void k_leaf_run(const Item* items, const uint3
             if (p > L.begin && B.lcp) B.lcp[p] = depth + key_depth(key);
         } else {
             if (src) B.idx0[p] = id;
-            if (id < B.n) B.key0[p] = load_key(B.arena, B.alen, (uint64_t)B.offs[id] + depth + 8);
+            if (id < B.n) B.key0[p] = load_key(B.arena, B.alen, str_off(B, id) + depth + 8);
             else flag_error(B.cnt, ERR_BOUNDS);
         }
     }
@@ -668,6 +668,22 @@ __global__ __launch_bounds__(kWG) void k_leafbound(const Item* t, const Item* b,
     const key64 a = B.bkey[L.begin - 1], c = B.bkey[L.begin];
     if (a == c) { flag_error(B.cnt, ERR_LCP); return; }
     B.lcp[L.begin] = depth + lcp_key(a, c);
+}
+
+// Batch outputs: input index -> index inside its set (perm[pos] - first of
+// the set holding pos; a set occupies the same positions as its inputs).
+__global__ __launch_bounds__(kWG) void k_localize(uint32_t* perm, uint32_t n, const uint64_t* set_first,
+                                                   uint32_t nsets)
+{
+    const uint32_t pos = blockIdx.x * kWG + threadIdx.x;
+    if (pos >= n) return;
+    uint32_t lo = 0, hi = nsets;   // largest s with set_first[s] <= pos
+    while (lo + 1 < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (set_first[mid] <= pos) lo = mid;
+        else hi = mid;
+    }
+    perm[pos] -= (uint32_t)set_first[lo];
 }
 
 // plain copy of 16-byte words (calibrates the HBM byte counters)
@@ -771,7 +787,7 @@ struct ps5_ctx {
     unsigned long long* d_sums = nullptr;   // profiling: elements per kernel class
     unsigned long long* h_sums = nullptr;
     // tuning
-    uint32_t leaf_target = 512;      // strings per block the multi-workgroup levels aim for
+    uint32_t leaf_target = 256;      // strings per block the multi-workgroup levels aim for
     uint64_t seed = 0x5EED5EEDull;
     // profiling
     bool profile = false;
@@ -1003,8 +1019,48 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     return PS5_OK;
 }
 
-int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* offs, uint32_t n,
-             uint32_t* perm, uint32_t* lcp)
+// Roots of a sort: one segment [0, n), or one per set of a batch (sets never
+// mix: every set is its own round segment, so no LCP crosses a set border).
+int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nsets)
+{
+    hipStream_t st = c->stream;
+    if (!set_first) {
+        Item* list; int ci;
+        if (n <= kTinyT) { list = c->l_tiny[0]; ci = C_TINY; }
+        else if (n <= kBlkT) { list = c->l_block[0]; ci = C_BLOCK; }
+        else { list = c->l_round[0]; ci = C_LROUND; }
+        Launch L(c, K_MISC, 0);
+        hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, st, c->d_cnt, list, ci, n);
+        return PS5_OK;
+    }
+    std::vector<Item> tiny, block, large;
+    for (uint32_t s = 0; s < nsets; ++s) {
+        const uint64_t b = set_first[s], e = set_first[s + 1];
+        if (e < b || e > n) return set_err(PS5_ERR_ARG, "set boundaries must be non-decreasing and <= n");
+        if (e == b) continue;
+        Item it = { (uint32_t)b, (uint32_t)(e - b), (uint32_t)b, 0u };
+        if (it.size <= kTinyT) tiny.push_back(it);
+        else if (it.size <= kBlkT) block.push_back(it);
+        else large.push_back(it);
+    }
+    if (large.size() > c->cap_large || block.size() > c->cap_block || tiny.size() > c->cap_tiny)
+        return set_err(PS5_ERR_CAPACITY, "too many sets");
+    std::vector<uint32_t> cnt(C_NCNT, 0);
+    cnt[C_TINY] = (uint32_t)tiny.size();
+    cnt[C_BLOCK] = (uint32_t)block.size();
+    cnt[C_LROUND] = (uint32_t)large.size();
+    HIPCHK(hipStreamSynchronize(st));
+    if (!tiny.empty()) HIPCHK(hipMemcpy(c->l_tiny[0], tiny.data(), tiny.size() * sizeof(Item), hipMemcpyHostToDevice));
+    if (!block.empty())
+        HIPCHK(hipMemcpy(c->l_block[0], block.data(), block.size() * sizeof(Item), hipMemcpyHostToDevice));
+    if (!large.empty())
+        HIPCHK(hipMemcpy(c->l_round[0], large.data(), large.size() * sizeof(Item), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_cnt, cnt.data(), C_NCNT * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return PS5_OK;
+}
+
+int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* offs, const uint64_t* offs64,
+             uint32_t n, uint32_t* perm, uint32_t* lcp, const uint64_t* set_first, uint32_t nsets)
 {
     int rc = ensure_capacity(c, n);
     if (rc) return rc;
@@ -1019,6 +1075,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     B.arena = arena;
     B.alen = alen;
     B.offs = offs;
+    B.offs64 = offs64;
     B.key0 = c->key[0]; B.key1 = c->key[1];
     B.idx0 = c->idx[0]; B.idx1 = c->idx[1];
     B.perm = perm;
@@ -1030,14 +1087,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
         Launch L(c, K_INIT, (uint64_t)n * 24);
         hipLaunchKernelGGL(k_init, dim3(div_up(n, kWG)), dim3(kWG), 0, st, B, n);
     }
-    {
-        Item* list; int ci;
-        if (n <= kTinyT) { list = c->l_tiny[0]; ci = C_TINY; }
-        else if (n <= kBlkT) { list = c->l_block[0]; ci = C_BLOCK; }
-        else { list = c->l_round[0]; ci = C_LROUND; }
-        Launch L(c, K_MISC, 0);
-        hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, st, c->d_cnt, list, ci, n);
-    }
+    if ((rc = setup_roots(c, n, set_first, nsets))) return rc;
     HIPCHK(hipGetLastError());
     if ((rc = readback(c, c->l_round[0], C_LROUND))) return rc;
 
@@ -1191,8 +1241,40 @@ int ps5_sort_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, co
         return set_err(PS5_ERR_ARG, "arena_bytes must be in [1, 2^32)");
     if (((uintptr_t)d_arena & 7u) != 0) return set_err(PS5_ERR_ARG, "arena must be 8-byte aligned");
     HIPCHK(hipSetDevice(c->device));
-    int rc = run_sort(c, d_arena, arena_bytes, d_offs, (uint32_t)n, d_perm, d_lcp);
+    int rc = run_sort(c, d_arena, arena_bytes, d_offs, nullptr, (uint32_t)n, d_perm, d_lcp, nullptr, 0);
     if (rc) {
+        (void)hipStreamSynchronize(c->stream);
+        if (c->profile) collect_events(c);
+    }
+    return rc;
+}
+
+int ps5_sort_batch_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, const uint64_t* d_offs,
+                          const uint64_t* set_first, uint32_t nsets, uint32_t* d_perm, uint32_t* d_lcp)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
+    if (!set_first || nsets == 0) return set_err(PS5_ERR_ARG, "no sets");
+    const uint64_t n = set_first[nsets];
+    if (n == 0) return PS5_OK;
+    if (!d_arena || !d_offs || !d_perm) return set_err(PS5_ERR_ARG, "NULL device pointer");
+    if (n >= 0xFFFFFF00ull) return set_err(PS5_ERR_ARG, "n too large (limit 2^32-256 strings)");
+    if (set_first[0] != 0) return set_err(PS5_ERR_ARG, "set_first[0] must be 0");
+    if (arena_bytes == 0) return set_err(PS5_ERR_ARG, "empty arena");
+    if (((uintptr_t)d_arena & 7u) != 0) return set_err(PS5_ERR_ARG, "arena must be 8-byte aligned");
+    HIPCHK(hipSetDevice(c->device));
+    int rc = run_sort(c, d_arena, arena_bytes, nullptr, d_offs, (uint32_t)n, d_perm, d_lcp, set_first, nsets);
+    if (rc == PS5_OK) {
+        uint64_t* d_first = nullptr;
+        HIPCHK(hipMalloc(&d_first, (nsets + 1) * sizeof(uint64_t)));
+        hipError_t e = hipMemcpy(d_first, set_first, (nsets + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_localize, dim3(div_up(n, kWG)), dim3(kWG), 0, c->stream, d_perm, (uint32_t)n, d_first,
+                               nsets);
+            e = hipStreamSynchronize(c->stream);
+        }
+        (void)hipFree(d_first);
+        if (e != hipSuccess) return set_err(PS5_ERR_HIP, "batch localize: %s", hipGetErrorString(e));
+    } else {
         (void)hipStreamSynchronize(c->stream);
         if (c->profile) collect_events(c);
     }

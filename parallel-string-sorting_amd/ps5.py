@@ -26,6 +26,7 @@ EXPORTS = [
     "ps5_dev_copy",
     "ps5_sort_device_timed",
     "ps5_gen_uniform", "ps5_gen_windows", "ps5_gen_reference_random", "ps5_gen_prefixed",
+    "ps5_sort_batch_device", "ps5_gen_batch_plan", "ps5_gen_batch_fill",
 ]
 
 
@@ -83,6 +84,9 @@ def load():
     _sig(lib, "ps5_gen_reference_random", _U64, [_U64, _P, _U32, _P, _P, _U64])
     _sig(lib, "ps5_gen_prefixed", _U64, [_U64, _U64, _U32, ctypes.c_double, _P, _P, _U64,
                                          ctypes.POINTER(_U64)])
+    _sig(lib, "ps5_sort_batch_device", _I, [_P, _P, _U64, _P, _P, _U32, _P, _P])
+    _sig(lib, "ps5_gen_batch_plan", _I, [_U64, _U32, _U64, _U64, _U32, _U32, _P, _P])
+    _sig(lib, "ps5_gen_batch_fill", _I, [_U64, _U32, _U64, _U64, _U32, _U32, _P, _P])
     _lib = lib
     return lib
 
@@ -149,6 +153,23 @@ def gen_prefixed(seed, arena_bytes, nprefix=1000, zipf_s=1.1):
     n = lib.ps5_gen_prefixed(seed, arena_bytes, nprefix, zipf_s, _ptr(arena), _ptr(offs), cap,
                              ctypes.byref(used))
     return arena, offs[:n].copy()
+
+
+def gen_batch(seed, nsets, count_lo, count_hi, len_lo=4, len_hi=48):
+    """Batch of independent sets (ps5_gen_batch_plan/fill): returns
+    (arena uint8 with 16 bytes padding, offs uint64, set_first uint64[nsets+1])."""
+    lib = load()
+    counts = np.zeros(nsets, np.uint64)
+    nbytes = np.zeros(nsets, np.uint64)
+    _check(lib.ps5_gen_batch_plan(seed, nsets, count_lo, count_hi, len_lo, len_hi, _ptr(counts), _ptr(nbytes)),
+           "ps5_gen_batch_plan")
+    set_first = np.zeros(nsets + 1, np.uint64)
+    set_first[1:] = np.cumsum(counts)
+    arena = np.zeros(int(nbytes.sum()) + 16, np.uint8)
+    offs = np.zeros(int(set_first[-1]), np.uint64)
+    _check(lib.ps5_gen_batch_fill(seed, nsets, count_lo, count_hi, len_lo, len_hi, _ptr(arena), _ptr(offs)),
+           "ps5_gen_batch_fill")
+    return arena, offs, set_first
 
 
 def workload(name, scale=1.0):
@@ -244,6 +265,31 @@ class Context:
             da.upload(arena)
             do.upload(np.ascontiguousarray(offs, np.uint32))
             self.sort_device(da, arena_bytes, do, n, dp, dl)
+            perm = dp.download(np.uint32, n)
+            lcp = dl.download(np.uint32, n) if with_lcp else None
+        finally:
+            for b in (da, do, dp, dl):
+                if b is not None:
+                    b.free()
+        return perm, lcp
+
+    def sort_batch_device(self, d_arena, arena_bytes, d_offs64, set_first, d_perm, d_lcp=None):
+        sf = np.ascontiguousarray(set_first, np.uint64)
+        _check(load().ps5_sort_batch_device(self.h, d_arena.ptr, arena_bytes, d_offs64.ptr, _ptr(sf), len(sf) - 1,
+                                            d_perm.ptr, d_lcp.ptr if d_lcp is not None else None),
+               "ps5_sort_batch_device")
+
+    def sort_batch(self, arena, offs64, set_first, with_lcp=True):
+        """Upload a batch, sort every set, download: (perm, lcp) with per-set local indices."""
+        n = int(set_first[-1])
+        da = DeviceBuffer(arena.size)
+        do = DeviceBuffer(max(8, n * 8))
+        dp = DeviceBuffer(max(4, n * 4))
+        dl = DeviceBuffer(max(4, n * 4)) if with_lcp else None
+        try:
+            da.upload(arena)
+            do.upload(np.ascontiguousarray(offs64, np.uint64))
+            self.sort_batch_device(da, int(arena.size), do, set_first, dp, dl)
             perm = dp.download(np.uint32, n)
             lcp = dl.download(np.uint32, n) if with_lcp else None
         finally:

@@ -64,3 +64,13 @@ def test_missing_library_fails_loudly(ps5, monkeypatch):
     monkeypatch.setattr(ps5, "LIB_PATH", "/nonexistent/libps5_mi355x.so")
     with pytest.raises(ps5.Ps5Error):
         ps5.load()
+
+
+def test_reference_random_matches_fixture(ps5):
+    """the C++ restatement of psstest's random10 equals the independent
+    Python restatement stored in the golden fixture"""
+    import numpy as np
+    from strsets import unpack
+    z = np.load(os.path.join(ROOT, "tests", "golden", "random10_64k.npz"))
+    arena, offs = ps5.gen_reference_random(65536, b"0123456789")
+    assert unpack(arena, offs) == unpack(z["arena"], z["offs"])

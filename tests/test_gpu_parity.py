@@ -152,3 +152,35 @@ def test_ascii2gb_full(ps5, ctx, oracle):
     perm, lcp = ctx.sort(arena, offs)
     r, msg = oracle.verify(arena, offs, perm, lcp)
     assert r == 0, msg
+
+
+def _check_batch(oracle, arena, offs64, set_first, perm, lcp):
+    for s in range(len(set_first) - 1):
+        b, e = int(set_first[s]), int(set_first[s + 1])
+        if e == b:
+            continue
+        o = offs64[b:e]
+        op, ol = oracle.brute_sort(arena, o)
+        assert np.array_equal(perm[b:e], op), f"set {s}: permutation differs"
+        assert np.array_equal(lcp[b + 1:e], ol[1:]), f"set {s}: lcp differs"
+
+
+def test_batch_mixed_sets(ps5, ctx, oracle):
+    """many independent sets of uneven sizes and alphabets in one batch"""
+    say("batch 200 sets")
+    arena, offs, sf = ps5.gen_batch(0x5EED0004, 200, 1, 30000)
+    perm, lcp = ctx.sort_batch(arena, offs, sf)
+    _check_batch(oracle, arena, offs, sf, perm, lcp)
+
+
+def test_batch_edge_sets(ps5, ctx, oracle):
+    """empty sets, single strings, duplicates and a large set side by side"""
+    from strsets import pack
+    sets = [[], [b"z"], [b"ab"] * 70, random_strings(5000, 0, 20, b"ab", 5), [], [b""] * 3,
+            random_strings(70000, 1, 12, b"xyz", 6)]
+    flat = [s for st in sets for s in st]
+    arena, offs32 = pack(flat)
+    sf = np.zeros(len(sets) + 1, np.uint64)
+    sf[1:] = np.cumsum([len(st) for st in sets])
+    perm, lcp = ctx.sort_batch(arena, offs32.astype(np.uint64), sf)
+    _check_batch(oracle, arena, offs32.astype(np.uint64), sf, perm, lcp)
