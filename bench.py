@@ -6,6 +6,14 @@ strings with length uniform in [1,64] over bytes 33..126, seeded, arena and
 offsets already resident in HBM.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ascii2gb|random10mb|highlcp]
+
+Two further comparisons print their own JSON line when asked for:
+  --mode batch   independent sorts in one batched pass vs one sort call per set
+                 (default: 4096 sets, log-uniform 1k..2M strings, length U[4,48],
+                 alphabets ACGT / digits / bytes 33..126, seed 0x5EED0004)
+  --mode serve   Poisson arrivals served by the continuous-batching server vs a
+                 FIFO of plain sorts (log-uniform 10 KB..256 MB requests of
+                 prefix-heavy strings, seed 0x5EED0005)
 """
 import argparse
 import importlib.util
@@ -58,8 +66,180 @@ def cpu_baseline(ps5, workload, budget_s):
                       f"of the oracle's pS5 restatement (treebits 10, no LCP) with {threads} OpenMP threads"}
 
 
+def bench_batch(ps5, args):
+    """Batched pass over many independent sets vs one device sort per set."""
+    lib = ps5.load()
+    t0 = time.time()
+    arena, offs, sf = ps5.gen_batch(0x5EED0004, args.sets, args.count_lo, args.count_hi, 4, 48)
+    n = int(sf[-1])
+    print(f"[bench] batch: {args.sets} sets, {n} strings, {arena.size} bytes generated in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    ctx = ps5.Context(0)
+    assert lib.ps5_ctx_reserve(ctx.h, n) == 0
+    d_arena = ps5.DeviceBuffer(arena.size)
+    d_offs = ps5.DeviceBuffer(n * 8)
+    d_perm = ps5.DeviceBuffer(n * 4)
+    d_lcp = ps5.DeviceBuffer(n * 4)
+    d_arena.upload(arena)
+    d_offs.upload(offs)
+    for _ in range(args.warmup):
+        ctx.sort_batch_device(d_arena, int(arena.size), d_offs, sf, d_perm, d_lcp)
+    lib.ps5_device_synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.sort_batch_device(d_arena, int(arena.size), d_offs, sf, d_perm, d_lcp)
+    lib.ps5_device_synchronize()
+    batched_s = (time.perf_counter() - t) / args.steps
+    perm = d_perm.download(np.uint32, n)
+    lcp = d_lcp.download(np.uint32, n)
+    # plain path: one ps5_sort_device call per set (uint32 offsets local to the set)
+    starts = np.array([int(offs[int(sf[k])]) for k in range(args.sets)], np.uint64)
+    ends = np.append(starts[1:], np.uint64(arena.size - 16))      # sets are laid out back to back
+    local = np.empty(n, np.uint32)
+    for k in range(args.sets):
+        b, e = int(sf[k]), int(sf[k + 1])
+        local[b:e] = (offs[b:e] - starts[k]).astype(np.uint32)
+    d_local = ps5.DeviceBuffer(max(4, n * 4))
+    d_local.upload(local)
+    pctx = ps5.Context(0)
+    d_pp = ps5.DeviceBuffer(n * 4)
+    d_pl = ps5.DeviceBuffer(n * 4)
+    import ctypes
+    P = ctypes.c_void_p
+    ab, ob, pb, lb = d_arena.ptr.value, d_local.ptr.value, d_pp.ptr.value, d_pl.ptr.value
+
+    def plain_pass():
+        for k in range(args.sets):
+            b, e = int(sf[k]), int(sf[k + 1])
+            if e == b:
+                continue
+            rc = lib.ps5_sort_device(pctx.h, P(ab + int(starts[k])), int(ends[k] - starts[k]), P(ob + 4 * b),
+                                     e - b, P(pb + 4 * b), P(lb + 4 * b))
+            ps5._check(rc, "ps5_sort_device")
+    plain_pass()
+    lib.ps5_device_synchronize()
+    t = time.perf_counter()
+    plain_pass()
+    lib.ps5_device_synchronize()
+    plain_s = time.perf_counter() - t
+    pperm = d_pp.download(np.uint32, n)
+    plcp = d_pl.download(np.uint32, n)
+    same = bool(np.array_equal(perm, pperm))
+    for k in range(args.sets):
+        b, e = int(sf[k]), int(sf[k + 1])
+        if e - b > 1:
+            same &= bool(np.array_equal(lcp[b + 1:e], plcp[b + 1:e]))
+    # oracle check of every 16th set
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ps5_oracle
+    bad = 0
+    for k in range(0, args.sets, 16):
+        b, e = int(sf[k]), int(sf[k + 1])
+        if e > b and ps5_oracle.verify(arena, offs[b:e], perm[b:e], lcp[b:e])[0] != 0:
+            bad += 1
+    out = {"metric": "batched vs plain sorts/s", "sets": args.sets, "strings": n, "arena_bytes": int(arena.size),
+           "batched_sorts_per_s": round(args.sets / batched_s, 2), "batched_ms": round(batched_s * 1e3, 3),
+           "plain_sorts_per_s": round(args.sets / plain_s, 2), "plain_ms": round(plain_s * 1e3, 3),
+           "speedup": round(plain_s / batched_s, 2), "batched_chars_gbs": round(arena.size / batched_s / 1e9, 2),
+           "identical_to_plain": same, "oracle_checked_sets": (args.sets + 15) // 16, "oracle_failures": bad,
+           "config": {"workload": f"{args.sets} sets, strings log-uniform [{args.count_lo}, {args.count_hi}], "
+                                  "length U[4,48], alphabet ACGT/digits/bytes 33..126, seed 0x5EED0004"}}
+    print(json.dumps(out), flush=True)
+
+
+def bench_serve(ps5, args):
+    """Poisson arrivals: continuous-batching server vs FIFO of plain sorts."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(0x5EED0005)
+    sizes = np.exp(rng.uniform(np.log(args.req_lo), np.log(args.req_hi), args.requests)).astype(np.int64)
+    t0 = time.time()
+    with ThreadPoolExecutor(16) as ex:
+        reqs = list(ex.map(lambda i: ps5.gen_prefixed(0x5EED0005 * 1000 + i, int(sizes[i])), range(args.requests)))
+    total = sum(int(a.size) for a, _ in reqs)
+    print(f"[bench] serve: {args.requests} requests, {total} bytes generated in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    ctx = ps5.Context(0)
+    max_b = max(int(a.size) for a, _ in reqs)
+    max_n = max(len(o) for _, o in reqs)
+    d_a, d_o = ps5.DeviceBuffer(max_b), ps5.DeviceBuffer(4 * max_n)
+    d_p, d_l = ps5.DeviceBuffer(4 * max_n), ps5.DeviceBuffer(4 * max_n)
+
+    def plain_sort(a, o):
+        """the unbatched path: upload, one device sort, download (buffers reused)"""
+        d_a.upload(a)
+        d_o.upload(o)
+        ctx.sort_device(d_a, int(a.size), d_o, len(o), d_p, d_l)
+        return d_p.download(np.uint32, len(o)), d_l.download(np.uint32, len(o))
+
+    for a, o in reqs[:3]:
+        plain_sort(a, o)
+    t = time.perf_counter()
+    alone = [plain_sort(a, o) for a, o in reqs]
+    plain_busy = time.perf_counter() - t
+    rate = args.load * args.requests / plain_busy           # requests per second offered
+    gaps = rng.exponential(1.0 / rate, args.requests)
+    arrivals = np.cumsum(gaps)
+
+    def run_fifo():
+        lat = []
+        start = time.perf_counter()
+        for i, (a, o) in enumerate(reqs):
+            now = time.perf_counter() - start
+            if now < arrivals[i]:
+                time.sleep(arrivals[i] - now)
+            plain_sort(a, o)
+            lat.append((time.perf_counter() - start - arrivals[i]) * 1e3)
+        return lat, time.perf_counter() - start
+
+    def run_served():
+        srv = ps5.Server(0, hbm_budget=args.budget_gb << 30, slots=args.slots)
+        ids = []
+        start = time.perf_counter()
+        for i, (a, o) in enumerate(reqs):
+            now = time.perf_counter() - start
+            if now < arrivals[i]:
+                time.sleep(arrivals[i] - now)
+            ids.append(srv.submit(a, o))
+        res = [srv.wait(i) for i in ids]
+        wall = time.perf_counter() - start
+        st = srv.stats()
+        srv.close()
+        return res, st, wall
+
+    fifo_lat, fifo_wall = run_fifo()
+    res, st, served_wall = run_served()
+    same = all(np.array_equal(p1, p2) and np.array_equal(l1[1:], l2[1:])
+               for (p1, l1), (p2, l2, _) in zip(alone, res))
+    served_lat = [ms for _, _, ms in res]
+    out = {"metric": "served vs plain p50/p99 latency (ms) and throughput", "requests": args.requests,
+           "offered_rate_per_s": round(rate, 2), "offered_load": args.load, "bytes": total,
+           "served": {"p50_ms": round(float(np.percentile(served_lat, 50)), 3),
+                      "p99_ms": round(float(np.percentile(served_lat, 99)), 3),
+                      "throughput_gbs": round(total / served_wall / 1e9, 3), "batches": st["batches"],
+                      "slots": args.slots, "hbm_budget_gb": args.budget_gb},
+           "plain_fifo": {"p50_ms": round(float(np.percentile(fifo_lat, 50)), 3),
+                          "p99_ms": round(float(np.percentile(fifo_lat, 99)), 3),
+                          "throughput_gbs": round(total / fifo_wall / 1e9, 3)},
+           "identical_to_alone": bool(same),
+           "config": {"workload": f"{args.requests} requests, size log-uniform [{args.req_lo}, {args.req_hi}] bytes, "
+                                  "one of 1000 prefixes (Zipf 1.1) + 10..60 random chars, Poisson arrivals at "
+                                  f"{args.load} of the plain path's capacity, seed 0x5EED0005",
+                      "latency": "submit (host arrays) to results in host memory, incl. PCIe both ways"}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"])
+    ap.add_argument("--sets", type=int, default=4096)
+    ap.add_argument("--count-lo", type=int, default=1000)
+    ap.add_argument("--count-hi", type=int, default=2_000_000)
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--req-lo", type=int, default=10_000)
+    ap.add_argument("--req-hi", type=int, default=256 << 20)
+    ap.add_argument("--load", type=float, default=0.7)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -76,6 +256,10 @@ def main():
         sys.exit(2)
 
     ps5 = load_ps5()
+    if args.mode == "batch":
+        return bench_batch(ps5, args)
+    if args.mode == "serve":
+        return bench_serve(ps5, args)
     t0 = time.time()
     arena, offs = ps5.workload(args.workload)
     n = len(offs)

@@ -60,9 +60,20 @@ void mi355x_ps5_sort_lcp(unsigned char** strings, uintptr_t* lcp, size_t n);
  * is left as it was.  abort()s on failure. */
 void mi355x_ps5_sort_out(unsigned char** strings, unsigned char** output, size_t n);
 
+/* Behind the Contest func_lcp_cache_type signature (contest.hpp:90-91): sorts,
+ * fills lcp[1..n-1] and the distinguishing characters cache[i] = s_i[lcp[i]]
+ * for i >= 1 (0 for equal neighbours), the output of
+ * StringShadowLcpCacheOutPtr (stringptr.hpp:860-969) checked by
+ * verify_lcp_cache (stringptr.hpp:977-1031).  lcp[0], cache[0] untouched.
+ * abort()s on failure. */
+void mi355x_ps5_sort_lcp_cache(unsigned char** strings, uintptr_t* lcp, unsigned char* cache, size_t n);
+
 /* Status-returning form of the three above: lcp and output may be NULL. */
 int ps5_sort_strings(unsigned char** strings, size_t n, uintptr_t* lcp,
                      unsigned char** output);
+/* ... and with the distinguishing characters (cache requires lcp). */
+int ps5_sort_strings_cache(unsigned char** strings, size_t n, uintptr_t* lcp,
+                           unsigned char* cache, unsigned char** output);
 
 /* ---- device-resident interface ----------------------------------------- */
 
@@ -87,6 +98,16 @@ void* ps5_ctx_stream(ps5_ctx* ctx);
 int ps5_sort_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
                     const uint32_t* d_offs, uint64_t n,
                     uint32_t* d_perm, uint32_t* d_lcp);
+
+/* Distinguishing characters of a finished sort (inputs as ps5_sort_device,
+ * d_perm and d_lcp its results): d_cache[i] = s_i[lcp[i]] for i >= 1, the
+ * first character of the i-th smallest string that its predecessor does not
+ * share, 0 when the two are equal; d_cache[0] untouched.  Replaces the
+ * character cache of StringShadowLcpCacheOutPtr (src/tools/stringptr.hpp:860-969,
+ * set by sample_sort_lcp, src/parallel/bingmann-parallel_sample_sort.hpp:366,385). */
+int ps5_dist_chars_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
+                          const uint32_t* d_offs, uint64_t n,
+                          const uint32_t* d_perm, const uint32_t* d_lcp, uint8_t* d_cache);
 
 /* Batch: sort nsets independent string sets in one pass over the device.
  * Set s is the strings with input indices [set_first[s], set_first[s+1])
@@ -126,6 +147,31 @@ int ps5_dev_copy(void* dst, const void* src, uint64_t bytes);
 int ps5_sort_device_timed(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
                           const uint32_t* d_offs, uint64_t n,
                           uint32_t* d_perm, uint32_t* d_lcp, double* ms);
+
+/* ---- serving: continuous batching under an HBM budget ------------------ */
+
+typedef struct ps5_server ps5_server;
+
+/* `slots` workers each own a sort context (HIP stream + device buffers).  A
+ * free slot takes every waiting request that fits under hbm_budget bytes of
+ * staged data (in arrival order; a request larger than the budget runs alone)
+ * and at most max_batch_strings strings (0: no limit), and sorts them as one
+ * batch; slots run concurrently on the device.  No counterpart in the
+ * reference (it sorts one set per call). */
+int ps5_server_create(ps5_server** out, int device, uint64_t hbm_budget, uint32_t slots,
+                      uint64_t max_batch_strings);
+void ps5_server_destroy(ps5_server* srv);
+/* Submit host strings (arena + uint32 offsets, caller keeps them alive until
+ * ps5_server_wait returns); results go to perm[n] and lcp[n] (lcp may be
+ * NULL, lcp[0] untouched).  Thread-safe.  Returns a request id (0 on error). */
+uint64_t ps5_server_submit(ps5_server* srv, const uint8_t* arena, uint64_t arena_bytes,
+                           const uint32_t* offs, uint64_t n, uint32_t* perm, uint32_t* lcp);
+/* Block until request `id` is done; its latency from submit to completion. */
+int ps5_server_wait(ps5_server* srv, uint64_t id, double* latency_ms);
+/* Latency percentiles over completed requests, batches run, and throughput
+ * (arena bytes of completed requests / time from first submit to last finish). */
+int ps5_server_stats(ps5_server* srv, double* p50_ms, double* p99_ms, uint64_t* completed,
+                     uint64_t* batches, double* throughput_gbs);
 
 /* ---- synthetic inputs (seeded, host memory) ---------------------------- */
 

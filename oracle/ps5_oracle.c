@@ -174,6 +174,25 @@ uint64_t ps5o_verify(const uint8_t* arena, const uint64_t* offs, uint64_t n,
     return bad + 1;
 }
 
+/* Distinguishing characters of a verified output, as verify_lcp_cache
+ * checks them (stringptr.hpp:1015-1021): cache[i] = get_char(s_i, lcp[i])
+ * for i >= 1 -- the character of s_i right after the prefix it shares with
+ * s_{i-1} (NUL for equal strings).  Returns 1 + the first i whose cache byte
+ * differs from the given one when `check` is non-zero, else fills cache. */
+uint64_t ps5o_dist_chars(const uint8_t* arena, const uint64_t* offs, uint64_t n,
+                         const uint32_t* perm, const uint32_t* lcp, uint8_t* cache, int check)
+{
+    for (uint64_t i = 1; i < n; ++i) {
+        const uint8_t c = arena[offs[perm[i]] + lcp[i]];
+        if (check) {
+            if (cache[i] != c) return i + 1;
+        } else {
+            cache[i] = c;
+        }
+    }
+    return 0;
+}
+
 static int cmp_u32(const void* x, const void* y)
 {
     uint32_t a = *(const uint32_t*)x, b = *(const uint32_t*)y;

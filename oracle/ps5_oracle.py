@@ -37,6 +37,8 @@ def load():
     lib.ps5o_lcp_key_depth.argtypes = [_U64]
     lib.ps5o_brute_sort.restype = None
     lib.ps5o_brute_sort.argtypes = [_P, _P, _U64, _P, _P]
+    lib.ps5o_dist_chars.restype = _U64
+    lib.ps5o_dist_chars.argtypes = [_P, _P, _U64, _P, _P, _P, ctypes.c_int]
     lib.ps5o_verify.restype = _U64
     lib.ps5o_verify.argtypes = [_P, _P, _U64, _P, _P, ctypes.c_char_p, _U64]
     lib.ps5o_canonicalize.restype = None
@@ -97,6 +99,17 @@ def verify(arena, offs, perm, lcp=None):
         lp = _p(lcp)
     r = load().ps5o_verify(_p(arena), _p(o), n, _p(perm), lp, msg, 512)
     return int(r), msg.value.decode()
+
+
+def dist_chars(arena, offs, perm, lcp):
+    """Distinguishing characters cache[i] = s_perm[i][lcp[i]], i >= 1 (cache[0] = 0)."""
+    n = len(offs)
+    o = offs64(offs)
+    perm = np.ascontiguousarray(perm, np.uint32)
+    lcp = np.ascontiguousarray(lcp, np.uint32)
+    cache = np.zeros(max(1, n), np.uint8)
+    load().ps5o_dist_chars(_p(arena), _p(o), n, _p(perm), _p(lcp), _p(cache), 0)
+    return cache[:n]
 
 
 def time_ps5(arena, offs, threads=0, with_lcp=False):

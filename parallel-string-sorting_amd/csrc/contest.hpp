@@ -2,7 +2,8 @@
 //
 // Mirrors src/tools/contest.hpp:38-186 of the reference: contestants register
 // themselves at static-initialisation time with a name, a description and a
-// function of one of the Contest signatures (func_type / func_lcp_type), and
+// function of one of the Contest signatures (func_type / func_lcp_type /
+// func_lcp_cache_type, src/tools/contest.hpp:88-91), and
 // the runner selects them by substring (-a) or exact name (-A), as
 // gopt_algorithm_select does (src/psstest.cpp:171-199).
 #pragma once
@@ -17,17 +18,21 @@ class Contestant {
 public:
     typedef void (*func_type)(unsigned char** strings, size_t n);
     typedef void (*func_lcp_type)(unsigned char** strings, uintptr_t* lcp, size_t n);
+    typedef void (*func_lcp_cache_type)(unsigned char** strings, uintptr_t* lcp, unsigned char* cache, size_t n);
 
     const char* m_algoname;
     const char* m_description;
     func_type m_run_func = nullptr;
     func_lcp_type m_run_lcp_func = nullptr;
+    func_lcp_cache_type m_run_lcp_cache_func = nullptr;
     bool m_parallel;
 
     Contestant(func_type f, const char* name, const char* desc, bool parallel);
     Contestant(func_lcp_type f, const char* name, const char* desc, bool parallel);
+    Contestant(func_lcp_cache_type f, const char* name, const char* desc, bool parallel);
 
     bool is_lcp_func() const { return m_run_lcp_func != nullptr; }
+    bool is_lcp_cache_func() const { return m_run_lcp_cache_func != nullptr; }
     bool is_parallel() const { return m_parallel; }
 };
 
@@ -47,6 +52,12 @@ inline Contestant::Contestant(func_type f, const char* name, const char* desc, b
 
 inline Contestant::Contestant(func_lcp_type f, const char* name, const char* desc, bool parallel)
     : m_algoname(name), m_description(desc), m_run_lcp_func(f), m_parallel(parallel)
+{
+    getContestSingleton()->register_contestant(this);
+}
+
+inline Contestant::Contestant(func_lcp_cache_type f, const char* name, const char* desc, bool parallel)
+    : m_algoname(name), m_description(desc), m_run_lcp_cache_func(f), m_parallel(parallel)
 {
     getContestSingleton()->register_contestant(this);
 }

@@ -4,6 +4,8 @@
 //   parallel_sample_sortBTCUI      src/parallel/bingmann-parallel_sample_sort.cpp:31-37
 //   parallel_sample_sortBTCUI_out  src/parallel/bingmann-parallel_sample_sort.cpp:44-62
 //   parallel_sample_sort_lcp_base  src/parallel/bingmann-parallel_sample_sort.hpp:1733-1748
+//   the LCP + character cache output of StringShadowLcpCacheOutPtr
+//                                  src/tools/stringptr.hpp:860-969
 // behind the signatures of src/tools/contest.hpp:88-91.
 //
 // The strings are staged to the device as one character arena plus uint32
@@ -34,6 +36,7 @@ struct Staging {
     uint32_t* d_offs = nullptr;
     uint32_t* d_perm = nullptr;
     uint32_t* d_lcp = nullptr;
+    uint8_t* d_cache = nullptr;
     uint64_t n_cap = 0;
 } g_st;
 
@@ -76,11 +79,16 @@ int ensure_staging(uint64_t arena_bytes, uint64_t n)
             *q = nullptr;
         }
         g_st.n_cap = 0;
+        if (g_st.d_cache) (void)hipFree(g_st.d_cache);
+        g_st.d_cache = nullptr;
         for (uint32_t** q : { &g_st.d_offs, &g_st.d_perm, &g_st.d_lcp }) {
             void* p = nullptr;
             if (ps5_dev_malloc(&p, n * sizeof(uint32_t))) return PS5_ERR_NOMEM;
             *q = (uint32_t*)p;
         }
+        void* p = nullptr;
+        if (ps5_dev_malloc(&p, n)) return PS5_ERR_NOMEM;
+        g_st.d_cache = (uint8_t*)p;
         g_st.n_cap = n;
     }
     return PS5_OK;
@@ -88,10 +96,11 @@ int ensure_staging(uint64_t arena_bytes, uint64_t n)
 
 }  // namespace
 
-extern "C" int ps5_sort_strings(unsigned char** strings, size_t n, uintptr_t* lcp, unsigned char** output)
+extern "C" int ps5_sort_strings_cache(unsigned char** strings, size_t n, uintptr_t* lcp, unsigned char* cache,
+                                      unsigned char** output)
 {
     if (n == 0) return PS5_OK;
-    if (!strings) return PS5_ERR_ARG;
+    if (!strings || (cache && !lcp)) return PS5_ERR_ARG;
     if (n >= 0xFFFFFF00ull) return PS5_ERR_ARG;
     std::lock_guard<std::mutex> lock(g_mu);
     int rc;
@@ -141,6 +150,12 @@ extern "C" int ps5_sort_strings(unsigned char** strings, size_t n, uintptr_t* lc
         if ((rc = ps5_memcpy_d2h(l.data(), g_st.d_lcp, n * sizeof(uint32_t)))) return rc;
         for (uint64_t i = 1; i < n; ++i) lcp[i] = l[i];
     }
+    if (cache && n > 1) {
+        if ((rc = ps5_dist_chars_device(g_ctx, g_st.d_arena, alen, g_st.d_offs, n, g_st.d_perm, g_st.d_lcp,
+                                        g_st.d_cache)))
+            return rc;
+        if ((rc = ps5_memcpy_d2h(cache + 1, g_st.d_cache + 1, n - 1))) return rc;
+    }
     unsigned char** out = output ? output : strings;
     if (out == strings) {
         std::vector<unsigned char*> orig(strings, strings + n);
@@ -149,6 +164,11 @@ extern "C" int ps5_sort_strings(unsigned char** strings, size_t n, uintptr_t* lc
         for (uint64_t i = 0; i < n; ++i) out[i] = strings[perm[i]];
     }
     return PS5_OK;
+}
+
+extern "C" int ps5_sort_strings(unsigned char** strings, size_t n, uintptr_t* lcp, unsigned char** output)
+{
+    return ps5_sort_strings_cache(strings, n, lcp, nullptr, output);
 }
 
 static void die_on(int rc, const char* who)
@@ -171,4 +191,9 @@ extern "C" void mi355x_ps5_sort_lcp(unsigned char** strings, uintptr_t* lcp, siz
 extern "C" void mi355x_ps5_sort_out(unsigned char** strings, unsigned char** output, size_t n)
 {
     die_on(ps5_sort_strings(strings, n, nullptr, output), "mi355x_ps5_sort_out");
+}
+
+extern "C" void mi355x_ps5_sort_lcp_cache(unsigned char** strings, uintptr_t* lcp, unsigned char* cache, size_t n)
+{
+    die_on(ps5_sort_strings_cache(strings, n, lcp, cache, nullptr), "mi355x_ps5_sort_lcp_cache");
 }

@@ -124,17 +124,32 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     const uint32_t w = wave_id(), lane = lane_id();
     const uint32_t cb = t * kTile + w * kWaveChunk;
     const uint32_t ce = (item_ok(seg, B) && t < D.T) ? min(seg.size, cb + kWaveChunk) : 0u;
-    for (uint32_t g = cb; g < ce; g += 64) {
-        const uint32_t p = g + lane;
-        const bool act = p < ce;
-        uint32_t b = kNoBucket;
-        if (act) b = classify_key(K[seg.begin + p], s_tree, k);
-        const uint64_t peers = match_bucket(b);
-        const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
-        if (act) {
-            const uint32_t old = s_hist[w][b];
-            if (r == 0) s_hist[w][b] = old + (uint32_t)__popcll(peers);
-            B.bktrank[seg.begin + p] = (b << kRankBits) | (old + r);
+    // two 64-key groups per iteration: both loads are in flight before the
+    // tree descents, then the groups are ranked in order (stable)
+    for (uint32_t g = cb; g < ce; g += 128) {
+        const uint32_t p0 = g + lane, p1 = g + 64 + lane;
+        const bool a0 = p0 < ce, a1 = p1 < ce;
+        const key64 k0 = a0 ? K[seg.begin + p0] : 0ull;
+        const key64 k1 = a1 ? K[seg.begin + p1] : 0ull;
+        const uint32_t b0 = a0 ? classify_key(k0, s_tree, k) : kNoBucket;
+        const uint32_t b1 = a1 ? classify_key(k1, s_tree, k) : kNoBucket;
+        {
+            const uint64_t peers = match_bucket(b0);
+            const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
+            if (a0) {
+                const uint32_t old = s_hist[w][b0];
+                if (r == 0) s_hist[w][b0] = old + (uint32_t)__popcll(peers);
+                B.bktrank[seg.begin + p0] = (b0 << kRankBits) | (old + r);
+            }
+        }
+        {
+            const uint64_t peers = match_bucket(b1);
+            const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
+            if (a1) {
+                const uint32_t old = s_hist[w][b1];
+                if (r == 0) s_hist[w][b1] = old + (uint32_t)__popcll(peers);
+                B.bktrank[seg.begin + p1] = (b1 << kRankBits) | (old + r);
+            }
         }
     }
     __syncthreads();
@@ -686,6 +701,23 @@ __global__ __launch_bounds__(kWG) void k_localize(uint32_t* perm, uint32_t n, co
     perm[pos] -= (uint32_t)set_first[lo];
 }
 
+// Distinguishing characters of a sorted order: cache[i] = s_i[lcp[i]] for
+// i >= 1, the first character in which the i-th string differs from its
+// predecessor, NUL when they are equal (set_cache in sample_sort_lcp,
+// bingmann-parallel_sample_sort.hpp:366,385; fill_lcp, stringptr.hpp:950-958;
+// checked by verify_lcp_cache, stringptr.hpp:1015-1021).  lcp[i] never
+// exceeds the length of s_i, so the byte read is inside the string.
+__global__ __launch_bounds__(kWG) void k_dist_chars(const uint8_t* arena, const uint32_t* offs,
+                                                    const uint32_t* perm, const uint32_t* lcp, uint32_t n,
+                                                    uint8_t* cache, uint32_t* cnt, uint64_t alen)
+{
+    const uint32_t i = blockIdx.x * kWG + threadIdx.x;
+    if (i == 0 || i >= n) return;
+    const uint64_t p = (uint64_t)offs[perm[i]] + lcp[i];
+    if (p >= alen) { flag_error(cnt, ERR_BOUNDS); return; }
+    cache[i] = arena[p];
+}
+
 // plain copy of 16-byte words (calibrates the HBM byte counters)
 __global__ __launch_bounds__(kWG) void k_copy16(const uint4* src, uint4* dst, uint64_t n16)
 {
@@ -779,11 +811,14 @@ struct ps5_ctx {
     LargeDesc* h_desc = nullptr;     // pinned
     TileRef* h_tiles = nullptr;      // pinned
     uint64_t tiles_cap = 0;
-    key64* trees = nullptr;
+    key64* trees = nullptr;          // 2^kMaxTreebits splitter slots per large segment
+    uint64_t trees_cap = 0;
     uint32_t* hist = nullptr;
     uint64_t hist_cap = 0;
     uint32_t* part = nullptr;
     uint64_t part_cap = 0;
+    uint64_t* d_sets = nullptr;      // set boundaries of a batch
+    uint64_t sets_cap = 0;
     unsigned long long* d_sums = nullptr;   // profiling: elements per kernel class
     unsigned long long* h_sums = nullptr;
     // tuning
@@ -821,6 +856,35 @@ void free_buffers(ps5_ctx* c)
     c->hist_cap = 0;
     c->part_cap = 0;
     c->tiles_cap = 0;
+    c->trees_cap = 0;
+}
+
+// Splitter trees for `nlarge` segments and `hl` histogram counters of one
+// level.  Called between levels, when the stream has drained (the level's
+// counts were just read back), so old buffers are idle.
+int grow_level(ps5_ctx* c, uint32_t nlarge, uint64_t hl)
+{
+    if (nlarge > c->trees_cap) {
+        if (c->trees) (void)hipFree(c->trees);
+        c->trees = nullptr;
+        c->trees_cap = 0;
+        const uint64_t want = std::max<uint64_t>(nlarge + nlarge / 2, 64);
+        HIPCHK(hipMalloc(&c->trees, want * (1u << kMaxTreebits) * sizeof(key64)));
+        c->trees_cap = want;
+    }
+    if (hl > c->hist_cap) {
+        if (c->hist) (void)hipFree(c->hist);
+        if (c->part) (void)hipFree(c->part);
+        c->hist = nullptr;
+        c->part = nullptr;
+        c->hist_cap = c->part_cap = 0;
+        const uint64_t want = std::min<uint64_t>(std::max<uint64_t>(hl + hl / 2, 1u << 20), 0xFFFFFFFFull);
+        HIPCHK(hipMalloc(&c->hist, want * sizeof(uint32_t)));
+        c->part_cap = want / kScanBlock + 2;
+        HIPCHK(hipMalloc(&c->part, c->part_cap * sizeof(uint32_t)));
+        c->hist_cap = want;
+    }
+    return PS5_OK;
 }
 
 int ensure_capacity(ps5_ctx* c, uint64_t n)
@@ -837,7 +901,8 @@ int ensure_capacity(ps5_ctx* c, uint64_t n)
     c->cap_large = (uint32_t)(cap / kBlkT + 16);
     c->cap_block = (uint32_t)(cap / kTinyT + 16);
     c->cap_tiny = (uint32_t)(cap + 16);
-    c->cap_run = (uint32_t)(cap + 16);
+    // an equal-key run of m > kTinyT strings yields ceil(m / kRunChunk) <= m / 64 chunks
+    c->cap_run = (uint32_t)(cap / 64 + 16);
     for (int i = 0; i < 2; ++i) {
         HIPCHK(hipMalloc(&c->l_round[i], (uint64_t)c->cap_large * sizeof(Item)));
         HIPCHK(hipMalloc(&c->l_lvl[i], (uint64_t)c->cap_large * sizeof(Item)));
@@ -845,13 +910,9 @@ int ensure_capacity(ps5_ctx* c, uint64_t n)
         HIPCHK(hipMalloc(&c->l_block[i], (uint64_t)c->cap_block * sizeof(Item)));
         HIPCHK(hipMalloc(&c->l_run[i], (uint64_t)c->cap_run * sizeof(Item)));
     }
-    // every large segment has at most size/kTile + 1 tiles
+    // every large segment has at most size/kTile + 1 tiles; the splitter
+    // trees and histograms of a level grow on demand (grow_level)
     c->tiles_cap = cap / kTile + c->cap_large + 1;
-    c->hist_cap = (uint64_t)kMaxBkt * kWaves * c->tiles_cap;
-    HIPCHK(hipMalloc(&c->hist, c->hist_cap * sizeof(uint32_t)));
-    c->part_cap = c->hist_cap / kScanBlock + 2;
-    HIPCHK(hipMalloc(&c->part, c->part_cap * sizeof(uint32_t)));
-    HIPCHK(hipMalloc(&c->trees, (uint64_t)c->cap_large * 1024 * sizeof(key64)));
     HIPCHK(hipMalloc(&c->d_desc, (uint64_t)c->cap_large * sizeof(LargeDesc)));
     HIPCHK(hipMalloc(&c->d_tiles, c->tiles_cap * sizeof(TileRef)));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_large), (uint64_t)c->cap_large * sizeof(Item)));
@@ -985,8 +1046,9 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         ntiles += T;
         hl += (uint64_t)((2u << D.k) - 1) * T * kWaves;
         elems += seg.size;
-        if (hl > c->hist_cap || hl > 0xFFFFFFFFull) return set_err(PS5_ERR_CAPACITY, "histogram capacity");
+        if (hl > 0xFFFFFFFFull) return set_err(PS5_ERR_CAPACITY, "histogram capacity");
     }
+    if (int rc = grow_level(c, nlarge, hl)) return rc;
     HIPCHK(hipMemcpyAsync(c->d_desc, c->h_desc, nlarge * sizeof(LargeDesc), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(c->d_tiles, c->h_tiles, ntiles * sizeof(TileRef), hipMemcpyHostToDevice, st));
     const uint32_t hl32 = (uint32_t)hl;
@@ -1212,6 +1274,7 @@ void ps5_ctx_destroy(ps5_ctx* c)
     free_buffers(c);
     if (c->d_cnt) (void)hipFree(c->d_cnt);
     if (c->d_sums) (void)hipFree(c->d_sums);
+    if (c->d_sets) (void)hipFree(c->d_sets);
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
     if (c->h_sums) (void)hipHostFree(c->h_sums);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
@@ -1249,6 +1312,22 @@ int ps5_sort_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, co
     return rc;
 }
 
+int ps5_dist_chars_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, const uint32_t* d_offs,
+                          uint64_t n, const uint32_t* d_perm, const uint32_t* d_lcp, uint8_t* d_cache)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
+    if (n < 2) return PS5_OK;
+    if (!d_arena || !d_offs || !d_perm || !d_lcp || !d_cache) return set_err(PS5_ERR_ARG, "NULL device pointer");
+    if (n >= 0xFFFFFF00ull) return set_err(PS5_ERR_ARG, "n too large (limit 2^32-256 strings)");
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemsetAsync(c->d_cnt, 0, C_NCNT * sizeof(uint32_t), c->stream));
+    hipLaunchKernelGGL(k_dist_chars, dim3(div_up(n, kWG)), dim3(kWG), 0, c->stream, d_arena, d_offs, d_perm, d_lcp,
+                       (uint32_t)n, d_cache, c->d_cnt, arena_bytes);
+    HIPCHK(hipGetLastError());
+    int rc = readback(c, nullptr, 0);
+    return rc;
+}
+
 int ps5_sort_batch_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, const uint64_t* d_offs,
                           const uint64_t* set_first, uint32_t nsets, uint32_t* d_perm, uint32_t* d_lcp)
 {
@@ -1264,16 +1343,19 @@ int ps5_sort_batch_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_byt
     HIPCHK(hipSetDevice(c->device));
     int rc = run_sort(c, d_arena, arena_bytes, nullptr, d_offs, (uint32_t)n, d_perm, d_lcp, set_first, nsets);
     if (rc == PS5_OK) {
-        uint64_t* d_first = nullptr;
-        HIPCHK(hipMalloc(&d_first, (nsets + 1) * sizeof(uint64_t)));
-        hipError_t e = hipMemcpy(d_first, set_first, (nsets + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(k_localize, dim3(div_up(n, kWG)), dim3(kWG), 0, c->stream, d_perm, (uint32_t)n, d_first,
-                               nsets);
-            e = hipStreamSynchronize(c->stream);
+        if (c->sets_cap < (uint64_t)nsets + 1) {
+            if (c->d_sets) (void)hipFree(c->d_sets);
+            c->d_sets = nullptr;
+            c->sets_cap = 0;
+            HIPCHK(hipMalloc(&c->d_sets, ((uint64_t)nsets + 1) * sizeof(uint64_t)));
+            c->sets_cap = (uint64_t)nsets + 1;
         }
-        (void)hipFree(d_first);
-        if (e != hipSuccess) return set_err(PS5_ERR_HIP, "batch localize: %s", hipGetErrorString(e));
+        HIPCHK(hipMemcpyAsync(c->d_sets, set_first, ((uint64_t)nsets + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
+                              c->stream));
+        hipLaunchKernelGGL(k_localize, dim3(div_up(n, kWG)), dim3(kWG), 0, c->stream, d_perm, (uint32_t)n, c->d_sets,
+                           nsets);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
     } else {
         (void)hipStreamSynchronize(c->stream);
         if (c->profile) collect_events(c);

@@ -10,7 +10,8 @@
 // src/tools/input.hpp:225-300).  -s limits the input size in bytes (K/M/G
 // suffixes).  Each run times the sort call only (CLOCK_MONOTONIC wall clock,
 // src/psstest.cpp:593-603), then checks order, permutation and (for LCP
-// contestants) every LCP value; the check is fatal.
+// contestants) every LCP value and (for LCP-cache contestants) every
+// distinguishing character, as verify_lcp_cache (src/tools/stringptr.hpp:977-1031).
 #include <algorithm>
 #include <cerrno>
 #include <cinttypes>
@@ -53,6 +54,8 @@ PSS_CONTESTANT_PARALLEL(mi355x_ps5_sort, "mi355x/parallel_sample_sortBTCU2",
                         "pS5 on MI355X: splitter tree, key cache, LDS blocks (gfx950)")
 PSS_CONTESTANT_PARALLEL(mi355x_ps5_sort_lcp, "mi355x/parallel_sample_sortBTCU2_lcp",
                         "pS5 on MI355X with LCP array output (gfx950)")
+PSS_CONTESTANT_PARALLEL(mi355x_ps5_sort_lcp_cache, "mi355x/parallel_sample_sortBTCU2_lcp_cache",
+                        "pS5 on MI355X with LCP array and distinguishing characters (gfx950)")
 PSS_CONTESTANT_PARALLEL(mi355x_parallel_sample_sortBTCU2_out, "mi355x/parallel_sample_sortBTCU2_out",
                         "pS5 on MI355X, separate output array (gfx950)")
 PSS_CONTESTANT(std_sort_strcmp, "std/stable_sort_strcmp", "std::stable_sort with strcmp (host, one thread)")
@@ -140,7 +143,8 @@ static bool load_input(const std::string& path, uint64_t size, Input& in)
 
 // ---- checking --------------------------------------------------------------
 
-static bool check(const Input& in, unsigned char** s, const uintptr_t* lcp, size_t n, std::string& why)
+static bool check(const Input& in, unsigned char** s, const uintptr_t* lcp, const unsigned char* cache, size_t n,
+                  std::string& why)
 {
     // permutation: every input string exactly once (by address)
     std::vector<const unsigned char*> a(s, s + n), b(n);
@@ -154,6 +158,7 @@ static bool check(const Input& in, unsigned char** s, const uintptr_t* lcp, size
         while (x[h] && x[h] == y[h]) ++h;
         if (x[h] > y[h]) { why = "invalid order at " + std::to_string(i); return false; }
         if (lcp && lcp[i] != h) { why = "wrong lcp at " + std::to_string(i); return false; }
+        if (cache && cache[i] != y[h]) { why = "wrong cache at " + std::to_string(i); return false; }
     }
     return true;
 }
@@ -200,21 +205,30 @@ int main(int argc, char** argv)
         for (int r = 0; r < repeats; ++r) {
             std::vector<unsigned char*> strings(n);
             for (size_t i = 0; i < n; ++i) strings[i] = in.data.data() + in.offs[i];
+            const bool with_lcp = c->is_lcp_func() || c->is_lcp_cache_func();
             std::vector<uintptr_t> lcp;
-            if (c->is_lcp_func()) {
+            std::vector<unsigned char> cache;
+            if (with_lcp) {
                 lcp.assign(n, (uintptr_t)-1);
                 if (n) lcp[0] = 42;   // must stay unchanged (src/psstest.cpp:578-584)
+            }
+            if (c->is_lcp_cache_func()) {   // src/psstest.cpp:586-591
+                cache.assign(n, (unsigned char)-1);
+                if (n) cache[0] = 0;
             }
             timespec t0, t1;
             clock_gettime(CLOCK_MONOTONIC, &t0);
             if (c->is_lcp_func()) c->m_run_lcp_func(strings.data(), lcp.data(), n);
+            else if (c->is_lcp_cache_func()) c->m_run_lcp_cache_func(strings.data(), lcp.data(), cache.data(), n);
             else c->m_run_func(strings.data(), n);
             clock_gettime(CLOCK_MONOTONIC, &t1);
             const double secs = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
             std::string status = "ok", why;
             if (!no_check) {
-                if (!check(in, strings.data(), c->is_lcp_func() ? lcp.data() : nullptr, n, why)) status = "failed";
-                if (c->is_lcp_func() && n && lcp[0] != 42) { status = "failed"; why = "lcp[0] was written"; }
+                if (!check(in, strings.data(), with_lcp ? lcp.data() : nullptr,
+                           c->is_lcp_cache_func() ? cache.data() : nullptr, n, why))
+                    status = "failed";
+                if (with_lcp && n && lcp[0] != 42) { status = "failed"; why = "lcp[0] was written"; }
             } else {
                 status = "unchecked";
             }

@@ -18,7 +18,8 @@ PS5_OK = 0
 # every symbol include/ps5_mi355x.h declares
 EXPORTS = [
     "ps5_last_error",
-    "mi355x_ps5_sort", "mi355x_ps5_sort_lcp", "mi355x_ps5_sort_out", "ps5_sort_strings",
+    "mi355x_ps5_sort", "mi355x_ps5_sort_lcp", "mi355x_ps5_sort_out", "mi355x_ps5_sort_lcp_cache",
+    "ps5_sort_strings", "ps5_sort_strings_cache", "ps5_dist_chars_device",
     "ps5_ctx_create", "ps5_ctx_destroy", "ps5_ctx_reserve", "ps5_ctx_stream",
     "ps5_sort_device", "ps5_ctx_set_profile", "ps5_ctx_kernel_count", "ps5_ctx_kernel_stat",
     "ps5_ctx_work_stats",
@@ -27,6 +28,7 @@ EXPORTS = [
     "ps5_sort_device_timed",
     "ps5_gen_uniform", "ps5_gen_windows", "ps5_gen_reference_random", "ps5_gen_prefixed",
     "ps5_sort_batch_device", "ps5_gen_batch_plan", "ps5_gen_batch_fill",
+    "ps5_server_create", "ps5_server_destroy", "ps5_server_submit", "ps5_server_wait", "ps5_server_stats",
 ]
 
 
@@ -60,6 +62,9 @@ def load():
     _sig(lib, "mi355x_ps5_sort_lcp", None, [_P, _P, ctypes.c_size_t])
     _sig(lib, "mi355x_ps5_sort_out", None, [_P, _P, ctypes.c_size_t])
     _sig(lib, "ps5_sort_strings", _I, [_P, ctypes.c_size_t, _P, _P])
+    _sig(lib, "mi355x_ps5_sort_lcp_cache", None, [_P, _P, _P, ctypes.c_size_t])
+    _sig(lib, "ps5_sort_strings_cache", _I, [_P, ctypes.c_size_t, _P, _P, _P])
+    _sig(lib, "ps5_dist_chars_device", _I, [_P, _P, _U64, _P, _U64, _P, _P, _P])
     _sig(lib, "ps5_ctx_create", _I, [ctypes.POINTER(_P), _I])
     _sig(lib, "ps5_ctx_destroy", None, [_P])
     _sig(lib, "ps5_ctx_reserve", _I, [_P, _U64])
@@ -87,6 +92,12 @@ def load():
     _sig(lib, "ps5_sort_batch_device", _I, [_P, _P, _U64, _P, _P, _U32, _P, _P])
     _sig(lib, "ps5_gen_batch_plan", _I, [_U64, _U32, _U64, _U64, _U32, _U32, _P, _P])
     _sig(lib, "ps5_gen_batch_fill", _I, [_U64, _U32, _U64, _U64, _U32, _U32, _P, _P])
+    _sig(lib, "ps5_server_create", _I, [ctypes.POINTER(_P), _I, _U64, _U32, _U64])
+    _sig(lib, "ps5_server_destroy", None, [_P])
+    _sig(lib, "ps5_server_submit", _U64, [_P, _P, _U64, _P, _U64, _P, _P])
+    _sig(lib, "ps5_server_wait", _I, [_P, _U64, ctypes.POINTER(ctypes.c_double)])
+    _sig(lib, "ps5_server_stats", _I, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(_U64), ctypes.POINTER(_U64), ctypes.POINTER(ctypes.c_double)])
     _lib = lib
     return lib
 
@@ -273,6 +284,27 @@ class Context:
                     b.free()
         return perm, lcp
 
+    def dist_chars_device(self, d_arena, arena_bytes, d_offs, n, d_perm, d_lcp, d_cache):
+        _check(load().ps5_dist_chars_device(self.h, d_arena.ptr, arena_bytes, d_offs.ptr, n, d_perm.ptr, d_lcp.ptr,
+                                            d_cache.ptr), "ps5_dist_chars_device")
+
+    def sort_cache(self, arena, offs):
+        """Upload, sort, distinguishing characters, download: (perm, lcp, cache)."""
+        n = len(offs)
+        bufs = [DeviceBuffer(arena.size), DeviceBuffer(max(4, n * 4)), DeviceBuffer(max(4, n * 4)),
+                DeviceBuffer(max(4, n * 4)), DeviceBuffer(max(1, n))]
+        da, do, dp, dl, dc = bufs
+        try:
+            da.upload(arena)
+            do.upload(np.ascontiguousarray(offs, np.uint32))
+            dc.upload(np.full(max(1, n), 0xEE, np.uint8))
+            self.sort_device(da, int(arena.size), do, n, dp, dl)
+            self.dist_chars_device(da, int(arena.size), do, n, dp, dl, dc)
+            return dp.download(np.uint32, n), dl.download(np.uint32, n), dc.download(np.uint8, n)
+        finally:
+            for b in bufs:
+                b.free()
+
     def sort_batch_device(self, d_arena, arena_bytes, d_offs64, set_first, d_perm, d_lcp=None):
         sf = np.ascontiguousarray(set_first, np.uint64)
         _check(load().ps5_sort_batch_device(self.h, d_arena.ptr, arena_bytes, d_offs64.ptr, _ptr(sf), len(sf) - 1,
@@ -320,15 +352,67 @@ class Context:
         return dict(zip(["rounds", "levels", "segments", "leaves", "syncs"], [v.value for v in vals]))
 
 
-def sort_strings(strings, with_lcp=False):
+class Server:
+    """Continuous-batching sort service (ps5_server_*)."""
+
+    def __init__(self, device=0, hbm_budget=8 << 30, slots=4, max_batch_strings=0):
+        p = _P()
+        _check(load().ps5_server_create(ctypes.byref(p), device, hbm_budget, slots, max_batch_strings),
+               "ps5_server_create")
+        self.h = p
+        self._keep = {}
+
+    def submit(self, arena, offs, with_lcp=True):
+        n = len(offs)
+        offs = np.ascontiguousarray(offs, np.uint32)
+        perm = np.zeros(max(1, n), np.uint32)
+        lcp = np.zeros(max(1, n), np.uint32) if with_lcp else None
+        rid = load().ps5_server_submit(self.h, _ptr(arena), int(arena.size), _ptr(offs), n, _ptr(perm),
+                                       _ptr(lcp) if with_lcp else None)
+        if rid == 0:
+            raise Ps5Error("ps5_server_submit failed")
+        self._keep[rid] = (arena, offs, perm, lcp, n)
+        return rid
+
+    def wait(self, rid):
+        ms = ctypes.c_double()
+        _check(load().ps5_server_wait(self.h, rid, ctypes.byref(ms)), "ps5_server_wait")
+        arena, offs, perm, lcp, n = self._keep.pop(rid)
+        return perm[:n], (lcp[:n] if lcp is not None else None), ms.value
+
+    def stats(self):
+        p50, p99, tp = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        done, batches = _U64(), _U64()
+        _check(load().ps5_server_stats(self.h, ctypes.byref(p50), ctypes.byref(p99), ctypes.byref(done),
+                                       ctypes.byref(batches), ctypes.byref(tp)), "ps5_server_stats")
+        return {"p50_ms": p50.value, "p99_ms": p99.value, "completed": done.value, "batches": batches.value,
+                "throughput_gbs": tp.value}
+
+    def close(self):
+        if self.h:
+            load().ps5_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sort_strings(strings, with_lcp=False, with_cache=False):
     """Sort a list of bytes objects through the Contest-shaped host entry
-    (ps5_sort_strings): returns (sorted list, lcp array or None)."""
+    (ps5_sort_strings_cache): returns (sorted list, order, lcp or None[, cache])."""
     lib = load()
     n = len(strings)
     bufs = [ctypes.create_string_buffer(s, len(s) + 1) for s in strings]
     arr = (ctypes.c_void_p * max(1, n))(*[ctypes.addressof(b) for b in bufs])
+    with_lcp = with_lcp or with_cache
     lcp = np.full(max(1, n), 42, np.uint64) if with_lcp else None
-    _check(lib.ps5_sort_strings(arr, n, _ptr(lcp) if with_lcp else None, None), "ps5_sort_strings")
+    cache = np.full(max(1, n), 0xEE, np.uint8) if with_cache else None
+    _check(lib.ps5_sort_strings_cache(arr, n, _ptr(lcp) if with_lcp else None,
+                                      _ptr(cache) if with_cache else None, None), "ps5_sort_strings_cache")
     addr = {ctypes.addressof(b): i for i, b in enumerate(bufs)}
     order = [addr[arr[i]] for i in range(n)]
-    return [strings[i] for i in order], order, (lcp[:n] if with_lcp else None)
+    out = [strings[i] for i in order], order, (lcp[:n] if with_lcp else None)
+    return out + (cache[:n],) if with_cache else out

@@ -130,6 +130,32 @@ def test_host_entry_contiguous_and_packed(ps5, oracle):
     assert np.array_equal(lcp[1:].astype(np.uint32), el[1:])
 
 
+@pytest.mark.parametrize("letters,hi,n", [(b"ab", 8, 20000), (b"ACGT", 30, 150000),
+                                          (bytes(range(1, 256)), 3, 5000)], ids=["ab", "acgt", "bytes"])
+def test_dist_chars_device(ctx, oracle, letters, hi, n):
+    """distinguishing characters on the device equal the oracle's; cache[0] untouched"""
+    strs = random_strings(n, 0, hi, letters, n + hi)
+    arena, offs = pack(strs)
+    perm, lcp, cache = ctx.sort_cache(arena, offs)
+    bp, bl = oracle.brute_sort(arena, offs)
+    assert np.array_equal(perm, bp) and np.array_equal(lcp[1:], bl[1:])
+    want = oracle.dist_chars(arena, offs, bp, bl)
+    assert cache[0] == 0xEE
+    assert np.array_equal(cache[1:], want[1:])
+
+
+def test_host_entry_lcp_cache(ps5, oracle):
+    """func_lcp_cache_type shape: lcp[0] and cache[0] untouched"""
+    strs = random_strings(30000, 0, 14, b"xyz0", 21)
+    out, order, lcp, cache = ps5.sort_strings(strs, with_cache=True)
+    arena, offs = pack(strs)
+    bp, bl = oracle.brute_sort(arena, offs)
+    assert order == list(bp)
+    assert lcp[0] == 42 and cache[0] == 0xEE
+    assert np.array_equal(lcp[1:].astype(np.uint32), bl[1:])
+    assert np.array_equal(cache[1:], oracle.dist_chars(arena, offs, bp, bl)[1:])
+
+
 def golden_files():
     if not os.path.isdir(GOLDEN):
         return []
@@ -184,3 +210,36 @@ def test_batch_edge_sets(ps5, ctx, oracle):
     sf[1:] = np.cumsum([len(st) for st in sets])
     perm, lcp = ctx.sort_batch(arena, offs32.astype(np.uint64), sf)
     _check_batch(oracle, arena, offs32.astype(np.uint64), sf, perm, lcp)
+
+
+@pytest.mark.parametrize("budget_mb,slots", [(4096, 4), (2, 2)])
+def test_serve_mix_equals_alone(ps5, ctx, oracle, budget_mb, slots):
+    """requests served concurrently (continuous batching) give exactly the
+    result each gets when sorted alone -- with a large budget (many requests
+    per batch) and a tiny one (requests wait for memory, one batch each)"""
+    say("serve", budget_mb, slots)
+    reqs = []
+    rng = np.random.default_rng(77)
+    for i in range(24):
+        size = int(np.exp(rng.uniform(np.log(10_000), np.log(3_000_000))))
+        if i % 3 == 0:
+            a, o = ps5.gen_prefixed(1000 + i, size)
+        elif i % 3 == 1:
+            a, o = ps5.gen_uniform(2000 + i, size, 0, 20, b"ACGT")
+        else:
+            a, o = ps5.gen_uniform(3000 + i, size, 1, 64, ps5.ALPHA_ASCII94)
+        reqs.append((a, o))
+    alone = [ctx.sort(a, o) for a, o in reqs]
+    srv = ps5.Server(0, hbm_budget=budget_mb << 20, slots=slots)
+    try:
+        ids = [srv.submit(a, o) for a, o in reqs]
+        mixed = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    say("serve stats", st)
+    assert st["completed"] == len(reqs)
+    for (a, o), (pa, la), (pm, lm, ms) in zip(reqs, alone, mixed):
+        assert np.array_equal(pa, pm)
+        assert np.array_equal(la[1:], lm[1:])
+        assert oracle.verify(a, o, pm, lm)[0] == 0

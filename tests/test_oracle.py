@@ -153,6 +153,32 @@ def test_verify_rejects_planted_mistakes(oracle):
     assert oracle.verify(arena, offs, p, lcp)[0] != 0
 
 
+# ---- distinguishing characters (verify_lcp_cache semantics) -----------------
+
+@pytest.mark.parametrize("seed,letters,hi", [(1, b"ab", 6), (2, b"ACGT", 12), (3, bytes(range(1, 256)), 4)])
+def test_dist_chars_definition(oracle, seed, letters, hi):
+    """cache[i] is the first byte of s_i not shared with s_{i-1}: it is NUL
+    exactly for equal neighbours and otherwise exceeds the predecessor's byte
+    at the same position (which a shifted position or the wrong string fails)"""
+    strs = random_strings(3000, 0, hi, letters, seed)
+    arena, offs = pack(strs)
+    perm, lcp = oracle.brute_sort(arena, offs)
+    cache = oracle.dist_chars(arena, offs, perm, lcp)
+    for i in range(1, len(strs)):
+        a, b = strs[perm[i - 1]] + b"\0", strs[perm[i]] + b"\0"
+        h = int(lcp[i])
+        assert cache[i] == b[h]
+        if strs[perm[i - 1]] == strs[perm[i]]:
+            assert cache[i] == 0
+        else:
+            assert cache[i] > a[h]
+    # planted mistakes: a shifted position and the predecessor's character
+    bad_shift = oracle.dist_chars(arena, offs, perm, np.maximum(lcp, 1) - 1)
+    assert not np.array_equal(bad_shift[1:], cache[1:])
+    bad_prev = oracle.dist_chars(arena, offs, np.concatenate([perm[:1], perm[:-1]]), lcp)
+    assert not np.array_equal(bad_prev[1:], cache[1:])
+
+
 # ---- golden fixtures --------------------------------------------------------
 
 def golden_files():
