@@ -228,8 +228,75 @@ def bench_serve(ps5, args):
     print(json.dumps(out), flush=True)
 
 
+def _kernel_base(name):
+    """'void ps5::(anonymous namespace)::k_block<2>(...)' -> 'k_block'"""
+    name = name.strip('"')
+    name = name.split("(", 1)[0] if not name.startswith("void ") else name[5:].split("(", 1)[0]
+    name = name.split("<", 1)[0]
+    return name.rsplit("::", 1)[-1].strip()
+
+
+def rocprof_summary(d):
+    """Per-kernel table from a rocprofv3 kernel trace (--stats) and two PMC
+    passes (FETCH_SIZE, WRITE_SIZE in KiB per dispatch), the counters scaled
+    so that k_copy16 (1 GiB read + 1 GiB written) reads 1 GiB each."""
+    import collections
+    import csv
+    import glob
+
+    def one(pattern):
+        hits = sorted(glob.glob(os.path.join(d, pattern), recursive=True))
+        if not hits:
+            raise SystemExit(f"no {pattern} under {d}")
+        return hits[0]
+
+    stats = collections.defaultdict(lambda: [0, 0.0])
+    for r in csv.DictReader(open(one("**/*kernel_stats.csv"))):
+        k = _kernel_base(r["Name"])
+        stats[k][0] += int(r["Calls"])
+        stats[k][1] += float(r["TotalDurationNs"])
+
+    def counters(pattern, cname):
+        per = collections.defaultdict(list)
+        for r in csv.DictReader(open(one(pattern))):
+            if r["Counter_Name"] == cname:
+                per[_kernel_base(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        return per
+    fe = counters("**/fetch*counter_collection.csv", "FETCH_SIZE")
+    wr = counters("**/write*counter_collection.csv", "WRITE_SIZE")
+    gib_kib = float(1 << 20)
+    ff = gib_kib / np.mean(fe["k_copy16"])
+    fw = gib_kib / np.mean(wr["k_copy16"])
+    lines = [f"calibration: k_copy16 moves 1 GiB in + 1 GiB out; FETCH_SIZE x{ff:.3f}, WRITE_SIZE x{fw:.3f}",
+             "",
+             "| kernel | dispatches | avg us | total ms | fetch MB/dispatch | write MB/dispatch | measured GB/s |",
+             "|---|---|---|---|---|---|---|"]
+    for k, (calls, tot_ns) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        avg_us = tot_ns / calls / 1e3
+        f = float(np.mean(fe[k])) * ff * 1024 / 1e6 if k in fe else 0.0
+        w = float(np.mean(wr[k])) * fw * 1024 / 1e6 if k in wr else 0.0
+        gbs = (f + w) * 1e6 / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
+        lines.append(f"| {k} | {calls} | {avg_us:.1f} | {tot_ns / 1e6:.3f} | {f:.1f} | {w:.1f} | {gbs:.1f} |")
+    print("\n".join(lines))
+
+
+def profiled_traffic(kernel, workload):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 summary
+    of this workload (profiles/<workload>_kernels.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_kernels.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == kernel:
+            return int((float(cells[4]) + float(cells[5])) * 1e6)
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--rocprof-summary", metavar="DIR",
+                    help="print the per-kernel table of a rocprofv3 trace + FETCH_SIZE + WRITE_SIZE run and exit")
     ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"])
     ap.add_argument("--sets", type=int, default=4096)
     ap.add_argument("--count-lo", type=int, default=1000)
@@ -255,6 +322,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.rocprof_summary:
+        return rocprof_summary(args.rocprof_summary)
     ps5 = load_ps5()
     if args.mode == "batch":
         return bench_batch(ps5, args)
@@ -314,6 +383,8 @@ def main():
     avg_ms = d["ms"] / d["launches"]
     bytes_per_launch = d["bytes"] / d["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    sort_bytes = sum(v["bytes"] for v in ks.values())       # every pass of one sort
+    sort_ms = float(np.median(step_ms))
     if args.profile:
         tot = sum(v["ms"] for v in ks.values())
         for k, v in sorted(ks.items(), key=lambda kv: -kv[1]["ms"]):
@@ -355,8 +426,12 @@ def main():
         "config": {"workload": WORKLOADS[args.workload], "strings": n, "arena_bytes": chars,
                    "output": "permutation + LCP array", "device_resident": True},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": int(bytes_per_launch)},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic(dom, args.workload),
+                     "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": int(bytes_per_launch),
+                     "whole_sort": {"alg_bytes": int(sort_bytes), "device_ms": round(sort_ms, 4),
+                                    "achieved": round(sort_bytes / (sort_ms * 1e-3) / 1e9, 2),
+                                    "frac": round(sort_bytes / (sort_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

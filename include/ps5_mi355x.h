@@ -216,9 +216,11 @@ uint64_t ps5_gen_prefixed(uint64_t seed, uint64_t arena_cap, uint32_t nprefix,
  * string count floor(exp(U(ln count_lo, ln count_hi))) and its alphabet (ACGT,
  * '0'..'9' or bytes 33..126, uniformly) from xoshiro256** seeded with s, its
  * string lengths U[len_lo, len_hi] from a stream seeded with
- * s ^ 0x5851F42D4C957F2D and its characters from one seeded with s ^ 0xC4A2C7E5.  plan: fills counts[nsets] and
- * bytes[nsets] (arena bytes of each set, NULs included).  fill: writes set k
- * at arena + sum(bytes[0..k)) with uint64 offsets at offs + sum(counts[0..k)),
+ * s ^ 0x5851F42D4C957F2D and its characters from one seeded with
+ * s ^ 0xC4A2C7E5.  plan: fills counts[nsets] and bytes[nsets] (arena bytes of
+ * each set: its strings with their NULs, then zero padding to a multiple of 8,
+ * so that every set starts 8-byte aligned).  fill: writes set k at
+ * arena + sum(bytes[0..k)) with uint64 offsets at offs + sum(counts[0..k)),
  * in parallel over sets. */
 int ps5_gen_batch_plan(uint64_t seed, uint32_t nsets, uint64_t count_lo, uint64_t count_hi,
                        uint32_t len_lo, uint32_t len_hi, uint64_t* counts, uint64_t* bytes);

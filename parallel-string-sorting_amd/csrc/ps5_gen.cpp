@@ -228,7 +228,7 @@ extern "C" int ps5_gen_batch_plan(uint64_t seed, uint32_t nsets, uint64_t count_
         uint64_t tot = 0;
         for (uint64_t i = 0; i < b.count; ++i) tot += lens.uniform(len_lo, len_hi) + 1;
         counts[k] = b.count;
-        bytes[k] = tot;
+        bytes[k] = (tot + 7) & ~7ull;      // every set starts 8-byte aligned
     });
     return PS5_OK;
 }
@@ -262,6 +262,7 @@ extern "C" int ps5_gen_batch_fill(uint64_t seed, uint32_t nsets, uint64_t count_
             arena[pos + L] = 0;
             pos += L + 1;
         }
+        if (pos < bfirst[k + 1]) memset(arena + pos, 0, bfirst[k + 1] - pos);
     });
     return PS5_OK;
 }

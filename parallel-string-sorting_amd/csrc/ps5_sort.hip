@@ -477,137 +477,36 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
     wave_append_segment(nr, ni, nxt);
 }
 
-// (key, position) order: positions are distinct, so the order is total and
-// equal keys keep their input order (stable).
-__device__ __forceinline__ bool kp_less(key64 ka, uint32_t pa, key64 kb, uint32_t pb)
+// Number of elements of a sorted run a[0, len) that precede `key` of a run
+// further right (before_eq: elements equal to key precede it, i.e. the run is
+// to the left of the key's run).  Binary lifting over a power-of-two L >= len.
+__device__ __forceinline__ uint32_t run_rank(const key64* a, uint32_t len, uint32_t L, key64 key, bool before_eq)
 {
-    return ka < kb || (ka == kb && pa < pb);
-}
-
-__device__ __forceinline__ key64 shfl_xor64(key64 v, int m)
-{
-    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
-    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
-    return ((key64)hi << 32) | lo;
-}
-
-// Bitonic sort of P = kWG*E (key, position) pairs held in registers: element
-// i = e*kWG + threadIdx.x lives in k[e], p[e] of that thread.  Partners at
-// distance j >= kWG are in the same thread, 64 <= j < kWG in another wave
-// (exchanged through LDS), j < 64 in the same wave (lane shuffles).
-template <int E>
-__device__ __forceinline__ void block_bitonic(key64 (&k)[E], uint32_t (&p)[E], key64* s_k, uint32_t* s_p)
-{
-    constexpr int P = kWG * E;
-    const uint32_t tid = threadIdx.x;
-#pragma unroll
-    for (int kk = 2; kk <= P; kk <<= 1) {
-#pragma unroll
-        for (int j = kk >> 1; j > 0; j >>= 1) {
-            if (j >= kWG) {
-                const int d = j / kWG;
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const int f = e ^ d;
-                    if (f > e) {
-                        const uint32_t i = (uint32_t)e * kWG + tid;
-                        const bool up = (i & (uint32_t)kk) == 0;
-                        if (kp_less(k[f], p[f], k[e], p[e]) == up) {
-                            const key64 tk = k[e]; k[e] = k[f]; k[f] = tk;
-                            const uint32_t tp = p[e]; p[e] = p[f]; p[f] = tp;
-                        }
-                    }
-                }
-            } else if (j >= 64) {
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    s_k[e * kWG + tid] = k[e];
-                    s_p[e * kWG + tid] = p[e];
-                }
-                __syncthreads();
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const uint32_t i = (uint32_t)e * kWG + tid;
-                    const key64 ok = s_k[i ^ (uint32_t)j];
-                    const uint32_t op = s_p[i ^ (uint32_t)j];
-                    const bool keep_min = ((i & (uint32_t)j) == 0) == ((i & (uint32_t)kk) == 0);
-                    const bool other_less = kp_less(ok, op, k[e], p[e]);
-                    if (other_less == keep_min) { k[e] = ok; p[e] = op; }
-                }
-                __syncthreads();
-            } else {
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const uint32_t i = (uint32_t)e * kWG + tid;
-                    const key64 ok = shfl_xor64(k[e], j);
-                    const uint32_t op = (uint32_t)__shfl_xor((int)p[e], j, 64);
-                    const bool keep_min = ((i & (uint32_t)j) == 0) == ((i & (uint32_t)kk) == 0);
-                    const bool other_less = kp_less(ok, op, k[e], p[e]);
-                    if (other_less == keep_min) { k[e] = ok; p[e] = op; }
-                }
-            }
+    uint32_t c = 0;
+    for (uint32_t s = L; s > 0; s >>= 1) {
+        if (c + s <= len) {
+            const key64 x = a[c + s - 1];
+            if (x < key || (before_eq && x == key)) c += s;
         }
     }
+    return c;
 }
 
-// Sort a block of m <= kWG*E strings and finish every position at this depth.
-template <int E>
-__device__ __forceinline__ void block_sort_finish(const Item& L, const Bufs& B, const Sink& nxt, uint32_t depth,
-                                                  key64* s_k, uint32_t* s_p, uint32_t* s_id)
-{
-    const uint32_t m = L.size, beg = L.begin, tid = threadIdx.x;
-    const int src = L.flags & F_BUFB;
-    const key64* K = src ? B.key1 : B.key0;
-    const uint32_t* I = src ? B.idx1 : B.idx0;
-    key64 k[E];
-    uint32_t p[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint32_t i = (uint32_t)e * kWG + tid;
-        k[e] = i < m ? K[beg + i] : ~0ull;   // padding sorts last (positions >= m)
-        p[e] = i;
-        if (i < m) s_id[i] = I[beg + i];
-    }
-    block_bitonic<E>(k, p, s_k, s_p);
-#pragma unroll
-    for (int e = 0; e < E; ++e) s_k[e * kWG + tid] = k[e];
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint32_t i = (uint32_t)e * kWG + tid;   // sorted position
-        const bool act = i < m;
-        bool eqp = false, eqn = false;
-        key64 pk = 0;
-        if (act) {
-            if (i > 0) pk = s_k[i - 1];
-            eqp = i > 0 && pk == k[e];
-            eqn = i + 1 < m && s_k[i + 1] == k[e];
-            finish_position(B, beg + i, s_id[p[e]], k[e], i > 0, pk, !eqp && !eqn, depth);
-            if (i == 0 || i == m - 1) B.bkey[beg + i] = k[e];
-        }
-        // a run of equal keys without a NUL continues at depth + 8
-        const bool nr = act && !eqp && eqn && (k[e] & 0xFFull) != 0;
-        uint32_t rl = 0;
-        if (nr) {
-            rl = 1;
-            while (i + rl < m && s_k[i + rl] == k[e]) ++rl;
-        }
-        Item ni = { beg + i, rl, beg + i, 0u };
-        wave_append_segment(nr, ni, nxt);
-    }
-}
-
-// One workgroup sorts a block of 65..kBlkT strings completely on chip: a
-// bitonic network over (key, position) pairs in registers, then every sorted
-// position is finished -- LCP to its neighbour, identical strings done, runs
-// of equal keys without a NUL handed to round r+1 (the reference's
-// insertion-sort base case, lcp_inssort.hpp:36-187, and the equal-bucket
-// rules, .hpp:1531-1557, on sorted keys).
+// One workgroup sorts a block of 65..kBlkT strings completely on chip and
+// finishes every position at this depth (the reference's insertion-sort base
+// case, lcp_inssort.hpp:36-187, and the equal-bucket rules, .hpp:1531-1557,
+// on sorted keys).  Runs of 64 keys are sorted in the waves' registers
+// (wave_sort64), then merged pairwise: an element's place in the merged run
+// is its place in its own run plus the number of partner-run elements before
+// it (binary search in LDS; equal keys of the left run go first, so the
+// order stays stable).  Finishing a sorted position: LCP to its neighbour,
+// identical strings done, runs of equal keys without a NUL handed to round
+// r+1.
 __global__ __launch_bounds__(kWG) void k_block(const Item* items, uint32_t first, const uint32_t* cnt_items,
                                                 Bufs B, Sink nxt, uint32_t depth)
 {
-    __shared__ key64 s_k[kBlkT];
-    __shared__ uint32_t s_p[kBlkT];
+    __shared__ key64 s_k[2][kBlkT];
+    __shared__ uint32_t s_p[2][kBlkT];
     __shared__ uint32_t s_id[kBlkT];
     const uint32_t idx = first + blockIdx.x;
     if (idx >= *cnt_items) return;
@@ -616,9 +515,62 @@ __global__ __launch_bounds__(kWG) void k_block(const Item* items, uint32_t first
         if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
-    if (L.size <= kWG) block_sort_finish<1>(L, B, nxt, depth, s_k, s_p, s_id);
-    else if (L.size <= 2 * kWG) block_sort_finish<2>(L, B, nxt, depth, s_k, s_p, s_id);
-    else block_sort_finish<4>(L, B, nxt, depth, s_k, s_p, s_id);
+    const uint32_t m = L.size, beg = L.begin, tid = threadIdx.x, lane = lane_id();
+    const int src = L.flags & F_BUFB;
+    const key64* K = src ? B.key1 : B.key0;
+    const uint32_t* I = src ? B.idx1 : B.idx0;
+    const uint32_t R = (m + 63) / 64, P = R * 64;
+    // runs of 64 sorted in registers; padding (key ~0, position >= m) sorts last
+    for (uint32_t r = wave_id(); r < R; r += kWaves) {
+        const uint32_t i = r * 64 + lane;
+        key64 k = i < m ? K[beg + i] : ~0ull;
+        uint32_t p = i;
+        if (i < m) s_id[i] = I[beg + i];
+        wave_sort64(k, p);
+        s_k[0][i] = k;
+        s_p[0][i] = p;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (uint32_t Lr = 64; Lr < P; Lr <<= 1) {
+        for (uint32_t i = tid; i < P; i += kWG) {
+            const uint32_t run = i / Lr, t = i % Lr;
+            const uint32_t prt = run ^ 1u, pstart = prt * Lr;
+            const key64 k = s_k[cur][i];
+            const uint32_t p = s_p[cur][i];
+            uint32_t c = 0;
+            if (pstart < P) c = run_rank(&s_k[cur][pstart], min(Lr, P - pstart), Lr, k, prt < run);
+            const uint32_t dst = min(run, prt) * Lr + t + c;
+            s_k[cur ^ 1][dst] = k;
+            s_p[cur ^ 1][dst] = p;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    const key64* sk = s_k[cur];
+    const uint32_t* sp = s_p[cur];
+    for (uint32_t i = tid; i < ((m + kWG - 1) / kWG) * kWG; i += kWG) {
+        const bool act = i < m;     // sorted position
+        bool eqp = false, eqn = false;
+        key64 k = 0, pk = 0;
+        if (act) {
+            k = sk[i];
+            if (i > 0) pk = sk[i - 1];
+            eqp = i > 0 && pk == k;
+            eqn = i + 1 < m && sk[i + 1] == k;
+            finish_position(B, beg + i, s_id[sp[i]], k, i > 0, pk, !eqp && !eqn, depth);
+            if (i == 0 || i == m - 1) B.bkey[beg + i] = k;
+        }
+        // a run of equal keys without a NUL continues at depth + 8
+        const bool nr = act && !eqp && eqn && (k & 0xFFull) != 0;
+        uint32_t rl = 0;
+        if (nr) {
+            rl = 1;
+            while (i + rl < m && sk[i + rl] == k) ++rl;
+        }
+        Item ni = { beg + i, rl, beg + i, 0u };
+        wave_append_segment(nr, ni, nxt);
+    }
 }
 
 // One workgroup per chunk of an equal-key leaf (all keys equal).

@@ -4,6 +4,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -74,3 +75,21 @@ def test_reference_random_matches_fixture(ps5):
     z = np.load(os.path.join(ROOT, "tests", "golden", "random10_64k.npz"))
     arena, offs = ps5.gen_reference_random(65536, b"0123456789")
     assert unpack(arena, offs) == unpack(z["arena"], z["offs"])
+
+
+def test_batch_generator_layout(ps5):
+    """sets lie back to back, each 8-byte aligned, NUL-padded, with counts in range"""
+    arena, offs, sf = ps5.gen_batch(0x5EED0004, 64, 1000, 20000, 4, 48)
+    assert sf[0] == 0 and np.all(np.diff(sf.astype(np.int64)) >= 1000) and np.all(np.diff(sf.astype(np.int64)) <= 20000)
+    starts = offs[sf[:-1].astype(np.int64)]
+    assert np.all(starts % 8 == 0) and starts[0] == 0
+    for k in range(64):
+        b, e = int(sf[k]), int(sf[k + 1])
+        o = offs[b:e].astype(np.int64)
+        lens = np.diff(np.append(o, o[-1] + 1 + int(np.argmax(arena[o[-1]:] == 0)))) - 1
+        assert lens.min() >= 4 and lens.max() <= 48
+        end = int(starts[k + 1]) if k + 1 < 64 else arena.size - 16
+        tail = arena[o[-1] + lens[-1]:end]
+        assert np.all(tail == 0) and end - (o[-1] + lens[-1] + 1) < 8
+    again = ps5.gen_batch(0x5EED0004, 64, 1000, 20000, 4, 48)
+    assert np.array_equal(again[0], arena) and np.array_equal(again[1], offs)
