@@ -192,7 +192,7 @@ def bench_serve(ps5, args):
         return lat, time.perf_counter() - start
 
     def run_served():
-        srv = ps5.Server(0, hbm_budget=args.budget_gb << 30, slots=args.slots)
+        srv = ps5.Server(0, hbm_budget=args.budget_gb << 30, slots=args.slots, max_batch_strings=args.max_batch)
         ids = []
         start = time.perf_counter()
         for i, (a, o) in enumerate(reqs):
@@ -216,7 +216,7 @@ def bench_serve(ps5, args):
            "served": {"p50_ms": round(float(np.percentile(served_lat, 50)), 3),
                       "p99_ms": round(float(np.percentile(served_lat, 99)), 3),
                       "throughput_gbs": round(total / served_wall / 1e9, 3), "batches": st["batches"],
-                      "slots": args.slots, "hbm_budget_gb": args.budget_gb},
+                      "slots": args.slots, "hbm_budget_gb": args.budget_gb, "max_batch_strings": args.max_batch},
            "plain_fifo": {"p50_ms": round(float(np.percentile(fifo_lat, 50)), 3),
                           "p99_ms": round(float(np.percentile(fifo_lat, 99)), 3),
                           "throughput_gbs": round(total / fifo_wall / 1e9, 3)},
@@ -307,6 +307,7 @@ def main():
     ap.add_argument("--load", type=float, default=0.7)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=int, default=64)
+    ap.add_argument("--max-batch", type=int, default=16 << 20, help="strings per served batch (buffers pre-allocated)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)

@@ -148,6 +148,34 @@ int ps5_sort_device_timed(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_b
                           const uint32_t* d_offs, uint64_t n,
                           uint32_t* d_perm, uint32_t* d_lcp, double* ms);
 
+/* ---- tuning ------------------------------------------------------------ */
+
+/* Shape parameters of a context's sorts (they never change the output):
+ *   leaf_target    strings per bucket the multi-workgroup levels aim for,
+ *                  in [64, 1024] (default 256); sets how many levels run and
+ *                  how large the on-chip blocks are;
+ *   max_treebits   splitter-tree depth limit of one level, in [1, 10]
+ *                  (default 10: up to 1023 splitters, 2047 buckets, the
+ *                  reference's DefaultTreebits, bingmann-sample_sort.hpp:53);
+ *   max_oversample sample keys per splitter, even, in [2, 32] (default 32;
+ *                  the reference samples 2 per splitter, .hpp:1384-1410). */
+int ps5_ctx_set_tuning(ps5_ctx* ctx, uint32_t leaf_target, uint32_t max_treebits, uint32_t max_oversample);
+int ps5_ctx_get_tuning(ps5_ctx* ctx, uint32_t* leaf_target, uint32_t* max_treebits, uint32_t* max_oversample);
+
+/* Autotuner: sorts the given input (arguments as ps5_sort_device /
+ * ps5_sort_batch_device, outputs written as by them) under candidate
+ * settings -- coordinate descent over the three parameters, each candidate
+ * timed as the best of two sorts -- keeps the fastest in the context and
+ * returns it and its time.  No counterpart in the reference. */
+int ps5_autotune_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
+                        const uint32_t* d_offs, uint64_t n, uint32_t* d_perm, uint32_t* d_lcp,
+                        uint32_t* leaf_target, uint32_t* max_treebits, uint32_t* max_oversample,
+                        double* best_ms);
+int ps5_autotune_batch_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
+                              const uint64_t* d_offs, const uint64_t* set_first, uint32_t nsets,
+                              uint32_t* d_perm, uint32_t* d_lcp, uint32_t* leaf_target,
+                              uint32_t* max_treebits, uint32_t* max_oversample, double* best_ms);
+
 /* ---- serving: continuous batching under an HBM budget ------------------ */
 
 typedef struct ps5_server ps5_server;
@@ -156,7 +184,10 @@ typedef struct ps5_server ps5_server;
  * free slot takes every waiting request that fits under hbm_budget bytes of
  * staged data (in arrival order; a request larger than the budget runs alone)
  * and at most max_batch_strings strings (0: no limit), and sorts them as one
- * batch; slots run concurrently on the device.  No counterpart in the
+ * batch; slots run concurrently on the device.  With max_batch_strings > 0
+ * every slot allocates its device buffers for such a batch at creation (up to
+ * min(hbm_budget / slots, 64 * max_batch_strings) characters), so that the
+ * first requests do not wait for allocations.  No counterpart in the
  * reference (it sorts one set per call). */
 int ps5_server_create(ps5_server** out, int device, uint64_t hbm_budget, uint32_t slots,
                       uint64_t max_batch_strings);

@@ -9,10 +9,10 @@
 // behind the signatures of src/tools/contest.hpp:88-91.
 //
 // The strings are staged to the device as one character arena plus uint32
-// offsets.  When the pointers already describe a contiguous NUL-separated
-// arena in increasing order (the layout psstest builds, src/psstest.cpp:479-501)
-// that span is copied as is; otherwise the strings are packed first.  The
-// characters are never written.
+// offsets.  When the pointers already lie in one span in increasing order --
+// psstest's NUL-separated arena (src/psstest.cpp:479-501) or its suffixes of
+// one text (src/psstest.cpp:502-518) -- that span is copied as it is;
+// otherwise the strings are packed first.  The characters are never written.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -111,17 +111,24 @@ extern "C" int ps5_sort_strings_cache(unsigned char** strings, size_t n, uintptr
     parallel_for(n, [&](uint64_t b, uint64_t e) {
         for (uint64_t i = b; i < e; ++i) len[i] = strlen((const char*)strings[i]);
     });
-    bool contiguous = true;
-    for (uint64_t i = 0; i + 1 < n && contiguous; ++i)
-        contiguous = strings[i + 1] == strings[i] + len[i] + 1;
+    // One span: pointers in non-decreasing order, each string starting inside
+    // or right after what the strings before it cover.  That is psstest's
+    // arena (one NUL-separated string after the other) and its --suffix mode
+    // (every position of one text); every byte of the span belongs to some
+    // string, so the whole span is readable and is copied as it is.
+    bool span = true;
+    const uint8_t* span_end = strings[0] + len[0] + 1;
+    for (uint64_t i = 1; i < n && span; ++i) {
+        span = strings[i] >= strings[i - 1] && strings[i] <= span_end;
+        span_end = std::max<const uint8_t*>(span_end, strings[i] + len[i] + 1);
+    }
     std::vector<uint32_t> offs(n);
     std::vector<uint8_t> packed;
     const uint8_t* host_arena;
     uint64_t alen;
-    if (contiguous) {
+    if (span && (uint64_t)(span_end - strings[0]) <= 0xFFFFFFF0ull) {
         host_arena = strings[0];
-        alen = (uint64_t)(strings[n - 1] - strings[0]) + len[n - 1] + 1;
-        if (alen > 0xFFFFFFF0ull) return PS5_ERR_ARG;
+        alen = (uint64_t)(span_end - strings[0]);
         for (uint64_t i = 0; i < n; ++i) offs[i] = (uint32_t)(strings[i] - strings[0]);
     } else {
         uint64_t tot = 0;

@@ -137,6 +137,17 @@ __device__ __forceinline__ void wave_sort64(key64& k, uint32_t& p)
     wave_cx<64, 2>(k, p);  wave_cx<64, 1>(k, p);
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on
+// XCD i % 8), and every XCD has its own L2.  This bijection of [0, G) gives
+// each XCD one contiguous range of work items, so neighbouring tiles -- whose
+// histogram entries and bucket output ranges share cache lines -- meet in the
+// same L2.
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t bid, uint32_t G)
+{
+    const uint32_t x = bid % kXcds, y = bid / kXcds, q = G / kXcds, r = G % kXcds;
+    return x * q + min(x, r) + y;
+}
+
 // lanes holding the same 11-bit bucket id (all 64 lanes must execute this)
 __device__ __forceinline__ uint64_t match_bucket(uint32_t b)
 {

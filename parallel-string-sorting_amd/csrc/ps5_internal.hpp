@@ -1,0 +1,17 @@
+// ps5_internal.hpp -- library-internal helpers shared by the C ABI sources
+// (not part of the exported interface).
+#pragma once
+#include <cstdint>
+
+#include "../../include/ps5_mi355x.h"
+
+namespace ps5 {
+
+// Absolute uint64 offsets of a batch staged set by set: for string i of set s
+// (set_first[s] <= i < set_first[s+1]), offs64[i] = set_base[s] + offs32[i].
+// d_first / d_base are device arrays of nsets+1 / nsets entries.  Enqueued on
+// the context's stream (no synchronisation).
+int widen_offsets(ps5_ctx* c, const uint32_t* d_offs32, const uint64_t* d_first, const uint64_t* d_base,
+                  uint32_t nsets, uint64_t n, uint64_t* d_offs64);
+
+}  // namespace ps5

@@ -29,6 +29,7 @@ constexpr int kWG = 256;                     // threads per workgroup (small ker
 constexpr int kWaves = kWG / 64;             // wave64
 constexpr int kBigWG = 1024;                 // threads of the LDS-heavy kernels
 constexpr int kBigWaves = kBigWG / 64;
+constexpr uint32_t kXcds = 8;                // MI355X: 8 accelerator complex dies
 constexpr int kMaxTreebits = 10;             // reference DefaultTreebits
 constexpr int kMaxSpl = (1 << kMaxTreebits) - 1;    // 1023 splitters
 constexpr int kMaxBkt = 2 * kMaxSpl + 1;            // 2047 buckets
@@ -50,6 +51,7 @@ struct Item {            // a segment or a leaf
 };
 constexpr uint32_t F_BUFB = 1u;              // data lives in key[1]/idx[1]
 constexpr uint32_t F_SORTED = 2u;            // all keys equal (equal bucket)
+constexpr uint32_t F_IDENT = 4u;             // a root: idx[p] = p, not stored
 
 // device counters
 enum : int {

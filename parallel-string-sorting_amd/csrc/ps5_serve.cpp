@@ -7,7 +7,8 @@
 // (ps5_sort_batch_device: every request is its own set).  Slots run
 // concurrently, so requests that arrive while others are on the GPU are
 // admitted as soon as a slot and memory free up.  Each request's result
-// depends only on its own strings.
+// depends only on its own strings.  Requests are copied between their host
+// arrays and the slot's device buffers directly; device buffers only grow.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +22,7 @@
 #include <vector>
 
 #include "../../include/ps5_mi355x.h"
+#include "ps5_internal.hpp"
 
 namespace {
 
@@ -46,6 +48,8 @@ struct ps5_server {
     int device = 0;
     uint64_t budget = 0;              // device bytes for staged requests
     uint64_t max_batch = 0;           // strings per batch
+    uint64_t warm_strings = 0;        // per-slot buffers allocated up front
+    uint64_t warm_bytes = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
@@ -62,23 +66,75 @@ struct ps5_server {
 
 namespace {
 
-// device bytes a request occupies while it runs: its characters, uint64
-// offsets, results, and the sort context's work buffers (~48 B per string)
-uint64_t request_bytes(uint64_t arena_bytes, uint64_t n) { return arena_bytes + 16 + n * (8 + 8 + 48) + 4096; }
+// device bytes a request occupies while it runs: its characters, uint32 and
+// uint64 offsets, results, and the sort context's work buffers (~72 B per string)
+uint64_t request_bytes(uint64_t arena_bytes, uint64_t n) { return arena_bytes + 16 + n * (4 + 8 + 8 + 72) + 4096; }
 
+// device buffer that only grows, at least by half each time
+template <class T>
+int grow(T** p, uint64_t* cap, uint64_t need)
+{
+    if (need <= *cap) return PS5_OK;
+    const uint64_t want = std::max<uint64_t>(need, *cap + *cap / 2);
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc(reinterpret_cast<void**>(p), want * sizeof(T)) != hipSuccess) return PS5_ERR_NOMEM;
+    *cap = want;
+    return PS5_OK;
+}
+
+struct SlotBuffers {
+    uint8_t* arena = nullptr;
+    uint64_t arena_cap = 0;
+    uint32_t* offs32 = nullptr;
+    uint64_t offs32_cap = 0;
+    uint64_t* offs64 = nullptr;
+    uint64_t offs64_cap = 0;
+    uint32_t* perm = nullptr;
+    uint64_t perm_cap = 0;
+    uint32_t* lcp = nullptr;
+    uint64_t lcp_cap = 0;
+    uint64_t* first = nullptr;       // set boundaries (nsets + 1)
+    uint64_t first_cap = 0;
+    uint64_t* base = nullptr;        // arena position of every set
+    uint64_t base_cap = 0;
+
+    int reserve(uint64_t abytes, uint64_t n, uint64_t nsets)
+    {
+        int rc;
+        if ((rc = grow(&arena, &arena_cap, abytes)) || (rc = grow(&offs32, &offs32_cap, n)) ||
+            (rc = grow(&offs64, &offs64_cap, n)) || (rc = grow(&perm, &perm_cap, n)) ||
+            (rc = grow(&lcp, &lcp_cap, n)) || (rc = grow(&first, &first_cap, nsets + 1)) ||
+            (rc = grow(&base, &base_cap, nsets)))
+            return rc;
+        return PS5_OK;
+    }
+    void release()
+    {
+        for (void* q : { (void*)arena, (void*)offs32, (void*)offs64, (void*)perm, (void*)lcp, (void*)first,
+                         (void*)base })
+            if (q) (void)hipFree(q);
+    }
+};
+
+// One slot: admit a batch, stage every request straight from its host arrays
+// into the slot's device buffers (no host-side packing), sort the batch, copy
+// each request's results straight back.
 void run_slot(ps5_server* s)
 {
     if (hipSetDevice(s->device) != hipSuccess) return;
     ps5_ctx* ctx = nullptr;
     if (ps5_ctx_create(&ctx, s->device) != PS5_OK) return;
     hipStream_t st = (hipStream_t)ps5_ctx_stream(ctx);
-    uint8_t* d_arena = nullptr;
-    uint64_t* d_offs = nullptr;
-    uint32_t *d_perm = nullptr, *d_lcp = nullptr;
-    uint64_t cap_bytes = 0, cap_n = 0;
-    std::vector<uint64_t> h_offs;
-    std::vector<uint8_t> h_arena;
-    std::vector<uint32_t> h_perm, h_lcp;
+    SlotBuffers D;
+    // warm start: buffers for one batch of the configured size, so that the
+    // first requests do not pay for device allocations
+    if (s->warm_strings) {
+        (void)ps5_ctx_reserve(ctx, s->warm_strings);
+        (void)D.reserve(s->warm_bytes + 16, s->warm_strings, 64);
+    }
+    std::vector<uint64_t> set_first, set_base;
     for (;;) {
         std::vector<Request*> batch;
         uint64_t need = 0, nstr = 0;
@@ -104,63 +160,49 @@ void run_slot(ps5_server* s)
             s->in_use += need;
         }
         const auto t_start = Clock::now();
-        // stage the batch: one arena, uint64 offsets, set boundaries
+        const uint32_t nsets = (uint32_t)batch.size();
+        set_first.assign(nsets + 1, 0);
+        set_base.assign(nsets, 0);
         uint64_t abytes = 0;
-        std::vector<uint64_t> set_first(batch.size() + 1, 0);
-        for (size_t i = 0; i < batch.size(); ++i) {
-            abytes += (batch[i]->arena_bytes + 7) & ~7ull;
+        for (uint32_t i = 0; i < nsets; ++i) {
+            set_base[i] = abytes;
+            abytes += (batch[i]->arena_bytes + 7) & ~7ull;     // every set starts 8-byte aligned
             set_first[i + 1] = set_first[i] + batch[i]->n;
         }
-        int rc = PS5_OK;
-        if (abytes + 16 > cap_bytes || nstr > cap_n) {
-            if (d_arena) (void)hipFree(d_arena);
-            if (d_offs) (void)hipFree(d_offs);
-            if (d_perm) (void)hipFree(d_perm);
-            if (d_lcp) (void)hipFree(d_lcp);
-            d_arena = nullptr; d_offs = nullptr; d_perm = nullptr; d_lcp = nullptr;
-            cap_bytes = std::max(abytes + 16, cap_bytes);
-            cap_n = std::max<uint64_t>(nstr, std::max<uint64_t>(cap_n, 1));
-            if (hipMalloc(&d_arena, cap_bytes) != hipSuccess || hipMalloc(&d_offs, cap_n * 8) != hipSuccess ||
-                hipMalloc(&d_perm, cap_n * 4) != hipSuccess || hipMalloc(&d_lcp, cap_n * 4) != hipSuccess)
-                rc = PS5_ERR_NOMEM;
-        }
+        int rc = D.reserve(abytes + 16, std::max<uint64_t>(nstr, 1), nsets);
+        auto ok = [&](hipError_t e) { if (e != hipSuccess && rc == PS5_OK) rc = PS5_ERR_HIP; };
         if (rc == PS5_OK) {
-            h_arena.resize(abytes + 16);
-            h_offs.resize(nstr);
-            uint64_t pos = 0;
-            for (size_t i = 0; i < batch.size(); ++i) {
+            for (uint32_t i = 0; i < nsets; ++i) {
                 const Request* r = batch[i];
-                memcpy(h_arena.data() + pos, r->arena, r->arena_bytes);
                 const uint64_t pad = ((r->arena_bytes + 7) & ~7ull) - r->arena_bytes;
-                memset(h_arena.data() + pos + r->arena_bytes, 0, pad);
-                for (uint64_t j = 0; j < r->n; ++j) h_offs[set_first[i] + j] = pos + r->offs[j];
-                pos += r->arena_bytes + pad;
+                if (r->arena_bytes)
+                    ok(hipMemcpyAsync(D.arena + set_base[i], r->arena, r->arena_bytes, hipMemcpyHostToDevice, st));
+                if (pad) ok(hipMemsetAsync(D.arena + set_base[i] + r->arena_bytes, 0, pad, st));
+                if (r->n)
+                    ok(hipMemcpyAsync(D.offs32 + set_first[i], r->offs, r->n * 4, hipMemcpyHostToDevice, st));
             }
-            memset(h_arena.data() + abytes, 0, 16);
-            if (hipMemcpyAsync(d_arena, h_arena.data(), abytes + 16, hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(d_offs, h_offs.data(), nstr * 8, hipMemcpyHostToDevice, st) != hipSuccess)
-                rc = PS5_ERR_HIP;
+            ok(hipMemsetAsync(D.arena + abytes, 0, 16, st));
+            ok(hipMemcpyAsync(D.first, set_first.data(), (nsets + 1) * 8, hipMemcpyHostToDevice, st));
+            ok(hipMemcpyAsync(D.base, set_base.data(), nsets * 8, hipMemcpyHostToDevice, st));
         }
         if (rc == PS5_OK && nstr > 0)
-            rc = ps5_sort_batch_device(ctx, d_arena, abytes + 16, d_offs, set_first.data(),
-                                       (uint32_t)batch.size(), d_perm, d_lcp);
+            rc = ps5::widen_offsets(ctx, D.offs32, D.first, D.base, nsets, nstr, D.offs64);
+        if (rc == PS5_OK && nstr > 0)
+            rc = ps5_sort_batch_device(ctx, D.arena, abytes + 16, D.offs64, set_first.data(), nsets, D.perm, D.lcp);
         if (rc == PS5_OK && nstr > 0) {
-            h_perm.resize(nstr);
-            h_lcp.resize(nstr);
-            if (hipMemcpyAsync(h_perm.data(), d_perm, nstr * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipMemcpyAsync(h_lcp.data(), d_lcp, nstr * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess)
-                rc = PS5_ERR_HIP;
+            for (uint32_t i = 0; i < nsets; ++i) {
+                const Request* r = batch[i];
+                if (r->n) ok(hipMemcpyAsync(r->perm, D.perm + set_first[i], r->n * 4, hipMemcpyDeviceToHost, st));
+                if (r->lcp && r->n > 1)
+                    ok(hipMemcpyAsync(r->lcp + 1, D.lcp + set_first[i] + 1, (r->n - 1) * 4, hipMemcpyDeviceToHost,
+                                      st));
+            }
         }
+        ok(hipStreamSynchronize(st));
         const auto t_done = Clock::now();
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            for (size_t i = 0; i < batch.size(); ++i) {
-                Request* r = batch[i];
-                if (rc == PS5_OK && r->n) {
-                    memcpy(r->perm, h_perm.data() + set_first[i], r->n * 4);
-                    if (r->lcp && r->n > 1) memcpy(r->lcp + 1, h_lcp.data() + set_first[i] + 1, (r->n - 1) * 4);
-                }
+            for (Request* r : batch) {
                 r->status = rc;
                 r->t_start = t_start;
                 r->t_done = t_done;
@@ -176,10 +218,7 @@ void run_slot(ps5_server* s)
         s->cv_done.notify_all();
         s->cv_work.notify_all();
     }
-    if (d_arena) (void)hipFree(d_arena);
-    if (d_offs) (void)hipFree(d_offs);
-    if (d_perm) (void)hipFree(d_perm);
-    if (d_lcp) (void)hipFree(d_lcp);
+    D.release();
     ps5_ctx_destroy(ctx);
 }
 
@@ -194,6 +233,10 @@ int ps5_server_create(ps5_server** out, int device, uint64_t hbm_budget, uint32_
     s->device = device;
     s->budget = hbm_budget;
     s->max_batch = max_batch_strings ? max_batch_strings : (1ull << 31);
+    if (max_batch_strings) {    // each slot starts with buffers for a full batch
+        s->warm_strings = max_batch_strings;
+        s->warm_bytes = std::min<uint64_t>(hbm_budget / slots, max_batch_strings * 64);
+    }
     for (uint32_t i = 0; i < slots; ++i) s->workers.emplace_back(run_slot, s);
     *out = s;
     return PS5_OK;

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "ps5_device.hpp"
+#include "ps5_internal.hpp"
 #include "ps5_kernels.hpp"
 #include "../../include/ps5_mi355x.h"
 
@@ -33,13 +34,14 @@ using namespace ps5;
 // kernels
 // ===========================================================================
 
-// Round 0 keys: key0[i] = 8 characters of string i at depth 0, idx0[i] = i.
+// Round 0 keys: key0[i] = 8 characters of string i at depth 0.  The index
+// at every root position is the position itself (root items carry F_IDENT),
+// so idx0 is not written.
 __global__ __launch_bounds__(kWG) void k_init(Bufs B, uint32_t n)
 {
     const uint32_t i = blockIdx.x * kWG + threadIdx.x;
     if (i >= n) return;
     B.key0[i] = load_key(B.arena, B.alen, str_off(B, i));
-    B.idx0[i] = i;
 }
 
 // Reset counters and place the root item.
@@ -48,7 +50,7 @@ __global__ void k_setup(uint32_t* cnt, Item* list, int c_index, uint32_t n)
     if (threadIdx.x < C_NCNT) cnt[threadIdx.x] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
-        Item it = { 0u, n, 0u, 0u };
+        Item it = { 0u, n, 0u, F_IDENT };
         list[0] = it;
         cnt[c_index] = 1;
     }
@@ -110,8 +112,9 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
                                                 const key64* trees, uint32_t* hist)
 {
     __shared__ key64 s_tree[1024];
-    __shared__ uint32_t s_hist[kWaves][2048];
-    const TileRef tr = tiles[blockIdx.x];
+    __shared__ uint16_t s_hist[kWaves][2048];   // a wave chunk counts at most kWaveChunk keys
+    static_assert(kWaveChunk <= 65535, "per-wave bucket counts must fit 16 bits");
+    const TileRef tr = tiles[xcd_contiguous(blockIdx.x, gridDim.x)];
     const LargeDesc D = desc[tr.seg];
     const Item seg = D.seg;
     const int k = (int)D.k, nbkt = (2 << k) - 1;
@@ -138,7 +141,7 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
             const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
             if (a0) {
                 const uint32_t old = s_hist[w][b0];
-                if (r == 0) s_hist[w][b0] = old + (uint32_t)__popcll(peers);
+                if (r == 0) s_hist[w][b0] = (uint16_t)(old + (uint32_t)__popcll(peers));
                 B.bktrank[seg.begin + p0] = (b0 << kRankBits) | (old + r);
             }
         }
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
             const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
             if (a1) {
                 const uint32_t old = s_hist[w][b1];
-                if (r == 0) s_hist[w][b1] = old + (uint32_t)__popcll(peers);
+                if (r == 0) s_hist[w][b1] = (uint16_t)(old + (uint32_t)__popcll(peers));
                 B.bktrank[seg.begin + p1] = (b1 << kRankBits) | (old + r);
             }
         }
@@ -242,8 +245,9 @@ __global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* desc, const
     uint32_t* s_base = reinterpret_cast<uint32_t*>(smem + kScatBaseOff);
     uint32_t* s_tmp = reinterpret_cast<uint32_t*>(smem + kScatTmpOff);
 
-    const TileRef tr = tiles[blockIdx.x / kWaves];
-    const uint32_t w = blockIdx.x % kWaves;
+    const uint32_t chunk = xcd_contiguous(blockIdx.x, gridDim.x);
+    const TileRef tr = tiles[chunk / kWaves];
+    const uint32_t w = chunk % kWaves;
     const LargeDesc D = desc[tr.seg];
     const Item seg = D.seg;
     const int k = (int)D.k, nbkt = (2 << k) - 1;
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* desc, const
         const uint32_t l = b < (uint32_t)nbkt ? s_start[b] + (br & kRankMask) : 0xFFFFFFFFu;
         if (l >= m) { flag_error(B.cnt, ERR_BOUNDS); continue; }
         s_key[l] = K[q];
-        s_idx[l] = I[q];
+        s_idx[l] = (seg.flags & F_IDENT) ? q : I[q];
         s_bkt[l] = (uint16_t)b;
     }
     __syncthreads();
@@ -437,7 +441,7 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
     uint32_t id = 0;
     if (act) {
         key = src ? B.key1[L.begin + lane] : B.key0[L.begin + lane];
-        id = src ? B.idx1[L.begin + lane] : B.idx0[L.begin + lane];
+        id = (L.flags & F_IDENT) ? L.begin + lane : src ? B.idx1[L.begin + lane] : B.idx0[L.begin + lane];
     }
     if (act) s_k[w][lane] = key;
     __syncthreads();
@@ -525,7 +529,7 @@ __global__ __launch_bounds__(kWG) void k_block(const Item* items, uint32_t first
         const uint32_t i = r * 64 + lane;
         key64 k = i < m ? K[beg + i] : ~0ull;
         uint32_t p = i;
-        if (i < m) s_id[i] = I[beg + i];
+        if (i < m) s_id[i] = (L.flags & F_IDENT) ? beg + i : I[beg + i];
         wave_sort64(k, p);
         s_k[0][i] = k;
         s_p[0][i] = p;
@@ -595,12 +599,12 @@ __global__ __launch_bounds__(kWG) void k_leaf_run(const Item* items, const uint3
     const bool term = (key & 0xFFull) == 0;
     const bool single = L.size == 1;
     for (uint32_t p = cb + threadIdx.x; p < ce; p += kWG) {
-        const uint32_t id = I[p];
+        const uint32_t id = (L.flags & F_IDENT) ? p : I[p];
         if (single || term) {
             B.perm[p] = id;
             if (p > L.begin && B.lcp) B.lcp[p] = depth + key_depth(key);
         } else {
-            if (src) B.idx0[p] = id;
+            if (src || (L.flags & F_IDENT)) B.idx0[p] = id;
             if (id < B.n) B.key0[p] = load_key(B.arena, B.alen, str_off(B, id) + depth + 8);
             else flag_error(B.cnt, ERR_BOUNDS);
         }
@@ -668,6 +672,21 @@ __global__ __launch_bounds__(kWG) void k_dist_chars(const uint8_t* arena, const 
     const uint64_t p = (uint64_t)offs[perm[i]] + lcp[i];
     if (p >= alen) { flag_error(cnt, ERR_BOUNDS); return; }
     cache[i] = arena[p];
+}
+
+// offs64[i] = base[s] + offs32[i] for the set s holding string i
+__global__ __launch_bounds__(kWG) void k_widen(const uint32_t* offs32, const uint64_t* first, const uint64_t* base,
+                                               uint32_t nsets, uint64_t n, uint64_t* offs64)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x;
+    if (i >= n) return;
+    uint32_t lo = 0, hi = nsets;   // largest s with first[s] <= i
+    while (lo + 1 < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (first[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    offs64[i] = base[lo] + offs32[i];
 }
 
 // plain copy of 16-byte words (calibrates the HBM byte counters)
@@ -756,6 +775,9 @@ struct ps5_ctx {
     uint32_t cap_large = 0, cap_block = 0, cap_tiny = 0, cap_run = 0;
     uint32_t* d_cnt = nullptr;
     uint32_t* h_cnt = nullptr;       // pinned
+    uint32_t* h_upcnt = nullptr;     // pinned: counters uploaded with batch roots
+    Item* h_roots = nullptr;         // pinned: root items of a batch
+    uint64_t roots_cap = 0;
     Item* h_large = nullptr;         // pinned copy of a large list
     // multi-workgroup step of a level: descriptors and tile table
     LargeDesc* d_desc = nullptr;
@@ -775,6 +797,8 @@ struct ps5_ctx {
     unsigned long long* h_sums = nullptr;
     // tuning
     uint32_t leaf_target = 256;      // strings per block the multi-workgroup levels aim for
+    uint32_t max_treebits = kMaxTreebits;   // splitter-tree depth limit of one level
+    uint32_t max_oversample = 32;    // sample keys per splitter (limited by kMaxSamples)
     uint64_t seed = 0x5EED5EEDull;
     // profiling
     bool profile = false;
@@ -842,8 +866,10 @@ int grow_level(ps5_ctx* c, uint32_t nlarge, uint64_t hl)
 int ensure_capacity(ps5_ctx* c, uint64_t n)
 {
     if (n <= c->cap) return PS5_OK;
+    // a context that has to grow again grows by half at least
+    uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(n, 1024), c->cap + c->cap / 2);
+    cap = std::min<uint64_t>(cap, std::max<uint64_t>(n, 0xFFFFFF00ull));
     free_buffers(c);
-    uint64_t cap = std::max<uint64_t>(n, 1024);
     for (int i = 0; i < 2; ++i) {
         HIPCHK(hipMalloc(&c->key[i], cap * sizeof(key64)));
         HIPCHK(hipMalloc(&c->idx[i], cap * sizeof(uint32_t)));
@@ -970,12 +996,12 @@ void sum_sizes(ps5_ctx* c, const Item* list, uint32_t first, const uint32_t* cnt
 // Treebits of a multi-workgroup step on m strings: the fewest levels of at
 // most 1023 splitters that bring buckets down to about leaf_target strings,
 // with the fan-out spread evenly over those levels.
-int large_treebits(uint32_t m, uint32_t target)
+int large_treebits(uint32_t m, uint32_t target, int maxbits)
 {
     const double need = std::log2(std::max(2.0, (double)m / target));
-    const int levels = std::max(1, (int)std::ceil(need / kMaxTreebits));
+    const int levels = std::max(1, (int)std::ceil(need / maxbits));
     const int k = (int)std::ceil(need / levels);
-    return std::max(1, std::min(kMaxTreebits, k));
+    return std::max(1, std::min(maxbits, k));
 }
 
 // One multi-workgroup sample-sort step over all large segments of a level.
@@ -990,9 +1016,9 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         D.seg = seg;
         D.T = T;
         D.hist_base = (uint32_t)hl;
-        D.k = (uint32_t)large_treebits(seg.size, c->leaf_target);
+        D.k = (uint32_t)large_treebits(seg.size, c->leaf_target, (int)c->max_treebits);
         const uint32_t nspl = (1u << D.k) - 1;
-        D.oversample = std::max<uint32_t>(2, std::min<uint32_t>(32, (kMaxSamples / nspl) & ~1u));
+        D.oversample = std::max<uint32_t>(2, std::min<uint32_t>(c->max_oversample, (kMaxSamples / nspl) & ~1u));
         if (ntiles + T > c->tiles_cap) return set_err(PS5_ERR_CAPACITY, "tile table capacity");
         for (uint32_t t = 0; t < T; ++t) c->h_tiles[ntiles + t] = TileRef{ i, t };
         ntiles += T;
@@ -1047,29 +1073,41 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
         hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, st, c->d_cnt, list, ci, n);
         return PS5_OK;
     }
-    std::vector<Item> tiny, block, large;
-    for (uint32_t s = 0; s < nsets; ++s) {
-        const uint64_t b = set_first[s], e = set_first[s + 1];
-        if (e < b || e > n) return set_err(PS5_ERR_ARG, "set boundaries must be non-decreasing and <= n");
-        if (e == b) continue;
-        Item it = { (uint32_t)b, (uint32_t)(e - b), (uint32_t)b, 0u };
-        if (it.size <= kTinyT) tiny.push_back(it);
-        else if (it.size <= kBlkT) block.push_back(it);
-        else large.push_back(it);
+    // root items staged in pinned memory and copied on the context's stream
+    // (a blocking copy on the null stream would wait for every other
+    // context's work on the device)
+    if ((uint64_t)nsets > c->roots_cap) {
+        const uint64_t want = std::max<uint64_t>(nsets, 2 * c->roots_cap);
+        if (c->h_roots) (void)hipHostFree(c->h_roots);
+        c->h_roots = nullptr;
+        c->roots_cap = 0;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_roots), want * sizeof(Item)));
+        c->roots_cap = want;
     }
-    if (large.size() > c->cap_large || block.size() > c->cap_block || tiny.size() > c->cap_tiny)
-        return set_err(PS5_ERR_CAPACITY, "too many sets");
-    std::vector<uint32_t> cnt(C_NCNT, 0);
-    cnt[C_TINY] = (uint32_t)tiny.size();
-    cnt[C_BLOCK] = (uint32_t)block.size();
-    cnt[C_LROUND] = (uint32_t)large.size();
-    HIPCHK(hipStreamSynchronize(st));
-    if (!tiny.empty()) HIPCHK(hipMemcpy(c->l_tiny[0], tiny.data(), tiny.size() * sizeof(Item), hipMemcpyHostToDevice));
-    if (!block.empty())
-        HIPCHK(hipMemcpy(c->l_block[0], block.data(), block.size() * sizeof(Item), hipMemcpyHostToDevice));
-    if (!large.empty())
-        HIPCHK(hipMemcpy(c->l_round[0], large.data(), large.size() * sizeof(Item), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_cnt, cnt.data(), C_NCNT * sizeof(uint32_t), hipMemcpyHostToDevice));
+    uint32_t nt = 0, nb = 0, nl = 0;
+    for (int pass = 0; pass < 3; ++pass) {   // tiny, then block, then large items
+        for (uint32_t s = 0; s < nsets; ++s) {
+            const uint64_t b = set_first[s], e = set_first[s + 1];
+            if (e < b || e > n) return set_err(PS5_ERR_ARG, "set boundaries must be non-decreasing and <= n");
+            if (e == b) continue;
+            const Item it = { (uint32_t)b, (uint32_t)(e - b), (uint32_t)b, F_IDENT };
+            const int cls = it.size <= kTinyT ? 0 : it.size <= kBlkT ? 1 : 2;
+            if (cls != pass) continue;
+            c->h_roots[nt + nb + nl] = it;
+            (pass == 0 ? nt : pass == 1 ? nb : nl)++;
+        }
+    }
+    if (nl > c->cap_large || nb > c->cap_block || nt > c->cap_tiny) return set_err(PS5_ERR_CAPACITY, "too many sets");
+    HIPCHK(hipStreamSynchronize(st));     // h_upcnt may still feed an earlier copy
+    for (int i = 0; i < C_NCNT; ++i) c->h_upcnt[i] = 0;
+    c->h_upcnt[C_TINY] = nt;
+    c->h_upcnt[C_BLOCK] = nb;
+    c->h_upcnt[C_LROUND] = nl;
+    const Item* h = c->h_roots;
+    if (nt) HIPCHK(hipMemcpyAsync(c->l_tiny[0], h, nt * sizeof(Item), hipMemcpyHostToDevice, st));
+    if (nb) HIPCHK(hipMemcpyAsync(c->l_block[0], h + nt, nb * sizeof(Item), hipMemcpyHostToDevice, st));
+    if (nl) HIPCHK(hipMemcpyAsync(c->l_round[0], h + nt + nb, nl * sizeof(Item), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->d_cnt, c->h_upcnt, C_NCNT * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     return PS5_OK;
 }
 
@@ -1098,7 +1136,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     B.bktrank = c->bktrank;
 
     {
-        Launch L(c, K_INIT, (uint64_t)n * 24);
+        Launch L(c, K_INIT, (uint64_t)n * 20);   // offs 4 R + 8 characters R + key 8 W
         hipLaunchKernelGGL(k_init, dim3(div_up(n, kWG)), dim3(kWG), 0, st, B, n);
     }
     if ((rc = setup_roots(c, n, set_first, nsets))) return rc;
@@ -1208,6 +1246,7 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_cnt, C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_cnt), C_NCNT * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_upcnt), C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&c->d_sums, S_NUM * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_sums), S_NUM * sizeof(unsigned long long));
     if (e != hipSuccess) {
@@ -1228,6 +1267,8 @@ void ps5_ctx_destroy(ps5_ctx* c)
     if (c->d_sums) (void)hipFree(c->d_sums);
     if (c->d_sets) (void)hipFree(c->d_sets);
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+    if (c->h_upcnt) (void)hipHostFree(c->h_upcnt);
+    if (c->h_roots) (void)hipHostFree(c->h_roots);
     if (c->h_sums) (void)hipHostFree(c->h_sums);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto& u : c->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
@@ -1335,6 +1376,29 @@ int ps5_sort_device_timed(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_byt
     return rc;
 }
 
+int ps5_ctx_set_tuning(ps5_ctx* c, uint32_t leaf_target, uint32_t max_treebits, uint32_t max_oversample)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
+    if (leaf_target < kTinyT || leaf_target > kBlkT) return set_err(PS5_ERR_ARG, "leaf_target must be in [64, 1024]");
+    if (max_treebits < 1 || max_treebits > (uint32_t)kMaxTreebits)
+        return set_err(PS5_ERR_ARG, "max_treebits must be in [1, 10]");
+    if (max_oversample < 2 || max_oversample > 32 || (max_oversample & 1))
+        return set_err(PS5_ERR_ARG, "max_oversample must be even and in [2, 32]");
+    c->leaf_target = leaf_target;
+    c->max_treebits = max_treebits;
+    c->max_oversample = max_oversample;
+    return PS5_OK;
+}
+
+int ps5_ctx_get_tuning(ps5_ctx* c, uint32_t* leaf_target, uint32_t* max_treebits, uint32_t* max_oversample)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
+    if (leaf_target) *leaf_target = c->leaf_target;
+    if (max_treebits) *max_treebits = c->max_treebits;
+    if (max_oversample) *max_oversample = c->max_oversample;
+    return PS5_OK;
+}
+
 int ps5_ctx_set_profile(ps5_ctx* c, int on)
 {
     if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
@@ -1413,3 +1477,19 @@ int ps5_device_synchronize(void)
 }
 
 }  // extern "C"
+
+namespace ps5 {
+
+int widen_offsets(ps5_ctx* c, const uint32_t* d_offs32, const uint64_t* d_first, const uint64_t* d_base,
+                  uint32_t nsets, uint64_t n, uint64_t* d_offs64)
+{
+    if (!c || nsets == 0) return set_err(PS5_ERR_ARG, "widen_offsets: no context or no sets");
+    if (n == 0) return PS5_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_widen, dim3(div_up(n, kWG)), dim3(kWG), 0, c->stream, d_offs32, d_first, d_base, nsets, n,
+                       d_offs64);
+    HIPCHK(hipGetLastError());
+    return PS5_OK;
+}
+
+}  // namespace ps5

@@ -2,13 +2,15 @@
 // checking and RESULT-line conventions (src/psstest.cpp), for the MI355X
 // contestants registered below.
 //
-//   psstest_gpu [-a substr]... [-A name]... [-s size] [-r repeats] [-N] [-L] input
+//   psstest_gpu [-a substr]... [-A name]... [-s size] [-r repeats] [-N] [-L] [--suffix] input
 //
 // input: random2 | random4 | random10 | random62 | random255 (the reference's
 // generators, src/tools/input.hpp:425-468), ascii (length U[1,64] over bytes
 // 33..126), or a file whose lines become the strings (load_plain,
-// src/tools/input.hpp:225-300).  -s limits the input size in bytes (K/M/G
-// suffixes).  Each run times the sort call only (CLOCK_MONOTONIC wall clock,
+// src/tools/input.hpp:225-300), possibly compressed with its decompressed size
+// in the name, e.g. text.1000000.gz (load_compressed, input.hpp:303-422).
+// --suffix sorts every suffix of the file instead (src/psstest.cpp:502-518).
+// -s limits the input size in bytes (K/M/G suffixes).  Each run times the sort call only (CLOCK_MONOTONIC wall clock,
 // src/psstest.cpp:593-603), then checks order, permutation and (for LCP
 // contestants) every LCP value and (for LCP-cache contestants) every
 // distinguishing character, as verify_lcp_cache (src/tools/stringptr.hpp:977-1031).
@@ -20,6 +22,9 @@
 #include <cstring>
 #include <ctime>
 #include <string>
+#include <csignal>
+#include <getopt.h>
+#include <sys/wait.h>
 #include <unistd.h>
 #include <vector>
 
@@ -84,6 +89,82 @@ static uint64_t parse_size(const char* s)
     return (uint64_t)(v * (double)m);
 }
 
+static bool g_suffix = false;   // --suffix: sort all suffixes of the input file
+
+// Whole file, at most `limit` bytes (load_plain, src/tools/input.hpp:225-300);
+// data gets 16 zero bytes of padding.
+static bool read_plain(const std::string& path, uint64_t limit, std::vector<uint8_t>& data, uint64_t& fs)
+{
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) { fprintf(stderr, "cannot open %s: %s\n", path.c_str(), strerror(errno)); return false; }
+    fseek(f, 0, SEEK_END);
+    fs = (uint64_t)ftell(f);
+    rewind(f);
+    if (limit && fs > limit) fs = limit;
+    data.assign(fs + 16, 0);
+    const bool ok = fread(data.data(), 1, fs, f) == fs;
+    fclose(f);
+    if (!ok) fprintf(stderr, "cannot read %s\n", path.c_str());
+    return ok;
+}
+
+// .gz/.bz2/.xz/.lzo/.zst files whose name carries the decompressed size
+// before the extension ("name.<bytes>.gz"), read through the decompressor
+// (load_compressed, src/tools/input.hpp:303-422).  handled = false when the
+// name has no such extension.
+static bool read_compressed(const std::string& path, uint64_t limit, std::vector<uint8_t>& data, uint64_t& fs,
+                            bool& handled)
+{
+    static const struct { const char* ext; const char* prog; } tools[] = {
+        { ".gz", "gzip" }, { ".bz2", "bzip2" }, { ".xz", "xz" }, { ".lzo", "lzop" }, { ".zst", "zstd" },
+    };
+    const char* prog = nullptr;
+    for (auto& t : tools) {
+        const size_t el = strlen(t.ext);
+        if (path.size() > el && path.compare(path.size() - el, el, t.ext) == 0) prog = t.prog;
+    }
+    handled = prog != nullptr;
+    if (!prog) return true;
+    const std::string::size_type dot = path.rfind('.');
+    std::string::size_type i = dot;
+    uint64_t size = 0, v = 1;
+    while (i > 0 && isdigit((unsigned char)path[i - 1])) {
+        size += (uint64_t)(path[i - 1] - '0') * v;
+        v *= 10;
+        --i;
+    }
+    if (size == 0 || i == 0 || path[i - 1] != '.') {
+        fprintf(stderr, "could not find the decompressed size in the file name %s\n", path.c_str());
+        return false;
+    }
+    if (limit && size > limit) size = limit;
+    int fd[2];
+    if (pipe(fd) != 0) { fprintf(stderr, "pipe: %s\n", strerror(errno)); return false; }
+    const pid_t pid = fork();
+    if (pid < 0) { fprintf(stderr, "fork: %s\n", strerror(errno)); return false; }
+    if (pid == 0) {     // child: the decompressor writes into the pipe
+        close(fd[0]);
+        dup2(fd[1], STDOUT_FILENO);
+        execlp(prog, prog, "-dc", path.c_str(), (char*)nullptr);
+        _exit(127);
+    }
+    close(fd[1]);
+    data.assign(size + 16, 0);
+    uint64_t pos = 0;
+    while (pos < size) {
+        const ssize_t rb = read(fd[0], data.data() + pos, std::min<uint64_t>(8u << 20, size - pos));
+        if (rb <= 0) break;
+        pos += (uint64_t)rb;
+    }
+    close(fd[0]);
+    kill(pid, SIGTERM);
+    int status = 0;
+    waitpid(pid, &status, 0);
+    if (pos < size) { fprintf(stderr, "%s: %s gave %" PRIu64 " of %" PRIu64 " bytes\n", path.c_str(), prog, pos, size); return false; }
+    fs = size;
+    return true;
+}
+
 static bool load_input(const std::string& path, uint64_t size, Input& in)
 {
     static const struct { const char* name; const char* letters; } gens[] = {
@@ -124,18 +205,22 @@ static bool load_input(const std::string& path, uint64_t size, Input& in)
         in.offs.resize(n);
         return true;
     }
-    FILE* f = fopen(path.c_str(), "rb");
-    if (!f) { fprintf(stderr, "cannot open %s: %s\n", path.c_str(), strerror(errno)); return false; }
-    fseek(f, 0, SEEK_END);
-    uint64_t fs = (uint64_t)ftell(f);
-    rewind(f);
-    if (size && fs > size) fs = size;
-    in.data.assign(fs + 16, 0);
-    if (fread(in.data.data(), 1, fs, f) != fs) { fclose(f); return false; }
-    fclose(f);
+    uint64_t fs = 0;
+    bool handled = false;
+    if (!read_compressed(path, size, in.data, fs, handled)) return false;
+    if (!handled && !read_plain(path, size, in.data, fs)) return false;
+    if (fs == 0) { fprintf(stderr, "%s is empty\n", path.c_str()); return false; }
+    if (fs > 0xFFFFFFF0ull) { fprintf(stderr, "%s: input above 4 GiB\n", path.c_str()); return false; }
+    if (g_suffix) {   // every position starts a string (src/psstest.cpp:502-518)
+        in.data[fs - 1] = 0;
+        in.offs.resize(fs);
+        for (uint64_t i = 0; i < fs; ++i) in.offs[i] = (uint32_t)i;
+        in.name += " (suffixes)";
+        return true;
+    }
     for (uint64_t i = 0; i < fs; ++i)
         if (in.data[i] == '\n') in.data[i] = 0;
-    if (fs) in.data[fs - 1] = 0;
+    in.data[fs - 1] = 0;
     for (uint64_t i = 0; i < fs; ++i)
         if (i == 0 || in.data[i - 1] == 0) in.offs.push_back((uint32_t)i);
     return true;
@@ -169,8 +254,10 @@ int main(int argc, char** argv)
     uint64_t size = 0;
     int repeats = 1, opt;
     bool no_check = false, list = false;
-    while ((opt = getopt(argc, argv, "a:A:s:r:NL")) != -1) {
+    static const option longopts[] = { { "suffix", no_argument, nullptr, 'S' }, { nullptr, 0, nullptr, 0 } };
+    while ((opt = getopt_long(argc, argv, "a:A:s:r:NLS", longopts, nullptr)) != -1) {
         switch (opt) {
+        case 'S': g_suffix = true; break;
         case 'a': sel.push_back(optarg); break;
         case 'A': sel_full.push_back(optarg); break;
         case 's': size = parse_size(optarg); break;
@@ -178,7 +265,8 @@ int main(int argc, char** argv)
         case 'N': no_check = true; break;
         case 'L': list = true; break;
         default:
-            fprintf(stderr, "usage: %s [-a substr] [-A name] [-s size] [-r repeats] [-N] [-L] input\n", argv[0]);
+            fprintf(stderr, "usage: %s [-a substr] [-A name] [-s size] [-r repeats] [-N] [-L] [--suffix] input\n",
+                    argv[0]);
             return 1;
         }
     }

@@ -29,6 +29,7 @@ EXPORTS = [
     "ps5_gen_uniform", "ps5_gen_windows", "ps5_gen_reference_random", "ps5_gen_prefixed",
     "ps5_sort_batch_device", "ps5_gen_batch_plan", "ps5_gen_batch_fill",
     "ps5_server_create", "ps5_server_destroy", "ps5_server_submit", "ps5_server_wait", "ps5_server_stats",
+    "ps5_ctx_set_tuning", "ps5_ctx_get_tuning", "ps5_autotune_device", "ps5_autotune_batch_device",
 ]
 
 
@@ -98,6 +99,12 @@ def load():
     _sig(lib, "ps5_server_wait", _I, [_P, _U64, ctypes.POINTER(ctypes.c_double)])
     _sig(lib, "ps5_server_stats", _I, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(_U64), ctypes.POINTER(_U64), ctypes.POINTER(ctypes.c_double)])
+    _sig(lib, "ps5_ctx_set_tuning", _I, [_P, _U32, _U32, _U32])
+    _sig(lib, "ps5_ctx_get_tuning", _I, [_P] + [ctypes.POINTER(_U32)] * 3)
+    _sig(lib, "ps5_autotune_device", _I, [_P, _P, _U64, _P, _U64, _P, _P] + [ctypes.POINTER(_U32)] * 3
+         + [ctypes.POINTER(ctypes.c_double)])
+    _sig(lib, "ps5_autotune_batch_device", _I, [_P, _P, _U64, _P, _P, _U32, _P, _P] + [ctypes.POINTER(_U32)] * 3
+         + [ctypes.POINTER(ctypes.c_double)])
     _lib = lib
     return lib
 
@@ -330,6 +337,33 @@ class Context:
                     b.free()
         return perm, lcp
 
+    def set_tuning(self, leaf_target=256, max_treebits=10, max_oversample=32):
+        _check(load().ps5_ctx_set_tuning(self.h, leaf_target, max_treebits, max_oversample), "ps5_ctx_set_tuning")
+
+    def get_tuning(self):
+        v = [_U32() for _ in range(3)]
+        _check(load().ps5_ctx_get_tuning(self.h, *[ctypes.byref(x) for x in v]), "ps5_ctx_get_tuning")
+        return tuple(x.value for x in v)
+
+    def autotune_device(self, d_arena, arena_bytes, d_offs, n, d_perm, d_lcp=None):
+        """Tune on this input; returns ((leaf_target, max_treebits, max_oversample), best ms)."""
+        v = [_U32() for _ in range(3)]
+        ms = ctypes.c_double()
+        _check(load().ps5_autotune_device(self.h, d_arena.ptr, arena_bytes, d_offs.ptr, n, d_perm.ptr,
+                                          d_lcp.ptr if d_lcp is not None else None,
+                                          *[ctypes.byref(x) for x in v], ctypes.byref(ms)), "ps5_autotune_device")
+        return tuple(x.value for x in v), ms.value
+
+    def autotune_batch_device(self, d_arena, arena_bytes, d_offs64, set_first, d_perm, d_lcp=None):
+        sf = np.ascontiguousarray(set_first, np.uint64)
+        v = [_U32() for _ in range(3)]
+        ms = ctypes.c_double()
+        _check(load().ps5_autotune_batch_device(self.h, d_arena.ptr, arena_bytes, d_offs64.ptr, _ptr(sf), len(sf) - 1,
+                                                d_perm.ptr, d_lcp.ptr if d_lcp is not None else None,
+                                                *[ctypes.byref(x) for x in v], ctypes.byref(ms)),
+               "ps5_autotune_batch_device")
+        return tuple(x.value for x in v), ms.value
+
     def set_profile(self, on):
         _check(load().ps5_ctx_set_profile(self.h, 1 if on else 0), "ps5_ctx_set_profile")
 
@@ -416,3 +450,18 @@ def sort_strings(strings, with_lcp=False, with_cache=False):
     order = [addr[arr[i]] for i in range(n)]
     out = [strings[i] for i in order], order, (lcp[:n] if with_lcp else None)
     return out + (cache[:n],) if with_cache else out
+
+
+def sort_suffixes(text, with_lcp=True):
+    """Sort every suffix of `text` (bytes) through the Contest-shaped host
+    entry, with the pointers into one buffer as psstest --suffix lays them out:
+    returns (order of suffix start positions, lcp or None)."""
+    lib = load()
+    n = len(text)
+    buf = ctypes.create_string_buffer(bytes(text), n + 16)
+    base = ctypes.addressof(buf)
+    arr = (ctypes.c_void_p * max(1, n))(*[base + i for i in range(n)])
+    lcp = np.full(max(1, n), 42, np.uint64) if with_lcp else None
+    _check(lib.ps5_sort_strings(arr, n, _ptr(lcp) if with_lcp else None, None), "ps5_sort_strings")
+    order = np.array([arr[i] - base for i in range(n)], np.int64)
+    return order, (lcp[:n] if with_lcp else None)

@@ -156,6 +156,102 @@ def test_host_entry_lcp_cache(ps5, oracle):
     assert np.array_equal(cache[1:], oracle.dist_chars(arena, offs, bp, bl)[1:])
 
 
+@pytest.mark.parametrize("kind", ["random4", "repeats"])
+def test_host_entry_suffixes(ps5, oracle, kind):
+    """suffixes of one text (psstest --suffix): the host entry copies the text
+    once instead of packing every suffix; deep LCPs take many rounds"""
+    rng = np.random.default_rng(5)
+    if kind == "random4":
+        text = bytes(rng.choice(list(b"ACGT"), 20000).tolist())
+    else:
+        unit = bytes(rng.choice(list(b"ab"), 37).tolist())
+        text = unit * 60 + bytes(rng.choice(list(b"abc"), 500).tolist())
+    text = text[:-1] + b"\0"   # the last byte is forced to NUL as psstest does
+    order, lcp = ps5.sort_suffixes(text)
+    arena = np.frombuffer(text + b"\0" * 16, np.uint8).copy()
+    offs = np.arange(len(text), dtype=np.uint32)
+    bp, bl = oracle.brute_sort(arena, offs)
+    assert np.array_equal(order, bp.astype(np.int64))
+    assert lcp[0] == 42 and np.array_equal(lcp[1:].astype(np.uint32), bl[1:])
+
+
+def test_runner_suffix_and_compressed(tmp_path):
+    """psstest_gpu reads a gzip file whose name gives the decompressed size,
+    as lines and as suffixes, and every contestant's output checks out"""
+    import gzip
+    import subprocess
+    rng = np.random.default_rng(8)
+    lines = [bytes(rng.choice(list(b"abcdef"), int(rng.integers(0, 30))).tolist()) for _ in range(3000)]
+    raw = b"\n".join(lines) + b"\n"
+    path = tmp_path / f"lines.{len(raw)}.gz"
+    path.write_bytes(gzip.compress(raw))
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "parallel-string-sorting_amd", "psstest_gpu")
+    for extra in ([], ["--suffix"]):
+        r = subprocess.run([exe, "-a", "mi355x", *extra, str(path)], capture_output=True, text=True, timeout=120)
+        say(r.stdout[-2000:], r.stderr[-2000:])
+        assert r.returncode == 0
+        res = [ln for ln in r.stdout.splitlines() if ln.startswith("RESULT")]
+        assert len(res) == 4 and all("status=ok" in ln for ln in res)
+        count = len(raw) if extra else raw.count(b"\n")
+        assert all(f"string_count={count}" in ln for ln in res)
+
+
+@pytest.mark.parametrize("tuning", [(64, 1, 2), (96, 4, 4), (512, 7, 8), (1024, 10, 32)])
+def test_tuning_does_not_change_output(ps5, oracle, tuning):
+    """every shape setting -- tiny trees with many levels, huge blocks, little
+    oversampling -- sorts to the same, exact output"""
+    say("tuning", tuning)
+    c = ps5.Context(0)
+    try:
+        c.set_tuning(*tuning)
+        assert c.get_tuning() == tuning
+        arena, offs = ps5.workload("ascii2gb", scale=1 / 256)
+        perm, lcp = c.sort(arena, offs)
+        op, ol = oracle.ps5_sort(arena, offs)
+        assert np.array_equal(perm, op) and np.array_equal(lcp[1:], ol[1:])
+        strs = random_strings(50000, 0, 10, b"ab", 31)
+        a2, o2 = pack(strs)
+        p2, l2 = c.sort(a2, o2)
+        bp, bl = oracle.brute_sort(a2, o2)
+        assert np.array_equal(p2, bp) and np.array_equal(l2[1:], bl[1:])
+    finally:
+        c.close()
+
+
+def test_tuning_rejects_bad_values(ps5):
+    c = ps5.Context(0)
+    try:
+        for bad in [(63, 10, 32), (1025, 10, 32), (256, 0, 32), (256, 11, 32), (256, 10, 3), (256, 10, 34)]:
+            with pytest.raises(ps5.Ps5Error):
+                c.set_tuning(*bad)
+    finally:
+        c.close()
+
+
+def test_autotune_picks_a_valid_setting(ps5, oracle):
+    """the tuner leaves a setting in the context and the sort after it is exact"""
+    arena, offs = ps5.workload("random10mb")
+    n = len(offs)
+    c = ps5.Context(0)
+    bufs = [ps5.DeviceBuffer(arena.size), ps5.DeviceBuffer(4 * n), ps5.DeviceBuffer(4 * n), ps5.DeviceBuffer(4 * n)]
+    try:
+        da, do, dp, dl = bufs
+        da.upload(arena)
+        do.upload(offs)
+        best, ms = c.autotune_device(da, int(arena.size), do, n, dp, dl)
+        say("autotune", best, ms)
+        assert c.get_tuning() == best and ms > 0
+        assert 64 <= best[0] <= 1024 and 1 <= best[1] <= 10 and 2 <= best[2] <= 32
+        c.sort_device(da, int(arena.size), do, n, dp, dl)
+        perm, lcp = dp.download(np.uint32, n), dl.download(np.uint32, n)
+        assert oracle.verify(arena, offs, perm, lcp)[0] == 0
+    finally:
+        for b in bufs:
+            b.free()
+        c.close()
+
+
 def golden_files():
     if not os.path.isdir(GOLDEN):
         return []
@@ -212,12 +308,13 @@ def test_batch_edge_sets(ps5, ctx, oracle):
     _check_batch(oracle, arena, offs32.astype(np.uint64), sf, perm, lcp)
 
 
-@pytest.mark.parametrize("budget_mb,slots", [(4096, 4), (2, 2)])
-def test_serve_mix_equals_alone(ps5, ctx, oracle, budget_mb, slots):
+@pytest.mark.parametrize("budget_mb,slots,max_batch", [(4096, 4, 0), (2, 2, 0), (512, 3, 1_000_000)])
+def test_serve_mix_equals_alone(ps5, ctx, oracle, budget_mb, slots, max_batch):
     """requests served concurrently (continuous batching) give exactly the
     result each gets when sorted alone -- with a large budget (many requests
-    per batch) and a tiny one (requests wait for memory, one batch each)"""
-    say("serve", budget_mb, slots)
+    per batch), a tiny one (requests wait for memory, one batch each) and
+    pre-allocated slots whose batches are capped by string count"""
+    say("serve", budget_mb, slots, max_batch)
     reqs = []
     rng = np.random.default_rng(77)
     for i in range(24):
@@ -230,7 +327,7 @@ def test_serve_mix_equals_alone(ps5, ctx, oracle, budget_mb, slots):
             a, o = ps5.gen_uniform(3000 + i, size, 1, 64, ps5.ALPHA_ASCII94)
         reqs.append((a, o))
     alone = [ctx.sort(a, o) for a, o in reqs]
-    srv = ps5.Server(0, hbm_budget=budget_mb << 20, slots=slots)
+    srv = ps5.Server(0, hbm_budget=budget_mb << 20, slots=slots, max_batch_strings=max_batch)
     try:
         ids = [srv.submit(a, o) for a, o in reqs]
         mixed = [srv.wait(i) for i in ids]
