@@ -203,8 +203,12 @@ __device__ __forceinline__ void wave_append_segment(bool pred, Item it, const Si
     uint32_t slot;
     slot = wave_append(t, &s.cnt[s.c_tiny]);
     if (t) put_item(s.tiny, s.cap_tiny, slot, it, s.cnt);
-    slot = wave_append(bl, &s.cnt[s.c_block]);
-    if (bl) put_item(s.block, s.cap_block, slot, it, s.cnt);
+#pragma unroll
+    for (int c = 0; c < kBlkClasses; ++c) {
+        const bool bc = bl && block_class(m) == c;
+        slot = wave_append(bc, &s.cnt[s.c_block + 2 * c]);
+        if (bc) put_item(s.block + (uint64_t)c * s.cap_block, s.cap_block, slot, it, s.cnt);
+    }
     slot = wave_append(lg, &s.cnt[s.c_large]);
     if (lg) put_item(s.large, s.cap_large, slot, it, s.cnt);
 }

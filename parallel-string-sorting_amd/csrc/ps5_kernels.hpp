@@ -39,6 +39,10 @@ constexpr int kTile = kWaveChunk * kWaves;   // keys per count workgroup
 constexpr uint32_t kTinyT = 64;              // leaves sorted by one wave
 constexpr uint32_t kBlkT = 1024;             // leaves sorted on chip by one workgroup
 constexpr uint32_t kRunChunk = 4096;         // equal-key leaves are processed in chunks
+// Blocks are kept in three size classes, each sorted by a kernel whose LDS
+// fits its largest block: (64, 256], (256, 512], (512, 1024].
+constexpr int kBlkClasses = 3;
+__host__ __device__ constexpr int block_class(uint32_t m) { return m <= 256 ? 0 : m <= 512 ? 1 : 2; }
 constexpr int kRankBits = 21;
 constexpr uint32_t kRankMask = (1u << kRankBits) - 1;
 constexpr uint32_t kNoBucket = 0x7FF;        // bucket id of inactive lanes
@@ -58,9 +62,9 @@ enum : int {
     C_LROUND = 0,  // [2] large segments starting a round (round parity)
     C_LLVL = 2,    // [2] large segments of the next level (level parity)
     C_TINY = 4,    // [2] leaf lists (round parity)
-    C_BLOCK = 6,   // [2]
-    C_RUN = 8,     // [2]
-    C_ERR = 10,    // error flags
+    C_BLOCK = 6,   // [3][2] blocks: C_BLOCK + 2 * size class + round parity
+    C_RUN = 12,    // [2]
+    C_ERR = 14,    // error flags
     C_NCNT = 16
 };
 constexpr uint32_t ERR_CAP = 1u, ERR_LCP = 2u, ERR_BOUNDS = 4u;
@@ -81,8 +85,9 @@ struct TileRef {         // one workgroup of the multi-workgroup step
 struct Sink {            // where new items are appended
     Item* large; Item* block; Item* tiny; Item* run;
     uint32_t* cnt;       // counter base
-    int c_large, c_block, c_tiny, c_run;     // counter indices
-    uint32_t cap_large, cap_block, cap_tiny, cap_run;
+    int c_large, c_block, c_tiny, c_run;     // counter indices (c_block: of size class 0)
+    uint32_t cap_large, cap_block, cap_tiny, cap_run;   // cap_block: per size class;
+                         // class c lives at block + c * cap_block, counter c_block + 2 * c
 };
 
 struct Bufs {

@@ -56,14 +56,11 @@ __global__ void k_setup(uint32_t* cnt, Item* list, int c_index, uint32_t n)
     }
 }
 
-__global__ void k_reset(uint32_t* cnt, int a, int b, int c, int d, int e)
+// zero the counters whose bit is set in `mask`
+__global__ void k_reset(uint32_t* cnt, uint32_t mask)
 {
     const int t = threadIdx.x;
-    if (t == 0 && a >= 0) cnt[a] = 0;
-    if (t == 1 && b >= 0) cnt[b] = 0;
-    if (t == 2 && c >= 0) cnt[c] = 0;
-    if (t == 3 && d >= 0) cnt[d] = 0;
-    if (t == 4 && e >= 0) cnt[e] = 0;
+    if (t < C_NCNT && ((mask >> t) & 1u)) cnt[t] = 0;
 }
 
 // ---- multi-workgroup sample-sort step (segments > kBlkT) -------------------
@@ -323,7 +320,7 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
 {
     constexpr int kPer = (kMaxBkt + kWG - 1) / kWG;   // 8
     const uint32_t dstbuf = (seg.flags & F_BUFB) ^ F_BUFB;
-    uint32_t c_t = 0, c_b = 0, c_l = 0, c_r = 0;
+    uint32_t c_t = 0, c_l = 0, c_r = 0, c_b[kBlkClasses] = { 0, 0, 0 };
     uint32_t st[kPer], sz[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
@@ -338,26 +335,29 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
             if (m == 0) continue;
             if (m <= kTinyT) c_t++;
             else if (b & 1) c_r += (m + kRunChunk - 1) / kRunChunk;
-            else if (m <= kBlkT) c_b++;
+            else if (m <= kBlkT) c_b[block_class(m)]++;
             else c_l++;
         }
     }
-    uint32_t tt, tb, tl, tr;
+    uint32_t tt, tl, tr, tb[kBlkClasses], o_b[kBlkClasses];
     uint32_t o_t = block_excl_scan<kWaves>(c_t, &tt, s_tmp);
-    uint32_t o_b = block_excl_scan<kWaves>(c_b, &tb, s_tmp);
     uint32_t o_l = block_excl_scan<kWaves>(c_l, &tl, s_tmp);
     uint32_t o_r = block_excl_scan<kWaves>(c_r, &tr, s_tmp);
+#pragma unroll
+    for (int c = 0; c < kBlkClasses; ++c) o_b[c] = block_excl_scan<kWaves>(c_b[c], &tb[c], s_tmp);
     if (threadIdx.x == 0) {
         s_tmp[8] = tt ? atomicAdd(&s.cnt[s.c_tiny], tt) : 0;
-        s_tmp[9] = tb ? atomicAdd(&s.cnt[s.c_block], tb) : 0;
-        s_tmp[10] = tl ? atomicAdd(&s.cnt[s.c_large], tl) : 0;
-        s_tmp[11] = tr ? atomicAdd(&s.cnt[s.c_run], tr) : 0;
+        s_tmp[9] = tl ? atomicAdd(&s.cnt[s.c_large], tl) : 0;
+        s_tmp[10] = tr ? atomicAdd(&s.cnt[s.c_run], tr) : 0;
+#pragma unroll
+        for (int c = 0; c < kBlkClasses; ++c) s_tmp[11 + c] = tb[c] ? atomicAdd(&s.cnt[s.c_block + 2 * c], tb[c]) : 0;
     }
     __syncthreads();
     o_t += s_tmp[8];
-    o_b += s_tmp[9];
-    o_l += s_tmp[10];
-    o_r += s_tmp[11];
+    o_l += s_tmp[9];
+    o_r += s_tmp[10];
+#pragma unroll
+    for (int c = 0; c < kBlkClasses; ++c) o_b[c] += s_tmp[11 + c];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int b = threadIdx.x * kPer + j;
@@ -378,7 +378,8 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
                 put_item(s.run, s.cap_run, o_r++, r, s.cnt);
             }
         } else if (m <= kBlkT) {
-            put_item(s.block, s.cap_block, o_b++, it, s.cnt);
+            const int c = block_class(m);
+            put_item(s.block + (uint64_t)c * s.cap_block, s.cap_block, o_b[c]++, it, s.cnt);
         } else {
             put_item(s.large, s.cap_large, o_l++, it, s.cnt);
         }
@@ -496,7 +497,7 @@ __device__ __forceinline__ uint32_t run_rank(const key64* a, uint32_t len, uint3
     return c;
 }
 
-// One workgroup sorts a block of 65..kBlkT strings completely on chip and
+// One workgroup sorts a block of 65..MAXM strings completely on chip and
 // finishes every position at this depth (the reference's insertion-sort base
 // case, lcp_inssort.hpp:36-187, and the equal-bucket rules, .hpp:1531-1557,
 // on sorted keys).  Runs of 64 keys are sorted in the waves' registers
@@ -506,16 +507,18 @@ __device__ __forceinline__ uint32_t run_rank(const key64* a, uint32_t len, uint3
 // order stays stable).  Finishing a sorted position: LCP to its neighbour,
 // identical strings done, runs of equal keys without a NUL handed to round
 // r+1.
+// (MAXM is the largest block of the size class; LDS holds 28 B per string.)
+template <uint32_t MAXM>
 __global__ __launch_bounds__(kWG) void k_block(const Item* items, uint32_t first, const uint32_t* cnt_items,
                                                 Bufs B, Sink nxt, uint32_t depth)
 {
-    __shared__ key64 s_k[2][kBlkT];
-    __shared__ uint32_t s_p[2][kBlkT];
-    __shared__ uint32_t s_id[kBlkT];
+    __shared__ key64 s_k[2][MAXM];
+    __shared__ uint32_t s_p[2][MAXM];
+    __shared__ uint32_t s_id[MAXM];
     const uint32_t idx = first + blockIdx.x;
     if (idx >= *cnt_items) return;
     const Item L = items[idx];
-    if (!item_ok(L, B) || L.size > kBlkT) {
+    if (!item_ok(L, B) || L.size > MAXM) {
         if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
@@ -623,17 +626,21 @@ __global__ __launch_bounds__(kWG) void k_leaf_run(const Item* items, const uint3
 // LCP between neighbouring leaves of one round segment:
 // depth + lcp_key(last key of the left leaf, first key of the right leaf)
 // (sample_sort_lcp, .hpp:311-396).
-__global__ __launch_bounds__(kWG) void k_leafbound(const Item* t, const Item* b, const Item* r, uint32_t nt,
-                                                   uint32_t nb, uint32_t nr, Bufs B, uint32_t depth)
+struct LeafLists {        // the leaf lists of a round, concatenated in this order
+    const Item* list[kBlkClasses + 2];   // tiny, blocks by class, equal-key chunks
+    uint32_t end[kBlkClasses + 2];       // running totals of the list lengths
+};
+
+__global__ __launch_bounds__(kWG) void k_leafbound(LeafLists LL, Bufs B, uint32_t depth)
 {
+    constexpr int NL = kBlkClasses + 2;
     const uint32_t i = blockIdx.x * kWG + threadIdx.x;
-    Item L;
-    if (i < nt) L = t[i];
-    else if (i < nt + nb) L = b[i - nt];
-    else if (i < nt + nb + nr) {
-        L = r[i - nt - nb];
-        if ((L.flags >> 8) != 0) return;
-    } else return;
+    if (i >= LL.end[NL - 1]) return;
+    int l = 0;
+#pragma unroll
+    for (int j = 0; j < NL - 1; ++j) l += i >= LL.end[j] ? 1 : 0;
+    const Item L = LL.list[l][i - (l ? LL.end[l - 1] : 0u)];
+    if (l == NL - 1 && (L.flags >> 8) != 0) return;   // only the first chunk of an equal-key leaf
     if (L.begin <= L.rbeg || !B.lcp) return;
     if (L.begin >= B.n) { flag_error(B.cnt, ERR_BOUNDS); return; }
     const key64 a = B.bkey[L.begin - 1], c = B.bkey[L.begin];
@@ -885,7 +892,7 @@ int ensure_capacity(ps5_ctx* c, uint64_t n)
         HIPCHK(hipMalloc(&c->l_round[i], (uint64_t)c->cap_large * sizeof(Item)));
         HIPCHK(hipMalloc(&c->l_lvl[i], (uint64_t)c->cap_large * sizeof(Item)));
         HIPCHK(hipMalloc(&c->l_tiny[i], (uint64_t)c->cap_tiny * sizeof(Item)));
-        HIPCHK(hipMalloc(&c->l_block[i], (uint64_t)c->cap_block * sizeof(Item)));
+        HIPCHK(hipMalloc(&c->l_block[i], (uint64_t)kBlkClasses * c->cap_block * sizeof(Item)));
         HIPCHK(hipMalloc(&c->l_run[i], (uint64_t)c->cap_run * sizeof(Item)));
     }
     // every large segment has at most size/kTile + 1 tiles; the splitter
@@ -1067,7 +1074,11 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
     if (!set_first) {
         Item* list; int ci;
         if (n <= kTinyT) { list = c->l_tiny[0]; ci = C_TINY; }
-        else if (n <= kBlkT) { list = c->l_block[0]; ci = C_BLOCK; }
+        else if (n <= kBlkT) {
+            const int cls = block_class(n);
+            list = c->l_block[0] + (uint64_t)cls * c->cap_block;
+            ci = C_BLOCK + 2 * cls;
+        }
         else { list = c->l_round[0]; ci = C_LROUND; }
         Launch L(c, K_MISC, 0);
         hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, st, c->d_cnt, list, ci, n);
@@ -1084,29 +1095,38 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_roots), want * sizeof(Item)));
         c->roots_cap = want;
     }
-    uint32_t nt = 0, nb = 0, nl = 0;
-    for (int pass = 0; pass < 3; ++pass) {   // tiny, then block, then large items
+    // kinds: 0 tiny, 1..kBlkClasses blocks by size class, last large
+    constexpr int NK = kBlkClasses + 2;
+    uint32_t cnt[NK] = {};
+    uint32_t tot = 0;
+    for (int kind = 0; kind < NK; ++kind) {
         for (uint32_t s = 0; s < nsets; ++s) {
             const uint64_t b = set_first[s], e = set_first[s + 1];
             if (e < b || e > n) return set_err(PS5_ERR_ARG, "set boundaries must be non-decreasing and <= n");
             if (e == b) continue;
             const Item it = { (uint32_t)b, (uint32_t)(e - b), (uint32_t)b, F_IDENT };
-            const int cls = it.size <= kTinyT ? 0 : it.size <= kBlkT ? 1 : 2;
-            if (cls != pass) continue;
-            c->h_roots[nt + nb + nl] = it;
-            (pass == 0 ? nt : pass == 1 ? nb : nl)++;
+            const int k = it.size <= kTinyT ? 0 : it.size <= kBlkT ? 1 + block_class(it.size) : NK - 1;
+            if (k != kind) continue;
+            c->h_roots[tot++] = it;
+            cnt[kind]++;
         }
     }
-    if (nl > c->cap_large || nb > c->cap_block || nt > c->cap_tiny) return set_err(PS5_ERR_CAPACITY, "too many sets");
+    if (cnt[NK - 1] > c->cap_large || cnt[0] > c->cap_tiny) return set_err(PS5_ERR_CAPACITY, "too many sets");
+    for (int k = 1; k <= kBlkClasses; ++k)
+        if (cnt[k] > c->cap_block) return set_err(PS5_ERR_CAPACITY, "too many sets");
     HIPCHK(hipStreamSynchronize(st));     // h_upcnt may still feed an earlier copy
     for (int i = 0; i < C_NCNT; ++i) c->h_upcnt[i] = 0;
-    c->h_upcnt[C_TINY] = nt;
-    c->h_upcnt[C_BLOCK] = nb;
-    c->h_upcnt[C_LROUND] = nl;
+    c->h_upcnt[C_TINY] = cnt[0];
+    for (int k = 1; k <= kBlkClasses; ++k) c->h_upcnt[C_BLOCK + 2 * (k - 1)] = cnt[k];
+    c->h_upcnt[C_LROUND] = cnt[NK - 1];
     const Item* h = c->h_roots;
-    if (nt) HIPCHK(hipMemcpyAsync(c->l_tiny[0], h, nt * sizeof(Item), hipMemcpyHostToDevice, st));
-    if (nb) HIPCHK(hipMemcpyAsync(c->l_block[0], h + nt, nb * sizeof(Item), hipMemcpyHostToDevice, st));
-    if (nl) HIPCHK(hipMemcpyAsync(c->l_round[0], h + nt + nb, nl * sizeof(Item), hipMemcpyHostToDevice, st));
+    for (int kind = 0; kind < NK; ++kind) {
+        Item* dst = kind == 0 ? c->l_tiny[0]
+                  : kind == NK - 1 ? c->l_round[0]
+                  : c->l_block[0] + (uint64_t)(kind - 1) * c->cap_block;
+        if (cnt[kind]) HIPCHK(hipMemcpyAsync(dst, h, cnt[kind] * sizeof(Item), hipMemcpyHostToDevice, st));
+        h += cnt[kind];
+    }
     HIPCHK(hipMemcpyAsync(c->d_cnt, c->h_upcnt, C_NCNT * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     return PS5_OK;
 }
@@ -1149,42 +1169,59 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
         c->st_rounds++;
         {   // the lists of round r+1 are empty again (consumed in round r-1)
             Launch L(c, K_MISC, 0);
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, C_LROUND + np, C_TINY + np,
-                               C_BLOCK + np, C_RUN + np, -1);
+            uint32_t mask = (1u << (C_LROUND + np)) | (1u << (C_TINY + np)) | (1u << (C_RUN + np));
+            for (int cls = 0; cls < kBlkClasses; ++cls) mask |= 1u << (C_BLOCK + 2 * cls + np);
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, mask);
         }
         const Sink nsink = make_sink(c, c->l_round[np], C_LROUND + np, np);
-        uint32_t blk_done = 0;
+        uint32_t blk_done[kBlkClasses] = {};
         int cur_ci = C_LROUND + rp;
         for (uint32_t level = 0;; ++level) {
             if (level > 4096) return set_err(PS5_ERR_INTERNAL, "sample-sort levels do not terminate");
             const uint32_t nlarge = c->h_cnt[cur_ci];
-            const uint32_t nblk = c->h_cnt[C_BLOCK + rp] - blk_done;
-            if (nlarge == 0 && nblk == 0) break;
+            uint32_t nblk[kBlkClasses], nblk_all = 0;
+            for (int cls = 0; cls < kBlkClasses; ++cls) {
+                nblk[cls] = c->h_cnt[C_BLOCK + 2 * cls + rp] - blk_done[cls];
+                nblk_all += nblk[cls];
+            }
+            if (nlarge == 0 && nblk_all == 0) break;
             c->st_levels++;
             Item* nx_large = c->l_lvl[level & 1];
             const int nx_ci = C_LLVL + (level & 1);
             {
                 Launch L(c, K_MISC, 0);
-                hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, nx_ci, -1, -1, -1, -1);
+                hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, 1u << nx_ci);
             }
             const Sink sink = make_sink(c, nx_large, nx_ci, rp);
             const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
             if (nlarge && (rc = large_step(c, B, nlarge, sink, seed))) return rc;
-            if (nblk) {
+            for (int cls = 0; cls < kBlkClasses; ++cls) {
+                if (!nblk[cls]) continue;
+                const Item* list = c->l_block[rp] + (uint64_t)cls * c->cap_block;
+                const uint32_t* cnt = c->d_cnt + C_BLOCK + 2 * cls + rp;
                 {
                     Launch L(c, K_BLOCK, 0);
-                    hipLaunchKernelGGL(k_block, dim3(nblk), dim3(kWG), 0, st, c->l_block[rp], blk_done,
-                                       c->d_cnt + C_BLOCK + rp, B, nsink, depth);
+                    if (cls == 0)
+                        hipLaunchKernelGGL(k_block<256>, dim3(nblk[cls]), dim3(kWG), 0, st, list, blk_done[cls], cnt,
+                                           B, nsink, depth);
+                    else if (cls == 1)
+                        hipLaunchKernelGGL(k_block<512>, dim3(nblk[cls]), dim3(kWG), 0, st, list, blk_done[cls], cnt,
+                                           B, nsink, depth);
+                    else
+                        hipLaunchKernelGGL(k_block<kBlkT>, dim3(nblk[cls]), dim3(kWG), 0, st, list, blk_done[cls],
+                                           cnt, B, nsink, depth);
                 }
-                sum_sizes(c, c->l_block[rp], blk_done, c->d_cnt + C_BLOCK + rp, blk_done + nblk, S_BLOCK);
-                blk_done += nblk;
+                sum_sizes(c, list, blk_done[cls], cnt, blk_done[cls] + nblk[cls], S_BLOCK);
+                blk_done[cls] += nblk[cls];
             }
             HIPCHK(hipGetLastError());
             if ((rc = readback(c, nx_large, nx_ci))) return rc;
             cur_ci = nx_ci;
         }
         // leaves of this round
-        const uint32_t nt = c->h_cnt[C_TINY + rp], nb = c->h_cnt[C_BLOCK + rp], nr = c->h_cnt[C_RUN + rp];
+        const uint32_t nt = c->h_cnt[C_TINY + rp], nr = c->h_cnt[C_RUN + rp];
+        uint32_t nb = 0;
+        for (int cls = 0; cls < kBlkClasses; ++cls) nb += c->h_cnt[C_BLOCK + 2 * cls + rp];
         c->st_leaves += nt + nb + nr;
         if (nt) {
             {
@@ -1204,13 +1241,23 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
         }
         if (nt + nb + nr && lcp) {
             Launch L(c, K_LEAFBOUND, (uint64_t)(nt + nb + nr) * 36);
-            hipLaunchKernelGGL(k_leafbound, dim3(div_up((uint64_t)nt + nb + nr, kWG)), dim3(kWG), 0, st,
-                               c->l_tiny[rp], c->l_block[rp], c->l_run[rp], nt, nb, nr, B, depth);
+            LeafLists LL;
+            uint32_t end = 0;
+            LL.list[0] = c->l_tiny[rp];
+            LL.end[0] = end += nt;
+            for (int cls = 0; cls < kBlkClasses; ++cls) {
+                LL.list[1 + cls] = c->l_block[rp] + (uint64_t)cls * c->cap_block;
+                LL.end[1 + cls] = end += c->h_cnt[C_BLOCK + 2 * cls + rp];
+            }
+            LL.list[kBlkClasses + 1] = c->l_run[rp];
+            LL.end[kBlkClasses + 1] = end += nr;
+            hipLaunchKernelGGL(k_leafbound, dim3(div_up((uint64_t)end, kWG)), dim3(kWG), 0, st, LL, B, depth);
         }
         HIPCHK(hipGetLastError());
         if ((rc = readback(c, c->l_round[np], C_LROUND + np))) return rc;
-        if (c->h_cnt[C_LROUND + np] + c->h_cnt[C_TINY + np] + c->h_cnt[C_BLOCK + np] + c->h_cnt[C_RUN + np] == 0)
-            break;
+        uint64_t pending = (uint64_t)c->h_cnt[C_LROUND + np] + c->h_cnt[C_TINY + np] + c->h_cnt[C_RUN + np];
+        for (int cls = 0; cls < kBlkClasses; ++cls) pending += c->h_cnt[C_BLOCK + 2 * cls + np];
+        if (pending == 0) break;
         if (depth > alen + 8) return set_err(PS5_ERR_INTERNAL, "depth exceeds the arena: no progress");
     }
     if (c->profile) {
