@@ -82,6 +82,12 @@ def bench_batch(ps5, args):
     d_lcp = ps5.DeviceBuffer(n * 4)
     d_arena.upload(arena)
     d_offs.upload(offs)
+    tuning = ctx.get_tuning()
+    if args.autotune:
+        t0 = time.time()
+        tuning, tms = ctx.autotune_batch_device(d_arena, int(arena.size), d_offs, sf, d_perm, d_lcp)
+        print(f"[bench] autotune: leaf_target/max_treebits/max_oversample = {tuning}, {tms:.3f} ms "
+              f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         ctx.sort_batch_device(d_arena, int(arena.size), d_offs, sf, d_perm, d_lcp)
     lib.ps5_device_synchronize()
@@ -142,6 +148,8 @@ def bench_batch(ps5, args):
            "plain_sorts_per_s": round(args.sets / plain_s, 2), "plain_ms": round(plain_s * 1e3, 3),
            "speedup": round(plain_s / batched_s, 2), "batched_chars_gbs": round(arena.size / batched_s / 1e9, 2),
            "identical_to_plain": same, "oracle_checked_sets": (args.sets + 15) // 16, "oracle_failures": bad,
+           "tuning": {"leaf_target": tuning[0], "max_treebits": tuning[1], "max_oversample": tuning[2],
+                      "autotuned": bool(args.autotune)},
            "config": {"workload": f"{args.sets} sets, strings log-uniform [{args.count_lo}, {args.count_hi}], "
                                   "length U[4,48], alphabet ACGT/digits/bytes 33..126, seed 0x5EED0004"}}
     print(json.dumps(out), flush=True)
@@ -173,8 +181,12 @@ def bench_serve(ps5, args):
 
     for a, o in reqs[:3]:
         plain_sort(a, o)
+    alone, alone_ms = [], []
     t = time.perf_counter()
-    alone = [plain_sort(a, o) for a, o in reqs]
+    for a, o in reqs:
+        t1 = time.perf_counter()
+        alone.append(plain_sort(a, o))
+        alone_ms.append((time.perf_counter() - t1) * 1e3)
     plain_busy = time.perf_counter() - t
     rate = args.load * args.requests / plain_busy           # requests per second offered
     gaps = rng.exponential(1.0 / rate, args.requests)
@@ -211,6 +223,9 @@ def bench_serve(ps5, args):
     same = all(np.array_equal(p1, p2) and np.array_equal(l1[1:], l2[1:])
                for (p1, l1), (p2, l2, _) in zip(alone, res))
     served_lat = [ms for _, _, ms in res]
+    big = sorted(range(args.requests), key=lambda i: -int(reqs[i][0].size))[:5]
+    largest = [{"bytes": int(reqs[i][0].size), "alone_ms": round(alone_ms[i], 3), "served_ms": round(served_lat[i], 3),
+                "fifo_ms": round(fifo_lat[i], 3)} for i in big]
     out = {"metric": "served vs plain p50/p99 latency (ms) and throughput", "requests": args.requests,
            "offered_rate_per_s": round(rate, 2), "offered_load": args.load, "bytes": total,
            "served": {"p50_ms": round(float(np.percentile(served_lat, 50)), 3),
@@ -220,6 +235,7 @@ def bench_serve(ps5, args):
            "plain_fifo": {"p50_ms": round(float(np.percentile(fifo_lat, 50)), 3),
                           "p99_ms": round(float(np.percentile(fifo_lat, 99)), 3),
                           "throughput_gbs": round(total / fifo_wall / 1e9, 3)},
+           "largest_requests": largest,
            "identical_to_alone": bool(same),
            "config": {"workload": f"{args.requests} requests, size log-uniform [{args.req_lo}, {args.req_hi}] bytes, "
                                   "one of 1000 prefixes (Zipf 1.1) + 10..60 random chars, Poisson arrivals at "
@@ -308,6 +324,8 @@ def main():
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--max-batch", type=int, default=16 << 20, help="strings per served batch (buffers pre-allocated)")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=None,
+                    help="tune the sort's shape on the workload before timing (default: on in batch mode)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -325,6 +343,8 @@ def main():
 
     if args.rocprof_summary:
         return rocprof_summary(args.rocprof_summary)
+    if args.autotune is None:
+        args.autotune = args.mode == "batch"
     ps5 = load_ps5()
     if args.mode == "batch":
         return bench_batch(ps5, args)
@@ -358,6 +378,12 @@ def main():
         dst.free()
         print(f"[bench] calibration copy: k_copy16 moved {cb} bytes read + {cb} bytes written per launch",
               file=sys.stderr, flush=True)
+    tuning = ctx.get_tuning()
+    if args.autotune:
+        t0 = time.time()
+        tuning, tms = ctx.autotune_device(d_arena, alen, d_offs, n, d_perm, d_lcp)
+        print(f"[bench] autotune: leaf_target/max_treebits/max_oversample = {tuning}, {tms:.3f} ms "
+              f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         ctx.sort_device(d_arena, alen, d_offs, n, d_perm, d_lcp)
     step_ms = []
@@ -425,7 +451,9 @@ def main():
         "dtype": "u8",
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.workload], "strings": n, "arena_bytes": chars,
-                   "output": "permutation + LCP array", "device_resident": True},
+                   "output": "permutation + LCP array", "device_resident": True,
+                   "tuning": {"leaf_target": tuning[0], "max_treebits": tuning[1], "max_oversample": tuning[2],
+                              "autotuned": bool(args.autotune)}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(dom, args.workload),

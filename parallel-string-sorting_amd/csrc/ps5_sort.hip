@@ -56,6 +56,21 @@ __global__ void k_setup(uint32_t* cnt, Item* list, int c_index, uint32_t n)
     }
 }
 
+// Counters and the first kPeek items of a work list in one buffer, so that a
+// single copy brings both to the host: out[0, C_NCNT) = counters, then
+// min(cnt[ci], kPeek) items from byte kStageHead on.
+constexpr uint32_t kPeek = 2048;
+constexpr uint32_t kStageHead = 64;
+static_assert(C_NCNT * 4 <= kStageHead, "counters must fit the stage head");
+__global__ __launch_bounds__(kWG) void k_stage(const uint32_t* cnt, const Item* list, int ci, unsigned char* out)
+{
+    if (threadIdx.x < C_NCNT) reinterpret_cast<uint32_t*>(out)[threadIdx.x] = cnt[threadIdx.x];
+    if (!list) return;
+    const uint32_t nl = min(cnt[ci], kPeek);
+    Item* dst = reinterpret_cast<Item*>(out + kStageHead);
+    for (uint32_t i = threadIdx.x; i < nl; i += kWG) dst[i] = list[i];
+}
+
 // zero the counters whose bit is set in `mask`
 __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 {
@@ -782,15 +797,17 @@ struct ps5_ctx {
     uint32_t cap_large = 0, cap_block = 0, cap_tiny = 0, cap_run = 0;
     uint32_t* d_cnt = nullptr;
     uint32_t* h_cnt = nullptr;       // pinned
+    unsigned char* d_stage = nullptr;    // counters + list head for one readback copy
+    unsigned char* h_stage = nullptr;    // pinned
     uint32_t* h_upcnt = nullptr;     // pinned: counters uploaded with batch roots
     Item* h_roots = nullptr;         // pinned: root items of a batch
     uint64_t roots_cap = 0;
     Item* h_large = nullptr;         // pinned copy of a large list
     // multi-workgroup step of a level: descriptors and tile table
+    // plan of a level: its nlarge descriptors followed by its tile table, in
+    // one buffer so that one copy uploads both
     LargeDesc* d_desc = nullptr;
-    TileRef* d_tiles = nullptr;
     LargeDesc* h_desc = nullptr;     // pinned
-    TileRef* h_tiles = nullptr;      // pinned
     uint64_t tiles_cap = 0;
     key64* trees = nullptr;          // 2^kMaxTreebits splitter slots per large segment
     uint64_t trees_cap = 0;
@@ -803,7 +820,7 @@ struct ps5_ctx {
     unsigned long long* d_sums = nullptr;   // profiling: elements per kernel class
     unsigned long long* h_sums = nullptr;
     // tuning
-    uint32_t leaf_target = 256;      // strings per block the multi-workgroup levels aim for
+    uint32_t leaf_target = 512;      // strings per block the multi-workgroup levels aim for
     uint32_t max_treebits = kMaxTreebits;   // splitter-tree depth limit of one level
     uint32_t max_oversample = 32;    // sample keys per splitter (limited by kMaxSamples)
     uint64_t seed = 0x5EED5EEDull;
@@ -828,13 +845,12 @@ void free_buffers(ps5_ctx* c)
         c->key[i] = nullptr; c->idx[i] = nullptr; c->l_round[i] = nullptr; c->l_lvl[i] = nullptr;
         c->l_tiny[i] = nullptr; c->l_block[i] = nullptr; c->l_run[i] = nullptr;
     }
-    f(c->bkey); f(c->bktrank); f(c->hist); f(c->part); f(c->trees); f(c->d_desc); f(c->d_tiles);
+    f(c->bkey); f(c->bktrank); f(c->hist); f(c->part); f(c->trees); f(c->d_desc);
     c->bkey = nullptr; c->bktrank = nullptr; c->hist = nullptr; c->part = nullptr; c->trees = nullptr;
-    c->d_desc = nullptr; c->d_tiles = nullptr;
+    c->d_desc = nullptr;
     if (c->h_large) (void)hipHostFree(c->h_large);
     if (c->h_desc) (void)hipHostFree(c->h_desc);
-    if (c->h_tiles) (void)hipHostFree(c->h_tiles);
-    c->h_large = nullptr; c->h_desc = nullptr; c->h_tiles = nullptr;
+    c->h_large = nullptr; c->h_desc = nullptr;
     c->cap = 0;
     c->hist_cap = 0;
     c->part_cap = 0;
@@ -898,11 +914,11 @@ int ensure_capacity(ps5_ctx* c, uint64_t n)
     // every large segment has at most size/kTile + 1 tiles; the splitter
     // trees and histograms of a level grow on demand (grow_level)
     c->tiles_cap = cap / kTile + c->cap_large + 1;
-    HIPCHK(hipMalloc(&c->d_desc, (uint64_t)c->cap_large * sizeof(LargeDesc)));
-    HIPCHK(hipMalloc(&c->d_tiles, c->tiles_cap * sizeof(TileRef)));
+    const uint64_t plan_bytes = (uint64_t)c->cap_large * sizeof(LargeDesc) + c->tiles_cap * sizeof(TileRef);
+    static_assert(sizeof(LargeDesc) % alignof(TileRef) == 0, "tile table must stay aligned after the descriptors");
+    HIPCHK(hipMalloc(&c->d_desc, plan_bytes));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_large), (uint64_t)c->cap_large * sizeof(Item)));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_desc), (uint64_t)c->cap_large * sizeof(LargeDesc)));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_tiles), c->tiles_cap * sizeof(TileRef)));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_desc), plan_bytes));
     c->cap = cap;
     return PS5_OK;
 }
@@ -973,17 +989,24 @@ Sink make_sink(ps5_ctx* c, Item* large, int c_large, int rp)
 // copy the counters (and the large list `list` with counter `ci`) to the host
 int readback(ps5_ctx* c, const Item* list, int ci)
 {
-    HIPCHK(hipMemcpyAsync(c->h_cnt, c->d_cnt, C_NCNT * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    // one staged copy holds the counters and the head of the list
+    hipLaunchKernelGGL(k_stage, dim3(1), dim3(kWG), 0, c->stream, c->d_cnt, list, ci, c->d_stage);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_stage, c->d_stage, kStageHead + (list ? kPeek * sizeof(Item) : 0),
+                          hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->st_syncs++;
+    memcpy(c->h_cnt, c->h_stage, C_NCNT * sizeof(uint32_t));
     if (c->h_cnt[C_ERR] & ERR_CAP) return set_err(PS5_ERR_CAPACITY, "work list capacity exceeded");
     if (c->h_cnt[C_ERR] & ERR_LCP) return set_err(PS5_ERR_INTERNAL, "equal keys across a leaf boundary");
     if (c->h_cnt[C_ERR] & ERR_BOUNDS) return set_err(PS5_ERR_INTERNAL, "device bounds check failed");
     if (list) {
         const uint32_t nl = c->h_cnt[ci];
         if (nl > c->cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
-        if (nl) {
-            HIPCHK(hipMemcpyAsync(c->h_large, list, nl * sizeof(Item), hipMemcpyDeviceToHost, c->stream));
+        memcpy(c->h_large, c->h_stage + kStageHead, std::min(nl, kPeek) * sizeof(Item));
+        if (nl > kPeek) {
+            HIPCHK(hipMemcpyAsync(c->h_large + kPeek, list + kPeek, (nl - kPeek) * sizeof(Item),
+                                  hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
             c->st_syncs++;
         }
@@ -1015,7 +1038,10 @@ int large_treebits(uint32_t m, uint32_t target, int maxbits)
 int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uint64_t seed)
 {
     hipStream_t st = c->stream;
-    uint64_t ntiles = 0, hl = 0, elems = 0;
+    uint64_t ntiles = 0, hl = 0, elems = 0, elems_ident = 0;
+    if (nlarge > c->cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
+    TileRef* h_tiles = reinterpret_cast<TileRef*>(c->h_desc + nlarge);
+    const TileRef* d_tiles = reinterpret_cast<const TileRef*>(c->d_desc + nlarge);
     for (uint32_t i = 0; i < nlarge; ++i) {
         const Item seg = c->h_large[i];
         const uint32_t T = div_up(seg.size, kTile);
@@ -1027,15 +1053,16 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         const uint32_t nspl = (1u << D.k) - 1;
         D.oversample = std::max<uint32_t>(2, std::min<uint32_t>(c->max_oversample, (kMaxSamples / nspl) & ~1u));
         if (ntiles + T > c->tiles_cap) return set_err(PS5_ERR_CAPACITY, "tile table capacity");
-        for (uint32_t t = 0; t < T; ++t) c->h_tiles[ntiles + t] = TileRef{ i, t };
+        for (uint32_t t = 0; t < T; ++t) h_tiles[ntiles + t] = TileRef{ i, t };
         ntiles += T;
         hl += (uint64_t)((2u << D.k) - 1) * T * kWaves;
         elems += seg.size;
+        if (seg.flags & F_IDENT) elems_ident += seg.size;
         if (hl > 0xFFFFFFFFull) return set_err(PS5_ERR_CAPACITY, "histogram capacity");
     }
     if (int rc = grow_level(c, nlarge, hl)) return rc;
-    HIPCHK(hipMemcpyAsync(c->d_desc, c->h_desc, nlarge * sizeof(LargeDesc), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(c->d_tiles, c->h_tiles, ntiles * sizeof(TileRef), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->d_desc, c->h_desc, nlarge * sizeof(LargeDesc) + ntiles * sizeof(TileRef),
+                          hipMemcpyHostToDevice, st));
     const uint32_t hl32 = (uint32_t)hl;
     {
         Launch L(c, K_TREE, (uint64_t)nlarge * kMaxSamples * 8);
@@ -1043,7 +1070,7 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     }
     {
         Launch L(c, K_COUNT, elems * 12 + hl * 4);
-        hipLaunchKernelGGL(k_count, dim3((uint32_t)ntiles), dim3(kWG), 0, st, c->d_desc, c->d_tiles, B, c->trees,
+        hipLaunchKernelGGL(k_count, dim3((uint32_t)ntiles), dim3(kWG), 0, st, c->d_desc, d_tiles, B, c->trees,
                            c->hist);
     }
     {
@@ -1054,9 +1081,10 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, c->hist, hl32, c->part);
     }
     {
-        Launch L(c, K_SCATTER, elems * 28 + hl * 8);
+        // bktrank 4 + key 8 + idx 4 R (no idx at a root), key 8 + idx 4 W
+        Launch L(c, K_SCATTER, elems * 28 - elems_ident * 4 + hl * 8);
         hipLaunchKernelGGL(k_scatter, dim3((uint32_t)ntiles * kWaves), dim3(kBigWG), kScatLds, st, c->d_desc,
-                           c->d_tiles, B, c->hist);
+                           d_tiles, B, c->hist);
     }
     {
         Launch L(c, K_CHILDREN, hl * 4 / kWaves);
@@ -1294,6 +1322,8 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     if (e == hipSuccess) e = hipMalloc(&c->d_cnt, C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_cnt), C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_upcnt), C_NCNT * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&c->d_stage, kStageHead + kPeek * sizeof(Item));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_stage), kStageHead + kPeek * sizeof(Item));
     if (e == hipSuccess) e = hipMalloc(&c->d_sums, S_NUM * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_sums), S_NUM * sizeof(unsigned long long));
     if (e != hipSuccess) {
@@ -1315,6 +1345,8 @@ void ps5_ctx_destroy(ps5_ctx* c)
     if (c->d_sets) (void)hipFree(c->d_sets);
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
     if (c->h_upcnt) (void)hipHostFree(c->h_upcnt);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->d_stage) (void)hipFree(c->d_stage);
     if (c->h_roots) (void)hipHostFree(c->h_roots);
     if (c->h_sums) (void)hipHostFree(c->h_sums);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
