@@ -267,21 +267,16 @@ __device__ __forceinline__ int inorder_of(int t, int k)
 
 // Bucket of a key: b = 2 * lower_bound + [splitter == key]
 // (ClassifyTreeSimple::find_bkt, bingmann-sample_sortBTC.hpp:45-72).  The
-// smallest splitter >= key is the node of the last left turn of the descent,
-// so the equality test needs no separate in-order splitter array.
-__device__ __forceinline__ uint32_t classify_key(key64 key, const key64* s_tree, int k)
+// descent i = 2i + (key > tree[i]) over the level-order tree ends at the
+// number of splitters below the key; the equality test reads that splitter
+// from the sorted array (one load instead of tracking a candidate per level).
+__device__ __forceinline__ uint32_t classify_key(key64 key, const key64* s_tree, const key64* s_sorted, int k)
 {
     uint32_t i = 1;
-    key64 cand = 0;
-    bool left = false;
-    for (int l = 0; l < k; ++l) {
-        const key64 t = s_tree[i];
-        const bool gt = key > t;
-        if (!gt) { cand = t; left = true; }
-        i = 2 * i + (gt ? 1u : 0u);
-    }
-    i -= (1u << k);
-    return 2 * i + ((left && cand == key) ? 1u : 0u);
+    for (int l = 0; l < k; ++l) i = 2 * i + (key > s_tree[i] ? 1u : 0u);
+    i -= 1u << k;
+    const uint32_t nspl = (1u << k) - 1;
+    return 2 * i + ((i < nspl && s_sorted[i] == key) ? 1u : 0u);
 }
 
 }  // namespace ps5

@@ -34,6 +34,7 @@ constexpr int kMaxTreebits = 10;             // reference DefaultTreebits
 constexpr int kMaxSpl = (1 << kMaxTreebits) - 1;    // 1023 splitters
 constexpr int kMaxBkt = 2 * kMaxSpl + 1;            // 2047 buckets
 constexpr int kMaxSamples = 4096;            // sample keys of one splitter tree
+constexpr uint32_t kTreeSlots = 2048;        // per segment: level-order tree [0,1024), sorted splitters [1024,2048)
 constexpr int kWaveChunk = 8192;             // keys ranked by one wave (multi-workgroup step)
 constexpr int kTile = kWaveChunk * kWaves;   // keys per count workgroup
 constexpr uint32_t kTinyT = 64;              // leaves sorted by one wave

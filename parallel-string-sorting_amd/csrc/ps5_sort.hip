@@ -84,8 +84,12 @@ __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 // D.oversample * nspl random keys are sorted; splitter i is sample
 // oversample/2 + oversample*i (the reference's choice for oversample factor 2,
 // bingmann-sample_sort_tree_builder.hpp:44-58) and the tree is stored in
-// level order: trees[seg*1024 + t], t = 1..nspl.
-__global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, uint64_t seed, key64* trees)
+// level order: trees[seg*2048 + t], t = 1..nspl, followed by the splitters in
+// sorted order at trees[seg*2048 + 1024 + i], i < nspl.  With from_arena (the root
+// of a sort whose keys are not extracted yet) the samples are read from the
+// arena directly.
+__global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, uint64_t seed, key64* trees,
+                                                  int from_arena)
 {
     __shared__ key64 s_samp[kMaxSamples];
     const LargeDesc D = desc[blockIdx.x];
@@ -103,14 +107,16 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
         key64 v = ~0ull;
         if (j < ss) {
             const uint64_t h = mix64(seed ^ ((uint64_t)seg.begin << 21) ^ (uint64_t)j);
-            v = K[seg.begin + (uint32_t)(((h >> 32) * (uint64_t)seg.size) >> 32)];
+            const uint32_t pos = seg.begin + (uint32_t)(((h >> 32) * (uint64_t)seg.size) >> 32);
+            v = from_arena ? load_key(B.arena, B.alen, str_off(B, pos)) : K[pos];
         }
         s_samp[j] = v;
     }
     __syncthreads();
     bitonic_sort_u64<kBigWG>(s_samp, P);
-    key64* tree = trees + (uint64_t)blockIdx.x * 1024;
+    key64* tree = trees + (uint64_t)blockIdx.x * kTreeSlots;
     for (int t = threadIdx.x + 1; t <= nspl; t += kBigWG) tree[t] = s_samp[a / 2 + a * inorder_of(t, k)];
+    for (int i = threadIdx.x; i < nspl; i += kBigWG) tree[1024 + i] = s_samp[a / 2 + a * i];
 }
 
 // Classify one tile of kTile keys.  Each wave ranks its own chunk of
@@ -119,59 +125,85 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
 // then tile, then wave -- the order of the reference's prefix sum
 // (.hpp:1437-1460) with wave chunks as the parts -- so an exclusive scan of
 // hist gives every chunk's base offset per bucket.  bktrank gets bucket and
-// rank inside the chunk.
-__global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const TileRef* tiles, Bufs B,
-                                                const key64* trees, uint32_t* hist)
+// rank inside the chunk.  With from_arena (the root of a sort, positions =
+// input indices) the keys are extracted from the arena here and stored: this
+// replaces the separate extraction pass and saves reading the keys back.
+// Dynamic LDS, sized by the largest tree of the level (kmax treebits):
+// tree[2^kmax], sorted splitters[2^kmax], per-wave 16-bit counts[kWaves][2^(kmax+1)].
+__host__ __device__ constexpr uint32_t count_lds_bytes(int kmax)
 {
-    __shared__ key64 s_tree[1024];
-    __shared__ uint16_t s_hist[kWaves][2048];   // a wave chunk counts at most kWaveChunk keys
+    return (2u << kmax) * 8u + kWaves * (2u << kmax) * 2u;
+}
+
+__global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const TileRef* tiles, Bufs B,
+                                                const key64* trees, uint32_t* hist, int from_arena, int kmax)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t nb2 = 2u << kmax;               // bucket slots per wave
+    key64* s_tree = reinterpret_cast<key64*>(smem);
+    key64* s_sorted = s_tree + (nb2 >> 1);
+    uint16_t* s_hist_base = reinterpret_cast<uint16_t*>(s_sorted + (nb2 >> 1));
     static_assert(kWaveChunk <= 65535, "per-wave bucket counts must fit 16 bits");
     const TileRef tr = tiles[xcd_contiguous(blockIdx.x, gridDim.x)];
     const LargeDesc D = desc[tr.seg];
     const Item seg = D.seg;
     const int k = (int)D.k, nbkt = (2 << k) - 1;
-    const key64* tree = trees + (uint64_t)tr.seg * 1024;
-    for (int i = threadIdx.x; i < 1024; i += kWG) s_tree[i] = tree[i];
-    for (int i = threadIdx.x; i < kWaves * 2048; i += kWG) (&s_hist[0][0])[i] = 0;
+    if (k > kmax) {   // the launch must size LDS for every tree of the level
+        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
+        return;
+    }
+    const key64* tree = trees + (uint64_t)tr.seg * kTreeSlots;
+    const int nspl = (1 << k) - 1;
+    for (int i = threadIdx.x; i <= nspl; i += kWG) s_tree[i] = tree[i];
+    for (int i = threadIdx.x; i < nspl; i += kWG) s_sorted[i] = tree[1024 + i];
+    for (uint32_t i = threadIdx.x; i < kWaves * nb2; i += kWG) s_hist_base[i] = 0;
     __syncthreads();
     const key64* K = (seg.flags & F_BUFB) ? B.key1 : B.key0;
     const uint32_t t = tr.t;
     const uint32_t w = wave_id(), lane = lane_id();
+    uint16_t* s_hist = s_hist_base + w * nb2;      // this wave's counts
     const uint32_t cb = t * kTile + w * kWaveChunk;
     const uint32_t ce = (item_ok(seg, B) && t < D.T) ? min(seg.size, cb + kWaveChunk) : 0u;
     // two 64-key groups per iteration: both loads are in flight before the
     // tree descents, then the groups are ranked in order (stable)
-    for (uint32_t g = cb; g < ce; g += 128) {
+    auto load2 = [&](uint32_t g, key64& k0, key64& k1) {
         const uint32_t p0 = g + lane, p1 = g + 64 + lane;
         const bool a0 = p0 < ce, a1 = p1 < ce;
-        const key64 k0 = a0 ? K[seg.begin + p0] : 0ull;
-        const key64 k1 = a1 ? K[seg.begin + p1] : 0ull;
-        const uint32_t b0 = a0 ? classify_key(k0, s_tree, k) : kNoBucket;
-        const uint32_t b1 = a1 ? classify_key(k1, s_tree, k) : kNoBucket;
-        {
-            const uint64_t peers = match_bucket(b0);
-            const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
-            if (a0) {
-                const uint32_t old = s_hist[w][b0];
-                if (r == 0) s_hist[w][b0] = (uint16_t)(old + (uint32_t)__popcll(peers));
-                B.bktrank[seg.begin + p0] = (b0 << kRankBits) | (old + r);
-            }
+        k0 = 0;
+        k1 = 0;
+        if (from_arena) {
+            if (a0) k0 = load_key(B.arena, B.alen, str_off(B, seg.begin + p0));
+            if (a1) k1 = load_key(B.arena, B.alen, str_off(B, seg.begin + p1));
+            if (a0) B.key0[seg.begin + p0] = k0;
+            if (a1) B.key0[seg.begin + p1] = k1;
+        } else {
+            if (a0) k0 = K[seg.begin + p0];
+            if (a1) k1 = K[seg.begin + p1];
         }
-        {
-            const uint64_t peers = match_bucket(b1);
-            const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
-            if (a1) {
-                const uint32_t old = s_hist[w][b1];
-                if (r == 0) s_hist[w][b1] = (uint16_t)(old + (uint32_t)__popcll(peers));
-                B.bktrank[seg.begin + p1] = (b1 << kRankBits) | (old + r);
-            }
+    };
+    auto rank1 = [&](uint32_t p, uint32_t b) {
+        const uint64_t peers = match_bucket(b);
+        const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
+        if (p < ce) {
+            const uint32_t old = s_hist[b];
+            if (r == 0) s_hist[b] = (uint16_t)(old + (uint32_t)__popcll(peers));
+            B.bktrank[seg.begin + p] = (b << kRankBits) | (old + r);
         }
+    };
+    for (uint32_t g = cb; g < ce; g += 128) {
+        key64 k0, k1;
+        load2(g, k0, k1);
+        const uint32_t p0 = g + lane, p1 = g + 64 + lane;
+        const uint32_t b0 = p0 < ce ? classify_key(k0, s_tree, s_sorted, k) : kNoBucket;
+        const uint32_t b1 = p1 < ce ? classify_key(k1, s_tree, s_sorted, k) : kNoBucket;
+        rank1(p0, b0);
+        rank1(p1, b1);
     }
     __syncthreads();
     uint32_t* h = hist + D.hist_base;
     for (int i = threadIdx.x; i < nbkt * kWaves; i += kWG) {
         const uint32_t b = (uint32_t)i / kWaves, ww = (uint32_t)i % kWaves;
-        h[((uint32_t)b * D.T + t) * kWaves + ww] = s_hist[ww][b];
+        h[((uint32_t)b * D.T + t) * kWaves + ww] = s_hist_base[ww * nb2 + b];
     }
 }
 
@@ -809,7 +841,7 @@ struct ps5_ctx {
     LargeDesc* d_desc = nullptr;
     LargeDesc* h_desc = nullptr;     // pinned
     uint64_t tiles_cap = 0;
-    key64* trees = nullptr;          // 2^kMaxTreebits splitter slots per large segment
+    key64* trees = nullptr;          // kTreeSlots splitter slots per large segment
     uint64_t trees_cap = 0;
     uint32_t* hist = nullptr;
     uint64_t hist_cap = 0;
@@ -868,7 +900,7 @@ int grow_level(ps5_ctx* c, uint32_t nlarge, uint64_t hl)
         c->trees = nullptr;
         c->trees_cap = 0;
         const uint64_t want = std::max<uint64_t>(nlarge + nlarge / 2, 64);
-        HIPCHK(hipMalloc(&c->trees, want * (1u << kMaxTreebits) * sizeof(key64)));
+        HIPCHK(hipMalloc(&c->trees, want * kTreeSlots * sizeof(key64)));
         c->trees_cap = want;
     }
     if (hl > c->hist_cap) {
@@ -1035,10 +1067,11 @@ int large_treebits(uint32_t m, uint32_t target, int maxbits)
 }
 
 // One multi-workgroup sample-sort step over all large segments of a level.
-int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uint64_t seed)
+int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uint64_t seed, bool from_arena)
 {
     hipStream_t st = c->stream;
     uint64_t ntiles = 0, hl = 0, elems = 0, elems_ident = 0;
+    int kmax = 1;
     if (nlarge > c->cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
     TileRef* h_tiles = reinterpret_cast<TileRef*>(c->h_desc + nlarge);
     const TileRef* d_tiles = reinterpret_cast<const TileRef*>(c->d_desc + nlarge);
@@ -1051,6 +1084,7 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         D.hist_base = (uint32_t)hl;
         D.k = (uint32_t)large_treebits(seg.size, c->leaf_target, (int)c->max_treebits);
         const uint32_t nspl = (1u << D.k) - 1;
+        kmax = std::max(kmax, (int)D.k);
         D.oversample = std::max<uint32_t>(2, std::min<uint32_t>(c->max_oversample, (kMaxSamples / nspl) & ~1u));
         if (ntiles + T > c->tiles_cap) return set_err(PS5_ERR_CAPACITY, "tile table capacity");
         for (uint32_t t = 0; t < T; ++t) h_tiles[ntiles + t] = TileRef{ i, t };
@@ -1066,12 +1100,13 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     const uint32_t hl32 = (uint32_t)hl;
     {
         Launch L(c, K_TREE, (uint64_t)nlarge * kMaxSamples * 8);
-        hipLaunchKernelGGL(k_tree, dim3(nlarge), dim3(kBigWG), 0, st, c->d_desc, B, seed, c->trees);
+        hipLaunchKernelGGL(k_tree, dim3(nlarge), dim3(kBigWG), 0, st, c->d_desc, B, seed, c->trees, (int)from_arena);
     }
     {
-        Launch L(c, K_COUNT, elems * 12 + hl * 4);
-        hipLaunchKernelGGL(k_count, dim3((uint32_t)ntiles), dim3(kWG), 0, st, c->d_desc, d_tiles, B, c->trees,
-                           c->hist);
+        // key 8 R + bktrank 4 W; extracting at the root: offs 4 R + 8 characters R + key 8 W + bktrank 4 W
+        Launch L(c, K_COUNT, (from_arena ? elems * 24 : elems * 12) + hl * 4);
+        hipLaunchKernelGGL(k_count, dim3((uint32_t)ntiles), dim3(kWG), count_lds_bytes(kmax), st, c->d_desc,
+                           d_tiles, B, c->trees, c->hist, (int)from_arena, kmax);
     }
     {
         const uint32_t nb = div_up(hl32, kScanBlock);
@@ -1183,13 +1218,22 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     B.bkey = c->bkey;
     B.bktrank = c->bktrank;
 
-    {
+    // One root larger than a block: its keys are extracted by the first
+    // classification pass itself, and the host knows the root without asking.
+    const bool fused_root = !set_first && n > kBlkT;
+    if (!fused_root) {
         Launch L(c, K_INIT, (uint64_t)n * 20);   // offs 4 R + 8 characters R + key 8 W
         hipLaunchKernelGGL(k_init, dim3(div_up(n, kWG)), dim3(kWG), 0, st, B, n);
     }
     if ((rc = setup_roots(c, n, set_first, nsets))) return rc;
     HIPCHK(hipGetLastError());
-    if ((rc = readback(c, c->l_round[0], C_LROUND))) return rc;
+    if (fused_root) {
+        for (int i = 0; i < C_NCNT; ++i) c->h_cnt[i] = 0;
+        c->h_cnt[C_LROUND] = 1;
+        c->h_large[0] = Item{ 0u, n, 0u, F_IDENT };
+    } else if ((rc = readback(c, c->l_round[0], C_LROUND))) {
+        return rc;
+    }
 
     for (uint32_t round = 0;; ++round) {
         const uint32_t depth = round * 8;
@@ -1222,7 +1266,8 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
             }
             const Sink sink = make_sink(c, nx_large, nx_ci, rp);
             const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
-            if (nlarge && (rc = large_step(c, B, nlarge, sink, seed))) return rc;
+            if (nlarge && (rc = large_step(c, B, nlarge, sink, seed, fused_root && round == 0 && level == 0)))
+                return rc;
             for (int cls = 0; cls < kBlkClasses; ++cls) {
                 if (!nblk[cls]) continue;
                 const Item* list = c->l_block[rp] + (uint64_t)cls * c->cap_block;
@@ -1316,6 +1361,8 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     *out = nullptr;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipFuncSetAttribute((const void*)k_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, kScatLds));
+    HIPCHK(hipFuncSetAttribute((const void*)k_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               count_lds_bytes(kMaxTreebits)));
     ps5_ctx* c = new ps5_ctx;
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
