@@ -238,23 +238,19 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total,
     return wpre + x - v;
 }
 
-// Bitonic sort of P (power of two) uint64 in LDS by a workgroup of NT threads.
-template <int NT>
-__device__ void bitonic_sort_u64(key64* s, int P)
+// Number of elements of a sorted run a[0, len) that precede `key` of a run
+// further right (before_eq: elements equal to key precede it, i.e. the run is
+// to the left of the key's run).  Binary lifting over a power-of-two L >= len.
+__device__ __forceinline__ uint32_t run_rank(const key64* a, uint32_t len, uint32_t L, key64 key, bool before_eq)
 {
-    for (int k = 2; k <= P; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < (P >> 1); i += NT) {
-                // i-th compare-exchange pair: low index with bit j cleared
-                const int lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-                const int hi = lo | j;
-                const key64 a = s[lo], b = s[hi];
-                const bool up = (lo & k) == 0;
-                if ((a > b) == up) { s[lo] = b; s[hi] = a; }
-            }
-            __syncthreads();
+    uint32_t c = 0;
+    for (uint32_t s = L; s > 0; s >>= 1) {
+        if (c + s <= len) {
+            const key64 x = a[c + s - 1];
+            if (x < key || (before_eq && x == key)) c += s;
         }
     }
+    return c;
 }
 
 // In-order index of level-order node t of a complete tree with k levels.

@@ -81,7 +81,8 @@ __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 // ---- multi-workgroup sample-sort step (segments > kBlkT) -------------------
 
 // Splitter trees of the large segments of a level, one workgroup each.
-// D.oversample * nspl random keys are sorted; splitter i is sample
+// D.oversample * nspl random keys are sorted (64-key runs in registers, then
+// merged in LDS); splitter i is sample
 // oversample/2 + oversample*i (the reference's choice for oversample factor 2,
 // bingmann-sample_sort_tree_builder.hpp:44-58) and the tree is stored in
 // level order: trees[seg*2048 + t], t = 1..nspl, followed by the splitters in
@@ -91,7 +92,7 @@ __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, uint64_t seed, key64* trees,
                                                   int from_arena)
 {
-    __shared__ key64 s_samp[kMaxSamples];
+    __shared__ key64 s_samp[2][kMaxSamples];
     const LargeDesc D = desc[blockIdx.x];
     const Item seg = D.seg;
     if (!item_ok(seg, B)) {
@@ -100,23 +101,43 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
     }
     const key64* K = (seg.flags & F_BUFB) ? B.key1 : B.key0;
     const int k = (int)D.k, nspl = (1 << k) - 1, a = (int)D.oversample;
-    const int ss = a * nspl;
-    int P = 2;
+    const uint32_t ss = (uint32_t)(a * nspl);
+    uint32_t P = 64;
     while (P < ss) P <<= 1;
-    for (int j = threadIdx.x; j < P; j += kBigWG) {
+    if (P > (uint32_t)kMaxSamples) {
+        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
+        return;
+    }
+    // the samples, sorted in runs of 64 in the waves' registers (padding ~0 last)
+    for (uint32_t r = wave_id(); r < P / 64; r += kBigWaves) {
+        const uint32_t j = r * 64 + lane_id();
         key64 v = ~0ull;
         if (j < ss) {
             const uint64_t h = mix64(seed ^ ((uint64_t)seg.begin << 21) ^ (uint64_t)j);
             const uint32_t pos = seg.begin + (uint32_t)(((h >> 32) * (uint64_t)seg.size) >> 32);
             v = from_arena ? load_key(B.arena, B.alen, str_off(B, pos)) : K[pos];
         }
-        s_samp[j] = v;
+        uint32_t p = j;
+        wave_sort64(v, p);
+        s_samp[0][j] = v;
     }
     __syncthreads();
-    bitonic_sort_u64<kBigWG>(s_samp, P);
+    // pairwise merges by rank (equal keys of the left run first)
+    int cur = 0;
+    for (uint32_t L = 64; L < P; L <<= 1) {
+        for (uint32_t i = threadIdx.x; i < P; i += kBigWG) {
+            const uint32_t run = i / L, t = i % L, prt = run ^ 1u;
+            const key64 v = s_samp[cur][i];
+            const uint32_t c = run_rank(&s_samp[cur][prt * L], L, L, v, prt < run);
+            s_samp[cur ^ 1][min(run, prt) * L + t + c] = v;
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    const key64* sorted = s_samp[cur];
     key64* tree = trees + (uint64_t)blockIdx.x * kTreeSlots;
-    for (int t = threadIdx.x + 1; t <= nspl; t += kBigWG) tree[t] = s_samp[a / 2 + a * inorder_of(t, k)];
-    for (int i = threadIdx.x; i < nspl; i += kBigWG) tree[1024 + i] = s_samp[a / 2 + a * i];
+    for (int t = threadIdx.x + 1; t <= nspl; t += kBigWG) tree[t] = sorted[a / 2 + a * inorder_of(t, k)];
+    for (int i = threadIdx.x; i < nspl; i += kBigWG) tree[1024 + i] = sorted[a / 2 + a * i];
 }
 
 // Classify one tile of kTile keys.  Each wave ranks its own chunk of
@@ -527,21 +548,6 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
     }
     Item ni = { L.begin + lane, len, L.begin + lane, 0u };
     wave_append_segment(nr, ni, nxt);
-}
-
-// Number of elements of a sorted run a[0, len) that precede `key` of a run
-// further right (before_eq: elements equal to key precede it, i.e. the run is
-// to the left of the key's run).  Binary lifting over a power-of-two L >= len.
-__device__ __forceinline__ uint32_t run_rank(const key64* a, uint32_t len, uint32_t L, key64 key, bool before_eq)
-{
-    uint32_t c = 0;
-    for (uint32_t s = L; s > 0; s >>= 1) {
-        if (c + s <= len) {
-            const key64 x = a[c + s - 1];
-            if (x < key || (before_eq && x == key)) c += s;
-        }
-    }
-    return c;
 }
 
 // One workgroup sorts a block of 65..MAXM strings completely on chip and
