@@ -293,6 +293,41 @@ def rocprof_summary(d):
         w = float(np.mean(wr[k])) * fw * 1024 / 1e6 if k in wr else 0.0
         gbs = (f + w) * 1e6 / (avg_us * 1e-6) / 1e9 if avg_us > 0 else 0.0
         lines.append(f"| {k} | {calls} | {avg_us:.1f} | {tot_ns / 1e6:.3f} | {f:.1f} | {w:.1f} | {gbs:.1f} |")
+    # SQ counters (one pass of its own), per dispatch, averaged
+    sq = sorted(glob.glob(os.path.join(d, "**", "sq*counter_collection.csv"), recursive=True))
+    if sq:
+        per = collections.defaultdict(lambda: collections.defaultdict(list))
+        for r in csv.DictReader(open(sq[0])):
+            per[_kernel_base(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        names = ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_WAIT_ANY",
+                 "SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES"]
+        lines += ["", "SQ counters per dispatch (mean; WAIT_* as a fraction of WAVE_CYCLES)", "",
+                  "| kernel | VALU | SALU | LDS | LDS bank conflicts | wait any | wait LDS issue |",
+                  "|---|---|---|---|---|---|---|"]
+        for k, _ in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+            if k not in per or not all(n in per[k] for n in names):
+                continue
+            m = {n: float(np.mean(per[k][n])) for n in names}
+            wc = max(m["SQ_WAVE_CYCLES"], 1.0)
+            lines.append(f"| {k} | {m['SQ_INSTS_VALU'] / 1e6:.1f}M | {m['SQ_INSTS_SALU'] / 1e6:.1f}M | "
+                         f"{m['SQ_INSTS_LDS'] / 1e6:.1f}M | {m['SQ_LDS_BANK_CONFLICT'] / 1e6:.1f}M | "
+                         f"{m['SQ_WAIT_ANY'] / wc:.2f} | {m['SQ_WAIT_INST_LDS'] / wc:.2f} |")
+    # dispatch timeline of the last complete sort (from one k_tree at level 0 to the next)
+    tr = sorted(glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True))
+    if tr:
+        rows = sorted(csv.DictReader(open(tr[0])), key=lambda r: int(r["Start_Timestamp"]))
+        starts = [i for i, r in enumerate(rows) if _kernel_base(r["Kernel_Name"]) == "k_setup"]
+        if len(starts) >= 2:
+            seq = rows[starts[-2]:starts[-1]]
+            t0 = int(seq[0]["Start_Timestamp"])
+            lines += ["", "Dispatch timeline of one sort (us from its first dispatch)", "",
+                      "| kernel | start | duration | grid |", "|---|---|---|---|"]
+            for r in seq:
+                name = r["Kernel_Name"].strip('"')
+                short = _kernel_base(name) + ("<" + name.split("<", 1)[1].split(">", 1)[0] + ">" if "<" in name else "")
+                a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+                lines.append(f"| {short} | {(a - t0) / 1e3:.1f} | {(b - a) / 1e3:.1f} | {r['Grid_Size_X']} |")
+            lines.append(f"| (end) | {(int(seq[-1]['End_Timestamp']) - t0) / 1e3:.1f} | | |")
     print("\n".join(lines))
 
 

@@ -71,6 +71,13 @@ __global__ __launch_bounds__(kWG) void k_stage(const uint32_t* cnt, const Item* 
     for (uint32_t i = threadIdx.x; i < nl; i += kWG) dst[i] = list[i];
 }
 
+// copy 8-byte words (the level plan, from mapped host memory to the device)
+__global__ __launch_bounds__(kWG) void k_fetch(const uint64_t* src, uint64_t* dst, uint64_t words)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * kWG)
+        dst[i] = src[i];
+}
+
 // zero the counters whose bit is set in `mask`
 __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 {
@@ -835,8 +842,10 @@ struct ps5_ctx {
     uint32_t cap_large = 0, cap_block = 0, cap_tiny = 0, cap_run = 0;
     uint32_t* d_cnt = nullptr;
     uint32_t* h_cnt = nullptr;       // pinned
-    unsigned char* d_stage = nullptr;    // counters + list head for one readback copy
-    unsigned char* h_stage = nullptr;    // pinned
+    // counters + head of a list, written by k_stage straight into mapped
+    // host memory (no copy engine on the readback path)
+    unsigned char* h_stage = nullptr;    // pinned, mapped, coherent
+    unsigned char* h_stage_dev = nullptr;    // its device address
     uint32_t* h_upcnt = nullptr;     // pinned: counters uploaded with batch roots
     Item* h_roots = nullptr;         // pinned: root items of a batch
     uint64_t roots_cap = 0;
@@ -845,7 +854,8 @@ struct ps5_ctx {
     // plan of a level: its nlarge descriptors followed by its tile table, in
     // one buffer so that one copy uploads both
     LargeDesc* d_desc = nullptr;
-    LargeDesc* h_desc = nullptr;     // pinned
+    LargeDesc* h_desc = nullptr;     // pinned, mapped: the plan is pulled by k_fetch
+    LargeDesc* h_desc_dev = nullptr; // its device address
     uint64_t tiles_cap = 0;
     key64* trees = nullptr;          // kTreeSlots splitter slots per large segment
     uint64_t trees_cap = 0;
@@ -888,7 +898,7 @@ void free_buffers(ps5_ctx* c)
     c->d_desc = nullptr;
     if (c->h_large) (void)hipHostFree(c->h_large);
     if (c->h_desc) (void)hipHostFree(c->h_desc);
-    c->h_large = nullptr; c->h_desc = nullptr;
+    c->h_large = nullptr; c->h_desc = nullptr; c->h_desc_dev = nullptr;
     c->cap = 0;
     c->hist_cap = 0;
     c->part_cap = 0;
@@ -954,9 +964,11 @@ int ensure_capacity(ps5_ctx* c, uint64_t n)
     c->tiles_cap = cap / kTile + c->cap_large + 1;
     const uint64_t plan_bytes = (uint64_t)c->cap_large * sizeof(LargeDesc) + c->tiles_cap * sizeof(TileRef);
     static_assert(sizeof(LargeDesc) % alignof(TileRef) == 0, "tile table must stay aligned after the descriptors");
+    static_assert(sizeof(LargeDesc) % 8 == 0 && sizeof(TileRef) % 8 == 0, "the plan is fetched in 8-byte words");
     HIPCHK(hipMalloc(&c->d_desc, plan_bytes));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_large), (uint64_t)c->cap_large * sizeof(Item)));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_desc), plan_bytes));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_desc), plan_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_desc_dev), c->h_desc, 0));
     c->cap = cap;
     return PS5_OK;
 }
@@ -1028,10 +1040,8 @@ Sink make_sink(ps5_ctx* c, Item* large, int c_large, int rp)
 int readback(ps5_ctx* c, const Item* list, int ci)
 {
     // one staged copy holds the counters and the head of the list
-    hipLaunchKernelGGL(k_stage, dim3(1), dim3(kWG), 0, c->stream, c->d_cnt, list, ci, c->d_stage);
+    hipLaunchKernelGGL(k_stage, dim3(1), dim3(kWG), 0, c->stream, c->d_cnt, list, ci, c->h_stage_dev);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(c->h_stage, c->d_stage, kStageHead + (list ? kPeek * sizeof(Item) : 0),
-                          hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->st_syncs++;
     memcpy(c->h_cnt, c->h_stage, C_NCNT * sizeof(uint32_t));
@@ -1101,8 +1111,12 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         if (hl > 0xFFFFFFFFull) return set_err(PS5_ERR_CAPACITY, "histogram capacity");
     }
     if (int rc = grow_level(c, nlarge, hl)) return rc;
-    HIPCHK(hipMemcpyAsync(c->d_desc, c->h_desc, nlarge * sizeof(LargeDesc) + ntiles * sizeof(TileRef),
-                          hipMemcpyHostToDevice, st));
+    {   // the plan: pulled from mapped host memory by a kernel (no copy engine)
+        const uint64_t words = (nlarge * sizeof(LargeDesc) + ntiles * sizeof(TileRef)) / 8;
+        hipLaunchKernelGGL(k_fetch, dim3((uint32_t)std::min<uint64_t>(div_up(words, kWG), 1024)), dim3(kWG), 0, st,
+                           reinterpret_cast<const uint64_t*>(c->h_desc_dev), reinterpret_cast<uint64_t*>(c->d_desc),
+                           words);
+    }
     const uint32_t hl32 = (uint32_t)hl;
     {
         Launch L(c, K_TREE, (uint64_t)nlarge * kMaxSamples * 8);
@@ -1375,8 +1389,10 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     if (e == hipSuccess) e = hipMalloc(&c->d_cnt, C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_cnt), C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_upcnt), C_NCNT * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&c->d_stage, kStageHead + kPeek * sizeof(Item));
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_stage), kStageHead + kPeek * sizeof(Item));
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&c->h_stage), kStageHead + kPeek * sizeof(Item),
+                          hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_stage_dev), c->h_stage, 0);
     if (e == hipSuccess) e = hipMalloc(&c->d_sums, S_NUM * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_sums), S_NUM * sizeof(unsigned long long));
     if (e != hipSuccess) {
@@ -1399,7 +1415,6 @@ void ps5_ctx_destroy(ps5_ctx* c)
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
     if (c->h_upcnt) (void)hipHostFree(c->h_upcnt);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
-    if (c->d_stage) (void)hipFree(c->d_stage);
     if (c->h_roots) (void)hipHostFree(c->h_roots);
     if (c->h_sums) (void)hipHostFree(c->h_sums);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
