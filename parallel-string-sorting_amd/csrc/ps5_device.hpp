@@ -120,7 +120,9 @@ __device__ __forceinline__ void wave_cx(key64& k, uint32_t& p)
     const key64 ok = lane_xor64<J>(k);
     const uint32_t op = lane_xor<J>(p);
     const bool keep_min = ((lane & (uint32_t)J) == 0) == ((lane & (uint32_t)KK) == 0);
-    if (kp_less(ok, op, k, p) == keep_min) { k = ok; p = op; }
+    const bool take = kp_less(ok, op, k, p) == keep_min;
+    k = take ? ok : k;     // selects, not a branch around the moves
+    p = take ? op : p;
 }
 
 // Sort the 64 (key, position) pairs of a wave ascending, lane i ends with the
@@ -244,6 +246,14 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total,
 __device__ __forceinline__ uint32_t run_rank(const key64* a, uint32_t len, uint32_t L, key64 key, bool before_eq)
 {
     uint32_t c = 0;
+    if (len == L) {     // a full run: a[c + s - 1] with c + s <= L always exists, except in the first step
+        if (a[L - 1] < key || (before_eq && a[L - 1] == key)) return L;
+        for (uint32_t s = L >> 1; s > 0; s >>= 1) {
+            const key64 x = a[c + s - 1];
+            c += (x < key || (before_eq && x == key)) ? s : 0u;
+        }
+        return c;
+    }
     for (uint32_t s = L; s > 0; s >>= 1) {
         if (c + s <= len) {
             const key64 x = a[c + s - 1];
