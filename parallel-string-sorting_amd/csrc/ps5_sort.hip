@@ -192,40 +192,41 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     uint16_t* s_hist = s_hist_base + w * nb2;      // this wave's counts
     const uint32_t cb = t * kTile + w * kWaveChunk;
     const uint32_t ce = (item_ok(seg, B) && t < D.T) ? min(seg.size, cb + kWaveChunk) : 0u;
-    // two 64-key groups per iteration: both loads are in flight before the
+    // kGroups 64-key groups per iteration: all loads are in flight before the
     // tree descents, then the groups are ranked in order (stable)
-    auto load2 = [&](uint32_t g, key64& k0, key64& k1) {
-        const uint32_t p0 = g + lane, p1 = g + 64 + lane;
-        const bool a0 = p0 < ce, a1 = p1 < ce;
-        k0 = 0;
-        k1 = 0;
-        if (from_arena) {
-            if (a0) k0 = load_key(B.arena, B.alen, str_off(B, seg.begin + p0));
-            if (a1) k1 = load_key(B.arena, B.alen, str_off(B, seg.begin + p1));
-            if (a0) B.key0[seg.begin + p0] = k0;
-            if (a1) B.key0[seg.begin + p1] = k1;
-        } else {
-            if (a0) k0 = K[seg.begin + p0];
-            if (a1) k1 = K[seg.begin + p1];
+    constexpr int kGroups = 4;
+    for (uint32_t g = cb; g < ce; g += 64 * kGroups) {
+        key64 kv[kGroups];
+#pragma unroll
+        for (int j = 0; j < kGroups; ++j) {
+            const uint32_t p = g + 64 * j + lane;
+            kv[j] = 0;
+            if (p < ce) {
+                if (from_arena) {
+                    kv[j] = load_key(B.arena, B.alen, str_off(B, seg.begin + p));
+                    B.key0[seg.begin + p] = kv[j];
+                } else {
+                    kv[j] = K[seg.begin + p];
+                }
+            }
         }
-    };
-    auto rank1 = [&](uint32_t p, uint32_t b) {
-        const uint64_t peers = match_bucket(b);
-        const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
-        if (p < ce) {
-            const uint32_t old = s_hist[b];
-            if (r == 0) s_hist[b] = (uint16_t)(old + (uint32_t)__popcll(peers));
-            B.bktrank[seg.begin + p] = (b << kRankBits) | (old + r);
+        uint32_t bv[kGroups];
+#pragma unroll
+        for (int j = 0; j < kGroups; ++j) {
+            const uint32_t p = g + 64 * j + lane;
+            bv[j] = p < ce ? classify_key(kv[j], s_tree, s_sorted, k) : kNoBucket;
         }
-    };
-    for (uint32_t g = cb; g < ce; g += 128) {
-        key64 k0, k1;
-        load2(g, k0, k1);
-        const uint32_t p0 = g + lane, p1 = g + 64 + lane;
-        const uint32_t b0 = p0 < ce ? classify_key(k0, s_tree, s_sorted, k) : kNoBucket;
-        const uint32_t b1 = p1 < ce ? classify_key(k1, s_tree, s_sorted, k) : kNoBucket;
-        rank1(p0, b0);
-        rank1(p1, b1);
+#pragma unroll
+        for (int j = 0; j < kGroups; ++j) {
+            const uint32_t p = g + 64 * j + lane;
+            const uint64_t peers = match_bucket(bv[j]);
+            const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
+            if (p < ce) {
+                const uint32_t old = s_hist[bv[j]];
+                if (r == 0) s_hist[bv[j]] = (uint16_t)(old + (uint32_t)__popcll(peers));
+                B.bktrank[seg.begin + p] = (bv[j] << kRankBits) | (old + r);
+            }
+        }
     }
     __syncthreads();
     uint32_t* h = hist + D.hist_base;
@@ -365,18 +366,39 @@ __global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* desc, const
     const uint32_t* I = src ? B.idx1 : B.idx0;
     key64* KD = src ? B.key0 : B.key1;
     uint32_t* ID = src ? B.idx0 : B.idx1;
-    for (uint32_t p = threadIdx.x; p < m; p += kBigWG) {
+    // every load of the thread's kPer elements is issued before the first is used
+    constexpr int kPer = kWaveChunk / kBigWG;
+    static_assert(kWaveChunk % kBigWG == 0, "a chunk is a whole number of workgroup passes");
+    const bool ident = seg.flags & F_IDENT;
+    uint32_t br[kPer], iv[kPer];
+    key64 kv[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const uint32_t p = e * kBigWG + threadIdx.x;
         const uint32_t q = seg.begin + cb + p;
-        const uint32_t br = B.bktrank[q];
-        const uint32_t b = br >> kRankBits;
-        const uint32_t l = b < (uint32_t)nbkt ? s_start[b] + (br & kRankMask) : 0xFFFFFFFFu;
+        br[e] = 0xFFFFFFFFu;
+        if (p < m) {
+            br[e] = B.bktrank[q];
+            kv[e] = K[q];
+            iv[e] = ident ? q : I[q];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const uint32_t p = e * kBigWG + threadIdx.x;
+        if (p >= m) continue;
+        const uint32_t b = br[e] >> kRankBits;
+        const uint32_t l = b < (uint32_t)nbkt ? s_start[b] + (br[e] & kRankMask) : 0xFFFFFFFFu;
         if (l >= m) { flag_error(B.cnt, ERR_BOUNDS); continue; }
-        s_key[l] = K[q];
-        s_idx[l] = (seg.flags & F_IDENT) ? q : I[q];
+        s_key[l] = kv[e];
+        s_idx[l] = iv[e];
         s_bkt[l] = (uint16_t)b;
     }
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < m; j += kBigWG) {
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const uint32_t j = e * kBigWG + threadIdx.x;
+        if (j >= m) continue;
         const uint32_t b = s_bkt[j];
         const uint32_t rel = s_base[b] + (j - s_start[b]);
         if (rel >= seg.size) { flag_error(B.cnt, ERR_BOUNDS); continue; }
