@@ -579,23 +579,50 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
     wave_append_segment(nr, ni, nxt);
 }
 
-// One workgroup sorts a block of 65..MAXM strings completely on chip and
-// finishes every position at this depth (the reference's insertion-sort base
-// case, lcp_inssort.hpp:36-187, and the equal-bucket rules, .hpp:1531-1557,
-// on sorted keys).  Runs of 64 keys are sorted in the waves' registers
-// (wave_sort64), then merged pairwise: an element's place in the merged run
-// is its place in its own run plus the number of partner-run elements before
-// it (binary search in LDS; equal keys of the left run go first, so the
-// order stays stable).  Finishing a sorted position: LCP to its neighbour,
-// identical strings done, runs of equal keys without a NUL handed to round
-// r+1.
-// (MAXM is the largest block of the size class; LDS holds 28 B per string.)
-template <uint32_t MAXM>
-__global__ __launch_bounds__(kWG) void k_block(const Item* items, uint32_t first, const uint32_t* cnt_items,
-                                                Bufs B, Sink nxt, uint32_t depth)
+// Sorting network over E (key, position) pairs in one thread's registers
+// (Batcher's odd-even merge sort: 5 comparators for 4, 19 for 8).  Positions
+// are distinct, so the (key, position) order is total and the result stable.
+template <int E>
+__device__ __forceinline__ void thread_sort(key64 (&k)[E], uint32_t (&p)[E])
 {
-    __shared__ key64 s_k[2][MAXM];
-    __shared__ uint32_t s_p[2][MAXM];
+    static_assert(E == 4 || E == 8, "networks for 4 and 8 elements");
+    constexpr int n4[][2] = { { 0, 1 }, { 2, 3 }, { 0, 2 }, { 1, 3 }, { 1, 2 } };
+    constexpr int n8[][2] = { { 0, 1 }, { 2, 3 }, { 4, 5 }, { 6, 7 }, { 0, 2 }, { 1, 3 }, { 4, 6 },
+                              { 5, 7 }, { 1, 2 }, { 5, 6 }, { 0, 4 }, { 1, 5 }, { 2, 6 }, { 3, 7 },
+                              { 2, 4 }, { 3, 5 }, { 1, 2 }, { 3, 4 }, { 5, 6 } };
+    constexpr int NC = E == 4 ? 5 : 19;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int a = E == 4 ? n4[c][0] : n8[c][0], b = E == 4 ? n4[c][1] : n8[c][1];
+        const bool sw = kp_less(k[b], p[b], k[a], p[a]);
+        const key64 ka = k[a], kb = k[b];
+        const uint32_t pa = p[a], pb = p[b];
+        k[a] = sw ? kb : ka;
+        k[b] = sw ? ka : kb;
+        p[a] = sw ? pb : pa;
+        p[b] = sw ? pa : pb;
+    }
+}
+
+// One workgroup of MAXM / E threads sorts a block of 65..MAXM strings
+// completely on chip and finishes every position at this depth (the
+// reference's insertion-sort base case, lcp_inssort.hpp:36-187, and the
+// equal-bucket rules, .hpp:1531-1557, on sorted keys).  Each thread sorts E
+// consecutive keys with a sorting network, then runs are merged pairwise by
+// merge path: a thread finds where its E outputs start in the two runs by
+// binary search (co-rank) and merges them sequentially; on equal keys the
+// left run goes first, so the order stays stable.  This costs far fewer
+// instructions per key than a bitonic network, and the kernel is bound by
+// instruction throughput.  Finishing a sorted position: LCP to its neighbour,
+// identical strings done, runs of equal keys without a NUL handed to round r+1.
+template <uint32_t MAXM, int E>
+__global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t first, const uint32_t* cnt_items,
+                                                     Bufs B, Sink nxt, uint32_t depth)
+{
+    constexpr uint32_t T = MAXM / E;
+    static_assert(T % 64 == 0 && MAXM <= 65536, "whole waves; positions fit 16 bits");
+    __shared__ key64 s_k[MAXM];
+    __shared__ uint16_t s_p[MAXM];
     __shared__ uint32_t s_id[MAXM];
     const uint32_t idx = first + blockIdx.x;
     if (idx >= *cnt_items) return;
@@ -604,58 +631,94 @@ __global__ __launch_bounds__(kWG) void k_block(const Item* items, uint32_t first
         if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
-    const uint32_t m = L.size, beg = L.begin, tid = threadIdx.x, lane = lane_id();
+    const uint32_t m = L.size, beg = L.begin, tid = threadIdx.x;
     const int src = L.flags & F_BUFB;
     const key64* K = src ? B.key1 : B.key0;
     const uint32_t* I = src ? B.idx1 : B.idx0;
-    const uint32_t R = (m + 63) / 64, P = R * 64;
-    // runs of 64 sorted in registers; padding (key ~0, position >= m) sorts last
-    for (uint32_t r = wave_id(); r < R; r += kWaves) {
-        const uint32_t i = r * 64 + lane;
-        key64 k = i < m ? K[beg + i] : ~0ull;
-        uint32_t p = i;
-        if (i < m) s_id[i] = (L.flags & F_IDENT) ? beg + i : I[beg + i];
-        wave_sort64(k, p);
-        s_k[0][i] = k;
-        s_p[0][i] = p;
+    const bool ident = L.flags & F_IDENT;
+    const uint32_t Ta = (m + E - 1) / E, Pm = Ta * E;   // threads in use, padded size
+    // coalesced loads; padding (key ~0, position >= m) sorts last
+    for (uint32_t i = tid; i < Pm; i += T) {
+        s_k[i] = i < m ? K[beg + i] : ~0ull;
+        if (i < m) s_id[i] = ident ? beg + i : I[beg + i];
     }
     __syncthreads();
-    int cur = 0;
-    for (uint32_t Lr = 64; Lr < P; Lr <<= 1) {
-        for (uint32_t i = tid; i < P; i += kWG) {
-            const uint32_t run = i / Lr, t = i % Lr;
-            const uint32_t prt = run ^ 1u, pstart = prt * Lr;
-            const key64 k = s_k[cur][i];
-            const uint32_t p = s_p[cur][i];
-            uint32_t c = 0;
-            if (pstart < P) c = run_rank(&s_k[cur][pstart], min(Lr, P - pstart), Lr, k, prt < run);
-            const uint32_t dst = min(run, prt) * Lr + t + c;
-            s_k[cur ^ 1][dst] = k;
-            s_p[cur ^ 1][dst] = p;
+    key64 k[E];
+    uint32_t p[E];
+    const uint32_t o = tid * E;
+    if (tid < Ta) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            k[e] = s_k[o + e];
+            p[e] = o + e;
+        }
+        thread_sort<E>(k, p);
+    }
+    __syncthreads();
+    if (tid < Ta) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            s_k[o + e] = k[e];
+            s_p[o + e] = (uint16_t)p[e];
+        }
+    }
+    __syncthreads();
+    for (uint32_t Lr = E; Lr < Pm; Lr <<= 1) {
+        if (tid < Ta) {
+            const uint32_t a0 = (o / (2 * Lr)) * (2 * Lr), la = min(Lr, Pm - a0);
+            const uint32_t b0 = a0 + Lr, lb = b0 < Pm ? min(Lr, Pm - b0) : 0u;
+            const uint32_t diag = o - a0;
+            uint32_t lo = diag > lb ? diag - lb : 0u, hi = min(diag, la);
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_k[a0 + mid] <= s_k[b0 + diag - 1 - mid]) lo = mid + 1;
+                else hi = mid;
+            }
+            uint32_t i = lo, j = diag - lo;
+            key64 ka = i < la ? s_k[a0 + i] : 0ull, kb = j < lb ? s_k[b0 + j] : 0ull;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const bool take_a = j >= lb || (i < la && ka <= kb);
+                k[e] = take_a ? ka : kb;
+                p[e] = take_a ? s_p[a0 + i] : s_p[b0 + j];
+                if (take_a) {
+                    ++i;
+                    ka = i < la ? s_k[a0 + i] : 0ull;
+                } else {
+                    ++j;
+                    kb = j < lb ? s_k[b0 + j] : 0ull;
+                }
+            }
         }
         __syncthreads();
-        cur ^= 1;
+        if (tid < Ta) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                s_k[o + e] = k[e];
+                s_p[o + e] = (uint16_t)p[e];
+            }
+        }
+        __syncthreads();
     }
-    const key64* sk = s_k[cur];
-    const uint32_t* sp = s_p[cur];
-    for (uint32_t i = tid; i < ((m + kWG - 1) / kWG) * kWG; i += kWG) {
+    for (uint32_t i = tid; i < ((m + T - 1) / T) * T; i += T) {
         const bool act = i < m;     // sorted position
         bool eqp = false, eqn = false;
-        key64 k = 0, pk = 0;
+        key64 kk = 0, pk = 0;
         if (act) {
-            k = sk[i];
-            if (i > 0) pk = sk[i - 1];
-            eqp = i > 0 && pk == k;
-            eqn = i + 1 < m && sk[i + 1] == k;
-            finish_position(B, beg + i, s_id[sp[i]], k, i > 0, pk, !eqp && !eqn, depth);
-            if (i == 0 || i == m - 1) B.bkey[beg + i] = k;
+            kk = s_k[i];
+            if (i > 0) pk = s_k[i - 1];
+            eqp = i > 0 && pk == kk;
+            eqn = i + 1 < m && s_k[i + 1] == kk;
+            finish_position(B, beg + i, s_id[s_p[i]], kk, i > 0, pk, !eqp && !eqn, depth);
+            if (i == 0 || i == m - 1) B.bkey[beg + i] = kk;
         }
         // a run of equal keys without a NUL continues at depth + 8
-        const bool nr = act && !eqp && eqn && (k & 0xFFull) != 0;
+        const bool nr = act && !eqp && eqn && (kk & 0xFFull) != 0;
+        if (__ballot(nr) == 0) continue;     // (uniform) nothing to append in this wave
         uint32_t rl = 0;
         if (nr) {
             rl = 1;
-            while (i + rl < m && sk[i + rl] == k) ++rl;
+            while (i + rl < m && s_k[i + rl] == kk) ++rl;
         }
         Item ni = { beg + i, rl, beg + i, 0u };
         wave_append_segment(nr, ni, nxt);
@@ -1317,14 +1380,14 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                 {
                     Launch L(c, K_BLOCK, 0);
                     if (cls == 0)
-                        hipLaunchKernelGGL(k_block<256>, dim3(nblk[cls]), dim3(kWG), 0, st, list, blk_done[cls], cnt,
-                                           B, nsink, depth);
+                        hipLaunchKernelGGL((k_block<256, 4>), dim3(nblk[cls]), dim3(256 / 4), 0, st, list,
+                                           blk_done[cls], cnt, B, nsink, depth);
                     else if (cls == 1)
-                        hipLaunchKernelGGL(k_block<512>, dim3(nblk[cls]), dim3(kWG), 0, st, list, blk_done[cls], cnt,
-                                           B, nsink, depth);
+                        hipLaunchKernelGGL((k_block<512, 8>), dim3(nblk[cls]), dim3(512 / 8), 0, st, list,
+                                           blk_done[cls], cnt, B, nsink, depth);
                     else
-                        hipLaunchKernelGGL(k_block<kBlkT>, dim3(nblk[cls]), dim3(kWG), 0, st, list, blk_done[cls],
-                                           cnt, B, nsink, depth);
+                        hipLaunchKernelGGL((k_block<kBlkT, 8>), dim3(nblk[cls]), dim3(kBlkT / 8), 0, st, list,
+                                           blk_done[cls], cnt, B, nsink, depth);
                 }
                 sum_sizes(c, list, blk_done[cls], cnt, blk_done[cls] + nblk[cls], S_BLOCK);
                 blk_done[cls] += nblk[cls];
