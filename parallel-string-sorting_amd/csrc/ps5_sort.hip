@@ -513,13 +513,27 @@ __device__ __forceinline__ void finish_position(const Bufs& B, uint32_t pos, uin
     }
 }
 
-// One wave per leaf of at most 64 strings: rank sort with broadcast compares
-// (the position each string takes in an insertion sort, computed directly).
+// LDS exchange between the lanes of one wave: its LDS operations execute in
+// order, so keeping the compiler from reordering them is enough.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One wave per leaf of at most 64 strings, finished completely: a rank sort
+// by broadcast compares (the slot each string takes in an LCP insertion sort,
+// lcp_inssort.hpp:36-187) at the leaf's depth, then, while runs of equal keys
+// without a NUL remain, their strings fetch their next 8 characters and every
+// such run is re-sorted in place.  Runs never leave the wave, so deep shared
+// prefixes cost one gather per string per 8 characters and no work-list
+// traffic (the reference likewise finishes small buckets by insertion sort).
 __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint32_t* cnt_items, Bufs B,
                                                    Sink nxt, uint32_t depth)
 {
     __shared__ key64 s_k[kWaves][64];
     __shared__ uint32_t s_i[kWaves][64];
+    __shared__ uint32_t s_r[kWaves][64];     // first sorted position of the run holding a position
     const uint32_t w = wave_id(), lane = lane_id();
     const uint32_t it = blockIdx.x * kWaves + w;
     const bool valid = it < *cnt_items;
@@ -532,51 +546,87 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
         if (lane == 0) flag_error(B.cnt, ERR_BOUNDS);
         L.size = 0;
     }
+    (void)nxt;               // tiny leaves never hand work to the next round
     const uint32_t m = L.size;
+    if (m == 0) return;      // (wave-uniform)
     const bool act = lane < m;
     const int src = L.flags & F_BUFB;
-    key64 key = ~0ull;
+    key64 k = ~0ull;
     uint32_t id = 0;
     if (act) {
-        key = src ? B.key1[L.begin + lane] : B.key0[L.begin + lane];
+        k = src ? B.key1[L.begin + lane] : B.key0[L.begin + lane];
         id = (L.flags & F_IDENT) ? L.begin + lane : src ? B.idx1[L.begin + lane] : B.idx0[L.begin + lane];
     }
-    if (act) s_k[w][lane] = key;
-    __syncthreads();
-    uint32_t rank = 0;
-    for (uint32_t i = 0; i < m; ++i) {
-        const key64 ki = s_k[w][i];
-        rank += (ki < key || (ki == key && i < lane)) ? 1u : 0u;
-    }
-    __syncthreads();
+    // every string of the leaf starts in run 0; (key, slot) orders a run stably
+    uint32_t run = 0;
+    bool live = act;         // position still has an undecided successor relation
     if (act) {
-        s_k[w][rank] = key;
-        s_i[w][rank] = id;
+        s_k[w][lane] = k;
+        s_r[w][lane] = 0;
     }
-    __syncthreads();
-    key64 k = 0, pk = 0, nk = 0;
-    if (act) {
-        k = s_k[w][lane];
-        id = s_i[w][lane];
-        if (lane > 0) pk = s_k[w][lane - 1];
-        if (lane + 1 < m) nk = s_k[w][lane + 1];
+    uint32_t cur = depth;
+    for (;;) {
+        // re-sort every live run by (key, slot); finished positions keep their slot
+        wave_lds_sync();
+        uint32_t rank = lane;
+        if (live) {
+            rank = run;
+            for (uint32_t j = run; j < m && s_r[w][j] == run; ++j) {
+                const key64 kj = s_k[w][j];
+                rank += (kj < k || (kj == k && j < lane)) ? 1u : 0u;
+            }
+        }
+        wave_lds_sync();
+        if (live) {
+            s_k[w][rank] = k;
+            s_i[w][rank] = id;
+        } else if (act) {
+            s_i[w][lane] = id;
+        }
+        wave_lds_sync();
+        if (act) {
+            k = s_k[w][lane];
+            id = s_i[w][lane];
+        }
+        wave_lds_sync();
+        // neighbours inside the same run at this depth
+        key64 pk = 0;
+        bool eqp = false, eqn = false, same_prev = false;
+        if (act && live) {
+            same_prev = lane > 0 && s_r[w][lane - 1] == run;
+            if (same_prev) pk = s_k[w][lane - 1];
+            eqp = same_prev && pk == k;
+            eqn = lane + 1 < m && s_r[w][lane + 1] == run && s_k[w][lane + 1] == k;
+        }
+        const bool term = (k & 0xFFull) == 0;
+        if (act && live) {
+            const uint32_t pos = L.begin + lane;
+            if (cur == depth && (lane == 0 || lane == m - 1)) B.bkey[pos] = k;
+            if (same_prev && !eqp && B.lcp) B.lcp[pos] = cur + lcp_key(pk, k);
+            if (eqp && term && B.lcp) B.lcp[pos] = cur + key_depth(k);
+            if ((!eqp && !eqn) || term) {
+                B.perm[pos] = id;     // decided: a single string or identical strings
+                live = false;
+            }
+        }
+        // new runs: a live position starts one where it differs from its predecessor
+        const uint64_t starts = __ballot(act && live && !eqp);
+        const uint64_t alive = __ballot(act && live);
+        if (alive == 0) break;
+        if (live) {
+            const uint64_t below = starts & ((2ull << lane) - 1);   // start bits at or below this lane
+            run = 63u - (uint32_t)__clzll((long long)below);
+        }
+        wave_lds_sync();
+        if (act) s_r[w][lane] = live ? run : 0xFFFFFFFFu;
+        // the next 8 characters of every string still in a run
+        cur += 8;
+        if (live) {
+            if (id < B.n) k = load_key(B.arena, B.alen, str_off(B, id) + cur);
+            else { flag_error(B.cnt, ERR_BOUNDS); k = 0; }
+            s_k[w][lane] = k;
+        }
     }
-    const bool eqp = act && lane > 0 && pk == k;
-    const bool eqn = act && lane + 1 < m && nk == k;
-    if (act) {
-        const uint32_t pos = L.begin + lane;
-        finish_position(B, pos, id, k, lane > 0, pk, !eqp && !eqn, depth);
-        if (lane == 0 || lane == m - 1) B.bkey[pos] = k;
-    }
-    // a run of equal keys without a NUL continues at depth + 8
-    const bool nr = act && !eqp && eqn && (k & 0xFFull) != 0;
-    uint32_t len = 0;
-    if (nr) {
-        len = 1;
-        while (lane + len < m && s_k[w][lane + len] == k) ++len;
-    }
-    Item ni = { L.begin + lane, len, L.begin + lane, 0u };
-    wave_append_segment(nr, ni, nxt);
 }
 
 // Sorting network over E (key, position) pairs in one thread's registers
@@ -959,6 +1009,7 @@ struct ps5_ctx {
     uint64_t seed = 0x5EED5EEDull;
     // profiling
     bool profile = false;
+    bool trace = false;              // PS5_TRACE set: per-round work to stderr (diagnostics)
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
     double k_ms[K_NUM] = {};
@@ -1401,6 +1452,11 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
         uint32_t nb = 0;
         for (int cls = 0; cls < kBlkClasses; ++cls) nb += c->h_cnt[C_BLOCK + 2 * cls + rp];
         c->st_leaves += nt + nb + nr;
+        if (c->trace) {
+            fprintf(stderr, "[ps5] round %u depth %u: tiny %u, blocks %u/%u/%u, equal chunks %u, levels %llu\n", round,
+                    depth, nt, c->h_cnt[C_BLOCK + rp], c->h_cnt[C_BLOCK + 2 + rp], c->h_cnt[C_BLOCK + 4 + rp], nr,
+                    (unsigned long long)c->st_levels);
+        }
         if (nt) {
             {
                 Launch L(c, K_LEAF_TINY, 0);
@@ -1470,6 +1526,7 @@ int ps5_ctx_create(ps5_ctx** out, int device)
                                count_lds_bytes(kMaxTreebits)));
     ps5_ctx* c = new ps5_ctx;
     c->device = device;
+    c->trace = getenv("PS5_TRACE") != nullptr;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_cnt, C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_cnt), C_NCNT * sizeof(uint32_t));
