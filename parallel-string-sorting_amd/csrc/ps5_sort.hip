@@ -564,7 +564,10 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
         s_k[w][lane] = k;
         s_r[w][lane] = 0;
     }
-    uint32_t cur = depth;
+    uint32_t cur = depth + 8 * (L.flags >> kFlagShift);
+    // a leaf handed over by a block one step deeper leaves bkey alone: its end
+    // positions keep the block's round-depth keys for the leaf boundaries
+    bool first_pass = (L.flags >> kFlagShift) == 0;
     for (;;) {
         // re-sort every live run by (key, slot); finished positions keep their slot
         wave_lds_sync();
@@ -601,7 +604,7 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
         const bool term = (k & 0xFFull) == 0;
         if (act && live) {
             const uint32_t pos = L.begin + lane;
-            if (cur == depth && (lane == 0 || lane == m - 1)) B.bkey[pos] = k;
+            if (first_pass && (lane == 0 || lane == m - 1)) B.bkey[pos] = k;
             if (same_prev && !eqp && B.lcp) B.lcp[pos] = cur + lcp_key(pk, k);
             if (eqp && term && B.lcp) B.lcp[pos] = cur + key_depth(k);
             if ((!eqp && !eqn) || term) {
@@ -620,6 +623,7 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
         wave_lds_sync();
         if (act) s_r[w][lane] = live ? run : 0xFFFFFFFFu;
         // the next 8 characters of every string still in a run
+        first_pass = false;
         cur += 8;
         if (live) {
             if (id < B.n) k = load_key(B.arena, B.alen, str_off(B, id) + cur);
@@ -667,7 +671,7 @@ __device__ __forceinline__ void thread_sort(key64 (&k)[E], uint32_t (&p)[E])
 // identical strings done, runs of equal keys without a NUL handed to round r+1.
 template <uint32_t MAXM, int E>
 __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t first, const uint32_t* cnt_items,
-                                                     Bufs B, Sink nxt, uint32_t depth)
+                                                     Bufs B, Sink nxt, Sink cur, uint32_t depth)
 {
     constexpr uint32_t T = MAXM / E;
     static_assert(T % 64 == 0 && MAXM <= 65536, "whole waves; positions fit 16 bits");
@@ -762,7 +766,10 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
             finish_position(B, beg + i, s_id[s_p[i]], kk, i > 0, pk, !eqp && !eqn, depth);
             if (i == 0 || i == m - 1) B.bkey[beg + i] = kk;
         }
-        // a run of equal keys without a NUL continues at depth + 8
+        // a run of equal keys without a NUL continues at depth + 8: a run of
+        // at most kTinyT strings as a tiny leaf of this round (its keys at
+        // depth + 8 are in place, finish_position fetched them), a longer one
+        // as a segment of the next round
         const bool nr = act && !eqp && eqn && (kk & 0xFFull) != 0;
         if (__ballot(nr) == 0) continue;     // (uniform) nothing to append in this wave
         uint32_t rl = 0;
@@ -770,8 +777,11 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
             rl = 1;
             while (i + rl < m && s_k[i + rl] == kk) ++rl;
         }
+        const bool small = nr && rl <= kTinyT;
+        const uint32_t slot = wave_append(small, &cur.cnt[cur.c_tiny]);
+        if (small) put_item(cur.tiny, cur.cap_tiny, slot, Item{ beg + i, rl, beg + i, 1u << kFlagShift }, cur.cnt);
         Item ni = { beg + i, rl, beg + i, 0u };
-        wave_append_segment(nr, ni, nxt);
+        wave_append_segment(nr && !small, ni, nxt);
     }
 }
 
@@ -835,7 +845,7 @@ __global__ __launch_bounds__(kWG) void k_leafbound(LeafLists LL, Bufs B, uint32_
 #pragma unroll
     for (int j = 0; j < NL - 1; ++j) l += i >= LL.end[j] ? 1 : 0;
     const Item L = LL.list[l][i - (l ? LL.end[l - 1] : 0u)];
-    if (l == NL - 1 && (L.flags >> 8) != 0) return;   // only the first chunk of an equal-key leaf
+    if (l == NL - 1 && (L.flags >> kFlagShift) != 0) return;   // only the first chunk of an equal-key leaf
     if (L.begin <= L.rbeg || !B.lcp) return;
     if (L.begin >= B.n) { flag_error(B.cnt, ERR_BOUNDS); return; }
     const key64 a = B.bkey[L.begin - 1], c = B.bkey[L.begin];
@@ -1432,13 +1442,13 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                     Launch L(c, K_BLOCK, 0);
                     if (cls == 0)
                         hipLaunchKernelGGL((k_block<256, 4>), dim3(nblk[cls]), dim3(256 / 4), 0, st, list,
-                                           blk_done[cls], cnt, B, nsink, depth);
+                                           blk_done[cls], cnt, B, nsink, sink, depth);
                     else if (cls == 1)
                         hipLaunchKernelGGL((k_block<512, 8>), dim3(nblk[cls]), dim3(512 / 8), 0, st, list,
-                                           blk_done[cls], cnt, B, nsink, depth);
+                                           blk_done[cls], cnt, B, nsink, sink, depth);
                     else
                         hipLaunchKernelGGL((k_block<kBlkT, 8>), dim3(nblk[cls]), dim3(kBlkT / 8), 0, st, list,
-                                           blk_done[cls], cnt, B, nsink, depth);
+                                           blk_done[cls], cnt, B, nsink, sink, depth);
                 }
                 sum_sizes(c, list, blk_done[cls], cnt, blk_done[cls] + nblk[cls], S_BLOCK);
                 blk_done[cls] += nblk[cls];
