@@ -57,9 +57,7 @@ struct Item {            // a segment or a leaf
 constexpr uint32_t F_BUFB = 1u;              // data lives in key[1]/idx[1]
 constexpr uint32_t F_SORTED = 2u;            // all keys equal (equal bucket)
 constexpr uint32_t F_IDENT = 4u;             // a root: idx[p] = p, not stored
-// flags >> kFlagShift: chunk index of an equal-key chunk; for a tiny leaf, how
-// many 8-character steps beyond the round's depth its keys are
-constexpr uint32_t kFlagShift = 8;
+constexpr uint32_t kFlagShift = 8;           // flags >> kFlagShift: chunk index of an equal-key chunk
 
 // device counters
 enum : int {

@@ -471,7 +471,7 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
             const uint32_t nch = (m + kRunChunk - 1) / kRunChunk;
             for (uint32_t c = 0; c < nch; ++c) {
                 Item r = it;
-                r.flags = dstbuf | F_SORTED | (c << 8);
+                r.flags = dstbuf | F_SORTED | (c << kFlagShift);
                 put_item(s.run, s.cap_run, o_r++, r, s.cnt);
             }
         } else if (m <= kBlkT) {
@@ -521,89 +521,67 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// One wave per leaf of at most 64 strings, finished completely: a rank sort
-// by broadcast compares (the slot each string takes in an LCP insertion sort,
-// lcp_inssort.hpp:36-187) at the leaf's depth, then, while runs of equal keys
-// without a NUL remain, their strings fetch their next 8 characters and every
-// such run is re-sorted in place.  Runs never leave the wave, so deep shared
-// prefixes cost one gather per string per 8 characters and no work-list
-// traffic (the reference likewise finishes small buckets by insertion sort).
-__global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint32_t* cnt_items, Bufs B,
-                                                   Sink nxt, uint32_t depth)
+// One wave finishes a leaf of m <= 64 strings at positions [begin, begin+m)
+// completely: a rank sort by broadcast compares (the slot each string takes
+// in an LCP insertion sort, lcp_inssort.hpp:36-187) at depth `cur`, then,
+// while runs of equal keys without a NUL remain, their strings fetch their
+// next 8 characters and every such run is re-sorted in place.  Runs never
+// leave the wave, so deep shared prefixes cost one gather per string per 8
+// characters and no work-list traffic (the reference likewise finishes small
+// buckets by insertion sort).  k, id: lane's key at `cur` and input index
+// (lanes >= m ignored).  s_k, s_i, s_r: this wave's 64-entry LDS arrays.
+// write_bkey: record the round-depth keys of the two end positions for the
+// leaf-boundary pass.  All 64 lanes must call this.
+__device__ void wave_finish_leaf(const Bufs& B, uint32_t begin, uint32_t m, key64 k, uint32_t id, uint32_t cur,
+                                 bool write_bkey, key64* s_k, uint32_t* s_i, uint32_t* s_r)
 {
-    __shared__ key64 s_k[kWaves][64];
-    __shared__ uint32_t s_i[kWaves][64];
-    __shared__ uint32_t s_r[kWaves][64];     // first sorted position of the run holding a position
-    const uint32_t w = wave_id(), lane = lane_id();
-    const uint32_t it = blockIdx.x * kWaves + w;
-    const bool valid = it < *cnt_items;
-    Item L = { 0u, 0u, 0u, 0u };
-    if (valid) L = items[it];
-    L.begin = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.begin);
-    L.size = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.size);
-    L.flags = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.flags);
-    if (valid && (!item_ok(L, B) || L.size > kTinyT)) {
-        if (lane == 0) flag_error(B.cnt, ERR_BOUNDS);
-        L.size = 0;
-    }
-    (void)nxt;               // tiny leaves never hand work to the next round
-    const uint32_t m = L.size;
-    if (m == 0) return;      // (wave-uniform)
+    const uint32_t lane = lane_id();
     const bool act = lane < m;
-    const int src = L.flags & F_BUFB;
-    key64 k = ~0ull;
-    uint32_t id = 0;
-    if (act) {
-        k = src ? B.key1[L.begin + lane] : B.key0[L.begin + lane];
-        id = (L.flags & F_IDENT) ? L.begin + lane : src ? B.idx1[L.begin + lane] : B.idx0[L.begin + lane];
-    }
     // every string of the leaf starts in run 0; (key, slot) orders a run stably
     uint32_t run = 0;
     bool live = act;         // position still has an undecided successor relation
+    wave_lds_sync();
     if (act) {
-        s_k[w][lane] = k;
-        s_r[w][lane] = 0;
+        s_k[lane] = k;
+        s_r[lane] = 0;
     }
-    uint32_t cur = depth + 8 * (L.flags >> kFlagShift);
-    // a leaf handed over by a block one step deeper leaves bkey alone: its end
-    // positions keep the block's round-depth keys for the leaf boundaries
-    bool first_pass = (L.flags >> kFlagShift) == 0;
+    bool first_pass = write_bkey;
     for (;;) {
         // re-sort every live run by (key, slot); finished positions keep their slot
         wave_lds_sync();
         uint32_t rank = lane;
         if (live) {
             rank = run;
-            for (uint32_t j = run; j < m && s_r[w][j] == run; ++j) {
-                const key64 kj = s_k[w][j];
+            for (uint32_t j = run; j < m && s_r[j] == run; ++j) {
+                const key64 kj = s_k[j];
                 rank += (kj < k || (kj == k && j < lane)) ? 1u : 0u;
             }
         }
         wave_lds_sync();
         if (live) {
-            s_k[w][rank] = k;
-            s_i[w][rank] = id;
+            s_k[rank] = k;
+            s_i[rank] = id;
         } else if (act) {
-            s_i[w][lane] = id;
+            s_i[lane] = id;
         }
         wave_lds_sync();
         if (act) {
-            k = s_k[w][lane];
-            id = s_i[w][lane];
+            k = s_k[lane];
+            id = s_i[lane];
         }
         wave_lds_sync();
         // neighbours inside the same run at this depth
         key64 pk = 0;
         bool eqp = false, eqn = false, same_prev = false;
         if (act && live) {
-            same_prev = lane > 0 && s_r[w][lane - 1] == run;
-            if (same_prev) pk = s_k[w][lane - 1];
+            same_prev = lane > 0 && s_r[lane - 1] == run;
+            if (same_prev) pk = s_k[lane - 1];
             eqp = same_prev && pk == k;
-            eqn = lane + 1 < m && s_r[w][lane + 1] == run && s_k[w][lane + 1] == k;
+            eqn = lane + 1 < m && s_r[lane + 1] == run && s_k[lane + 1] == k;
         }
         const bool term = (k & 0xFFull) == 0;
         if (act && live) {
-            const uint32_t pos = L.begin + lane;
+            const uint32_t pos = begin + lane;
             if (first_pass && (lane == 0 || lane == m - 1)) B.bkey[pos] = k;
             if (same_prev && !eqp && B.lcp) B.lcp[pos] = cur + lcp_key(pk, k);
             if (eqp && term && B.lcp) B.lcp[pos] = cur + key_depth(k);
@@ -621,16 +599,47 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
             run = 63u - (uint32_t)__clzll((long long)below);
         }
         wave_lds_sync();
-        if (act) s_r[w][lane] = live ? run : 0xFFFFFFFFu;
+        if (act) s_r[lane] = live ? run : 0xFFFFFFFFu;
         // the next 8 characters of every string still in a run
         first_pass = false;
         cur += 8;
         if (live) {
             if (id < B.n) k = load_key(B.arena, B.alen, str_off(B, id) + cur);
             else { flag_error(B.cnt, ERR_BOUNDS); k = 0; }
-            s_k[w][lane] = k;
+            s_k[lane] = k;
         }
     }
+}
+
+// One wave per tiny leaf (at most 64 strings), finished completely.
+__global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint32_t* cnt_items, Bufs B,
+                                                   uint32_t depth)
+{
+    __shared__ key64 s_k[kWaves][64];
+    __shared__ uint32_t s_i[kWaves][64];
+    __shared__ uint32_t s_r[kWaves][64];     // first sorted position of the run holding a position
+    const uint32_t w = wave_id(), lane = lane_id();
+    const uint32_t it = blockIdx.x * kWaves + w;
+    const bool valid = it < *cnt_items;
+    Item L = { 0u, 0u, 0u, 0u };
+    if (valid) L = items[it];
+    L.begin = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.begin);
+    L.size = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.size);
+    L.flags = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.flags);
+    if (valid && (!item_ok(L, B) || L.size > kTinyT)) {
+        if (lane == 0) flag_error(B.cnt, ERR_BOUNDS);
+        L.size = 0;
+    }
+    const uint32_t m = L.size;
+    if (m == 0) return;      // (wave-uniform)
+    const int src = L.flags & F_BUFB;
+    key64 k = ~0ull;
+    uint32_t id = 0;
+    if (lane < m) {
+        k = src ? B.key1[L.begin + lane] : B.key0[L.begin + lane];
+        id = (L.flags & F_IDENT) ? L.begin + lane : src ? B.idx1[L.begin + lane] : B.idx0[L.begin + lane];
+    }
+    wave_finish_leaf(B, L.begin, m, k, id, depth, true, s_k[w], s_i[w], s_r[w]);
 }
 
 // Sorting network over E (key, position) pairs in one thread's registers
@@ -671,13 +680,17 @@ __device__ __forceinline__ void thread_sort(key64 (&k)[E], uint32_t (&p)[E])
 // identical strings done, runs of equal keys without a NUL handed to round r+1.
 template <uint32_t MAXM, int E>
 __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t first, const uint32_t* cnt_items,
-                                                     Bufs B, Sink nxt, Sink cur, uint32_t depth)
+                                                     Bufs B, Sink nxt, uint32_t depth)
 {
     constexpr uint32_t T = MAXM / E;
     static_assert(T % 64 == 0 && MAXM <= 65536, "whole waves; positions fit 16 bits");
     __shared__ key64 s_k[MAXM];
     __shared__ uint16_t s_p[MAXM];
     __shared__ uint32_t s_id[MAXM];
+    constexpr uint32_t kMaxRuns = 64;            // short equal runs finished here (more go to the next round)
+    __shared__ uint32_t s_runs[kMaxRuns];        // (start << 16) | length
+    __shared__ uint32_t s_nruns;
+    static_assert(MAXM >= 128 * (T / 64), "the per-wave leaf arrays reuse s_k and s_id");
     const uint32_t idx = first + blockIdx.x;
     if (idx >= *cnt_items) return;
     const Item L = items[idx];
@@ -691,6 +704,7 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
     const uint32_t* I = src ? B.idx1 : B.idx0;
     const bool ident = L.flags & F_IDENT;
     const uint32_t Ta = (m + E - 1) / E, Pm = Ta * E;   // threads in use, padded size
+    if (tid == 0) s_nruns = 0;
     // coalesced loads; padding (key ~0, position >= m) sorts last
     for (uint32_t i = tid; i < Pm; i += T) {
         s_k[i] = i < m ? K[beg + i] : ~0ull;
@@ -766,10 +780,10 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
             finish_position(B, beg + i, s_id[s_p[i]], kk, i > 0, pk, !eqp && !eqn, depth);
             if (i == 0 || i == m - 1) B.bkey[beg + i] = kk;
         }
-        // a run of equal keys without a NUL continues at depth + 8: a run of
-        // at most kTinyT strings as a tiny leaf of this round (its keys at
-        // depth + 8 are in place, finish_position fetched them), a longer one
-        // as a segment of the next round
+        // a run of equal keys without a NUL continues at depth + 8 (its keys
+        // there are in place, finish_position fetched them): a run of at most
+        // kTinyT strings is finished below by this workgroup, a longer one is
+        // a segment of the next round
         const bool nr = act && !eqp && eqn && (kk & 0xFFull) != 0;
         if (__ballot(nr) == 0) continue;     // (uniform) nothing to append in this wave
         uint32_t rl = 0;
@@ -778,10 +792,27 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
             while (i + rl < m && s_k[i + rl] == kk) ++rl;
         }
         const bool small = nr && rl <= kTinyT;
-        const uint32_t slot = wave_append(small, &cur.cnt[cur.c_tiny]);
-        if (small) put_item(cur.tiny, cur.cap_tiny, slot, Item{ beg + i, rl, beg + i, 1u << kFlagShift }, cur.cnt);
+        const uint32_t slot = small ? atomicAdd(&s_nruns, 1u) : kMaxRuns;
+        const bool keep = slot < kMaxRuns;
+        if (keep) s_runs[slot] = (i << 16) | rl;
         Item ni = { beg + i, rl, beg + i, 0u };
-        wave_append_segment(nr && !small, ni, nxt);
+        wave_append_segment(nr && !keep, ni, nxt);
+    }
+    __syncthreads();     // the runs' keys and indices written above are visible to the whole group
+    const uint32_t nruns = min(s_nruns, kMaxRuns), w = wave_id(), lane = lane_id();
+    // the sorted block is no longer needed: each wave's leaf arrays live in s_k and s_id
+    key64* wk = s_k + w * 64;
+    uint32_t* wi = s_id + w * 128;
+    uint32_t* wr = wi + 64;
+    for (uint32_t r = w; r < nruns; r += T / 64) {
+        const uint32_t rs = s_runs[r] >> 16, rl = s_runs[r] & 0xFFFFu;
+        key64 kk = 0;
+        uint32_t id = 0;
+        if (lane < rl) {
+            kk = B.key0[beg + rs + lane];
+            id = B.idx0[beg + rs + lane];
+        }
+        wave_finish_leaf(B, beg + rs, rl, kk, id, depth + 8, false, wk, wi, wr);
     }
 }
 
@@ -791,7 +822,7 @@ __global__ __launch_bounds__(kWG) void k_leaf_run(const Item* items, const uint3
 {
     if (blockIdx.x >= *cnt_items) return;
     const Item L = items[blockIdx.x];
-    const uint32_t c = L.flags >> 8;
+    const uint32_t c = L.flags >> kFlagShift;
     if (!item_ok(L, B) || (uint64_t)c * kRunChunk >= L.size) {
         if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
@@ -918,7 +949,7 @@ __global__ __launch_bounds__(kWG) void k_sum_sizes(const Item* items, uint32_t f
     for (uint32_t i = first + blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
         const Item it = items[i];
         if (it.flags & F_SORTED) {   // a chunk of an equal-key leaf
-            const uint32_t c = it.flags >> 8, cb = c * kRunChunk;
+            const uint32_t c = it.flags >> kFlagShift, cb = c * kRunChunk;
             v += cb < it.size ? min(kRunChunk, it.size - cb) : 0u;
         } else {
             v += it.size;
@@ -1442,13 +1473,13 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                     Launch L(c, K_BLOCK, 0);
                     if (cls == 0)
                         hipLaunchKernelGGL((k_block<256, 4>), dim3(nblk[cls]), dim3(256 / 4), 0, st, list,
-                                           blk_done[cls], cnt, B, nsink, sink, depth);
+                                           blk_done[cls], cnt, B, nsink, depth);
                     else if (cls == 1)
                         hipLaunchKernelGGL((k_block<512, 8>), dim3(nblk[cls]), dim3(512 / 8), 0, st, list,
-                                           blk_done[cls], cnt, B, nsink, sink, depth);
+                                           blk_done[cls], cnt, B, nsink, depth);
                     else
                         hipLaunchKernelGGL((k_block<kBlkT, 8>), dim3(nblk[cls]), dim3(kBlkT / 8), 0, st, list,
-                                           blk_done[cls], cnt, B, nsink, sink, depth);
+                                           blk_done[cls], cnt, B, nsink, depth);
                 }
                 sum_sizes(c, list, blk_done[cls], cnt, blk_done[cls] + nblk[cls], S_BLOCK);
                 blk_done[cls] += nblk[cls];
@@ -1471,7 +1502,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
             {
                 Launch L(c, K_LEAF_TINY, 0);
                 hipLaunchKernelGGL(k_leaf_tiny, dim3(div_up(nt, kWaves)), dim3(kWG), 0, st, c->l_tiny[rp],
-                                   c->d_cnt + C_TINY + rp, B, nsink, depth);
+                                   c->d_cnt + C_TINY + rp, B, depth);
             }
             sum_sizes(c, c->l_tiny[rp], 0, c->d_cnt + C_TINY + rp, nt, S_TINY);
         }
