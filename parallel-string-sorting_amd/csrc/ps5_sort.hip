@@ -328,6 +328,29 @@ __global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* desc, const
     const uint32_t cb = t * kTile + w * kWaveChunk;
     if (!item_ok(seg, B) || t >= D.T || cb >= seg.size) return;
     const uint32_t m = min(seg.size - cb, (uint32_t)kWaveChunk);
+    const int src = seg.flags & F_BUFB;
+    const key64* K = src ? B.key1 : B.key0;
+    const uint32_t* I = src ? B.idx1 : B.idx0;
+    key64* KD = src ? B.key0 : B.key1;
+    uint32_t* ID = src ? B.idx0 : B.idx1;
+    // the chunk's loads go out first: every thread issues its kPer elements'
+    // loads, which then overlap the histogram prologue below
+    constexpr int kPer = kWaveChunk / kBigWG;
+    static_assert(kWaveChunk % kBigWG == 0, "a chunk is a whole number of workgroup passes");
+    const bool ident = seg.flags & F_IDENT;
+    uint32_t br[kPer], iv[kPer];
+    key64 kv[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const uint32_t p = e * kBigWG + threadIdx.x;
+        const uint32_t q = seg.begin + cb + p;
+        br[e] = 0xFFFFFFFFu;
+        if (p < m) {
+            br[e] = B.bktrank[q];
+            kv[e] = K[q];
+            iv[e] = ident ? q : I[q];
+        }
+    }
     const uint32_t* h = hscan + D.hist_base;
     const uint32_t segbase = h[0];
     // counts and global bases of this chunk per bucket; entry e's count is
@@ -360,28 +383,6 @@ __global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* desc, const
     if (tot0 + tot1 != m) {   // the histogram must account for every key of the chunk
         if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
-    }
-    const int src = seg.flags & F_BUFB;
-    const key64* K = src ? B.key1 : B.key0;
-    const uint32_t* I = src ? B.idx1 : B.idx0;
-    key64* KD = src ? B.key0 : B.key1;
-    uint32_t* ID = src ? B.idx0 : B.idx1;
-    // every load of the thread's kPer elements is issued before the first is used
-    constexpr int kPer = kWaveChunk / kBigWG;
-    static_assert(kWaveChunk % kBigWG == 0, "a chunk is a whole number of workgroup passes");
-    const bool ident = seg.flags & F_IDENT;
-    uint32_t br[kPer], iv[kPer];
-    key64 kv[kPer];
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-        const uint32_t p = e * kBigWG + threadIdx.x;
-        const uint32_t q = seg.begin + cb + p;
-        br[e] = 0xFFFFFFFFu;
-        if (p < m) {
-            br[e] = B.bktrank[q];
-            kv[e] = K[q];
-            iv[e] = ident ? q : I[q];
-        }
     }
 #pragma unroll
     for (int e = 0; e < kPer; ++e) {
