@@ -195,19 +195,37 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     // kGroups 64-key groups per iteration: all loads are in flight before the
     // tree descents, then the groups are ranked in order (stable)
     constexpr int kGroups = 4;
-    for (uint32_t g = cb; g < ce; g += 64 * kGroups) {
-        key64 kv[kGroups];
+    // extracting at the root, the string offsets of the next iteration are
+    // loaded one iteration ahead (the first link of offset -> characters)
+    uint64_t off_next[kGroups];
+    if (from_arena) {
 #pragma unroll
         for (int j = 0; j < kGroups; ++j) {
-            const uint32_t p = g + 64 * j + lane;
-            kv[j] = 0;
-            if (p < ce) {
-                if (from_arena) {
-                    kv[j] = load_key(B.arena, B.alen, str_off(B, seg.begin + p));
-                    B.key0[seg.begin + p] = kv[j];
-                } else {
-                    kv[j] = K[seg.begin + p];
-                }
+            const uint32_t p = cb + 64 * j + lane;
+            off_next[j] = p < ce ? str_off(B, seg.begin + p) : 0ull;
+        }
+    }
+    for (uint32_t g = cb; g < ce; g += 64 * kGroups) {
+        key64 kv[kGroups];
+        if (from_arena) {
+            uint64_t off[kGroups];
+#pragma unroll
+            for (int j = 0; j < kGroups; ++j) off[j] = off_next[j];
+#pragma unroll
+            for (int j = 0; j < kGroups; ++j) {
+                const uint32_t p = g + 64 * kGroups + 64 * j + lane;
+                off_next[j] = p < ce ? str_off(B, seg.begin + p) : 0ull;
+            }
+#pragma unroll
+            for (int j = 0; j < kGroups; ++j) {
+                const uint32_t p = g + 64 * j + lane;
+                kv[j] = p < ce ? load_key(B.arena, B.alen, off[j]) : 0ull;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kGroups; ++j) {
+                const uint32_t p = g + 64 * j + lane;
+                kv[j] = p < ce ? K[seg.begin + p] : 0ull;
             }
         }
         uint32_t bv[kGroups];
@@ -225,6 +243,13 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
                 const uint32_t old = s_hist[bv[j]];
                 if (r == 0) s_hist[bv[j]] = (uint16_t)(old + (uint32_t)__popcll(peers));
                 B.bktrank[seg.begin + p] = (bv[j] << kRankBits) | (old + r);
+            }
+        }
+        if (from_arena) {     // the extracted keys, for the scatter
+#pragma unroll
+            for (int j = 0; j < kGroups; ++j) {
+                const uint32_t p = g + 64 * j + lane;
+                if (p < ce) B.key0[seg.begin + p] = kv[j];
             }
         }
     }
