@@ -115,18 +115,29 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
         if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
-    // the samples, sorted in runs of 64 in the waves' registers (padding ~0 last)
-    for (uint32_t r = wave_id(); r < P / 64; r += kBigWaves) {
-        const uint32_t j = r * 64 + lane_id();
-        key64 v = ~0ull;
+    // every sample load is issued first (kPer per thread), then the samples
+    // are sorted in runs of 64 in the waves' registers (padding ~0 last)
+    constexpr int kPer = kMaxSamples / kBigWG;
+    static_assert(kMaxSamples % kBigWG == 0, "whole sample passes");
+    key64 v[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const uint32_t j = e * kBigWG + threadIdx.x;
+        v[e] = ~0ull;
         if (j < ss) {
             const uint64_t h = mix64(seed ^ ((uint64_t)seg.begin << 21) ^ (uint64_t)j);
             const uint32_t pos = seg.begin + (uint32_t)(((h >> 32) * (uint64_t)seg.size) >> 32);
-            v = from_arena ? load_key(B.arena, B.alen, str_off(B, pos)) : K[pos];
+            v[e] = from_arena ? load_key(B.arena, B.alen, str_off(B, pos)) : K[pos];
         }
+    }
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const uint32_t j = e * kBigWG + threadIdx.x;    // run j / 64 belongs to this wave
+        if (e * (uint32_t)kBigWG >= P) break;           // (uniform) P is a multiple of 64
+        if (j >= P) continue;                           // (wave-uniform)
         uint32_t p = j;
-        wave_sort64(v, p);
-        s_samp[0][j] = v;
+        wave_sort64(v[e], p);
+        s_samp[0][j] = v[e];
     }
     __syncthreads();
     // pairwise merges by rank (equal keys of the left run first)
