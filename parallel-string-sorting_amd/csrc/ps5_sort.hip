@@ -723,11 +723,10 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
     static_assert(T % 64 == 0 && MAXM <= 65536, "whole waves; positions fit 16 bits");
     __shared__ key64 s_k[MAXM];
     __shared__ uint16_t s_p[MAXM];
-    __shared__ uint32_t s_id[MAXM];
     constexpr uint32_t kMaxRuns = 64;            // short equal runs finished here (more go to the next round)
     __shared__ uint32_t s_runs[kMaxRuns];        // (start << 16) | length
     __shared__ uint32_t s_nruns;
-    static_assert(MAXM >= 128 * (T / 64), "the per-wave leaf arrays reuse s_k and s_id");
+    static_assert(MAXM * 8 >= 1024 * (T / 64), "the per-wave leaf arrays (1 KB each) reuse s_k");
     const uint32_t idx = first + blockIdx.x;
     if (idx >= *cnt_items) return;
     const Item L = items[idx];
@@ -745,7 +744,6 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
     // coalesced loads; padding (key ~0, position >= m) sorts last
     for (uint32_t i = tid; i < Pm; i += T) {
         s_k[i] = i < m ? K[beg + i] : ~0ull;
-        if (i < m) s_id[i] = ident ? beg + i : I[beg + i];
     }
     __syncthreads();
     key64 k[E];
@@ -805,7 +803,21 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
         }
         __syncthreads();
     }
-    for (uint32_t i = tid; i < ((m + T - 1) / T) * T; i += T) {
+    // the input index of every sorted position, read where it lies before
+    // finish_position may overwrite those slots (continuing runs store theirs)
+    constexpr int kPerThread = MAXM / T;
+    uint32_t ids[kPerThread];
+#pragma unroll
+    for (int e = 0; e < kPerThread; ++e) {
+        const uint32_t i = tid + (uint32_t)e * T;
+        ids[e] = 0;
+        if (i < m) ids[e] = ident ? beg + s_p[i] : I[beg + s_p[i]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < kPerThread; ++e) {
+        const uint32_t i = tid + (uint32_t)e * T;
+        if ((uint32_t)e * T >= m) break;     // (uniform) no position of this pass is in the block
         const bool act = i < m;     // sorted position
         bool eqp = false, eqn = false;
         key64 kk = 0, pk = 0;
@@ -814,7 +826,7 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
             if (i > 0) pk = s_k[i - 1];
             eqp = i > 0 && pk == kk;
             eqn = i + 1 < m && s_k[i + 1] == kk;
-            finish_position(B, beg + i, s_id[s_p[i]], kk, i > 0, pk, !eqp && !eqn, depth);
+            finish_position(B, beg + i, ids[e], kk, i > 0, pk, !eqp && !eqn, depth);
             if (i == 0 || i == m - 1) B.bkey[beg + i] = kk;
         }
         // a run of equal keys without a NUL continues at depth + 8 (its keys
@@ -837,9 +849,9 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
     }
     __syncthreads();     // the runs' keys and indices written above are visible to the whole group
     const uint32_t nruns = min(s_nruns, kMaxRuns), w = wave_id(), lane = lane_id();
-    // the sorted block is no longer needed: each wave's leaf arrays live in s_k and s_id
-    key64* wk = s_k + w * 64;
-    uint32_t* wi = s_id + w * 128;
+    // the sorted block is no longer needed: each wave's leaf arrays live in s_k
+    key64* wk = s_k + w * 128;
+    uint32_t* wi = reinterpret_cast<uint32_t*>(wk + 64);
     uint32_t* wr = wi + 64;
     for (uint32_t r = w; r < nruns; r += T / 64) {
         const uint32_t rs = s_runs[r] >> 16, rl = s_runs[r] & 0xFFFFu;
