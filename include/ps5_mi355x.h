@@ -68,6 +68,16 @@ void mi355x_ps5_sort_out(unsigned char** strings, unsigned char** output, size_t
  * abort()s on failure. */
 void mi355x_ps5_sort_lcp_cache(unsigned char** strings, uintptr_t* lcp, unsigned char* cache, size_t n);
 
+/* Host form of the merge, in place of the reference's
+ * parallelLcpMerge(const LcpCacheStringPtr* input, unsigned numInputs,
+ * string* output, size_t length) (eberle-parallel-lcp-merge-lcp-splitting.hpp:
+ * 443-469): runs[r][0 .. sizes[r]) are sorted strings with LCP arrays lcps[r]
+ * (lcps[r][0] ignored); the merged strings go to output and, if out_lcp is
+ * not NULL, their LCPs to out_lcp[1..] (out_lcp[0] untouched).  Returns a
+ * status code. */
+int ps5_lcp_merge_strings(unsigned char* const* const* runs, const uintptr_t* const* lcps, const size_t* sizes,
+                          unsigned nruns, unsigned char** output, uintptr_t* out_lcp);
+
 /* Status-returning form of the three above: lcp and output may be NULL. */
 int ps5_sort_strings(unsigned char** strings, size_t n, uintptr_t* lcp,
                      unsigned char** output);
@@ -108,6 +118,25 @@ int ps5_sort_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
 int ps5_dist_chars_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
                           const uint32_t* d_offs, uint64_t n,
                           const uint32_t* d_perm, const uint32_t* d_lcp, uint8_t* d_cache);
+
+/* Merge nruns sorted runs into one sorted sequence with its LCP array, on
+ * the device: the binary LCP merge of bingmann-lcp_mergesort_binary.hpp
+ * (lcp_merge_binary / lcp_compare), applied pairwise in rounds with merge
+ * path partitioning (the reference merges up to 8 streams at a time with an
+ * LCP loser tree, eberle-parallel-lcp-merge*.hpp, for its NUMA sort,
+ * eberle-ps5-parallel-toplevel-merge.hpp:55-176).
+ *   d_arena, arena_bytes, d_offs: the strings, as for ps5_sort_device;
+ *   run_first: host array of nruns+1 positions, run_first[0] = 0; run r is
+ *     d_idx[run_first[r] .. run_first[r+1]), string indices in sorted order,
+ *     with d_lcp[q] = LCP of the strings at q-1 and q (the first of a run
+ *     unused) -- exactly what ps5_sort_device returns for a subset;
+ *   d_idx_out, d_lcp_out: the merged order and its LCPs (d_lcp_out[0] unused).
+ * Equal strings keep run order, so runs that are sorted index ranges in
+ * ascending order merge to the same result as sorting everything at once. */
+int ps5_merge_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
+                     const uint32_t* d_offs, const uint64_t* run_first, uint32_t nruns,
+                     const uint32_t* d_idx, const uint32_t* d_lcp,
+                     uint32_t* d_idx_out, uint32_t* d_lcp_out);
 
 /* Batch: sort nsets independent string sets in one pass over the device.
  * Set s is the strings with input indices [set_first[s], set_first[s+1])

@@ -38,6 +38,9 @@ struct Staging {
     uint32_t* d_lcp = nullptr;
     uint8_t* d_cache = nullptr;
     uint64_t n_cap = 0;
+    uint32_t* d_out_idx = nullptr;   // merge outputs
+    uint32_t* d_out_lcp = nullptr;
+    uint64_t out_cap = 0;
 } g_st;
 
 unsigned host_threads()
@@ -203,4 +206,80 @@ extern "C" void mi355x_ps5_sort_out(unsigned char** strings, unsigned char** out
 extern "C" void mi355x_ps5_sort_lcp_cache(unsigned char** strings, uintptr_t* lcp, unsigned char* cache, size_t n)
 {
     die_on(ps5_sort_strings_cache(strings, n, lcp, cache, nullptr), "mi355x_ps5_sort_lcp_cache");
+}
+
+// ---- merge of sorted runs (host form) ---------------------------------------
+
+extern "C" int ps5_lcp_merge_strings(unsigned char* const* const* runs, const uintptr_t* const* lcps,
+                                     const size_t* sizes, unsigned nruns, unsigned char** output,
+                                     uintptr_t* out_lcp)
+{
+    if (nruns == 0) return PS5_OK;
+    if (!runs || !lcps || !sizes || !output) return PS5_ERR_ARG;
+    std::vector<uint64_t> run_first(nruns + 1, 0);
+    for (unsigned r = 0; r < nruns; ++r) {
+        if (sizes[r] && (!runs[r] || !lcps[r])) return PS5_ERR_ARG;
+        run_first[r + 1] = run_first[r] + sizes[r];
+    }
+    const uint64_t n = run_first[nruns];
+    if (n == 0) return PS5_OK;
+    if (n >= 0xFFFFFF00ull) return PS5_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_mu);
+    int rc;
+    if (!g_ctx && (rc = ps5_ctx_create(&g_ctx, 0))) return rc;
+    // the strings of all runs, in run order, packed into one arena: string q
+    // (q-th of the concatenation) gets index q
+    std::vector<const unsigned char*> str(n);
+    for (unsigned r = 0; r < nruns; ++r)
+        for (uint64_t i = 0; i < sizes[r]; ++i) str[run_first[r] + i] = runs[r][i];
+    std::vector<uint64_t> len(n);
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) len[i] = strlen((const char*)str[i]);
+    });
+    std::vector<uint32_t> offs(n), idx(n), lcp(n, 0);
+    uint64_t tot = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (tot > 0xFFFFFFF0ull) return PS5_ERR_ARG;
+        offs[i] = (uint32_t)tot;
+        idx[i] = (uint32_t)i;
+        tot += len[i] + 1;
+    }
+    if (tot > 0xFFFFFFF0ull) return PS5_ERR_ARG;
+    for (unsigned r = 0; r < nruns; ++r)
+        for (uint64_t i = 1; i < sizes[r]; ++i) {
+            if (lcps[r][i] > 0xFFFFFFFFull) return PS5_ERR_ARG;
+            lcp[run_first[r] + i] = (uint32_t)lcps[r][i];
+        }
+    std::vector<uint8_t> packed(tot);
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) memcpy(&packed[offs[i]], str[i], len[i] + 1);
+    });
+    if ((rc = ensure_staging(tot, n))) return rc;
+    if (n > g_st.out_cap) {
+        for (uint32_t** q : { &g_st.d_out_idx, &g_st.d_out_lcp }) {
+            if (*q) (void)hipFree(*q);
+            *q = nullptr;
+        }
+        g_st.out_cap = 0;
+        for (uint32_t** q : { &g_st.d_out_idx, &g_st.d_out_lcp }) {
+            void* p = nullptr;
+            if (ps5_dev_malloc(&p, n * sizeof(uint32_t))) return PS5_ERR_NOMEM;
+            *q = (uint32_t*)p;
+        }
+        g_st.out_cap = n;
+    }
+    if ((rc = ps5_memcpy_h2d(g_st.d_arena, packed.data(), tot))) return rc;
+    if ((rc = ps5_memcpy_h2d(g_st.d_offs, offs.data(), n * 4))) return rc;
+    if ((rc = ps5_memcpy_h2d(g_st.d_perm, idx.data(), n * 4))) return rc;
+    if ((rc = ps5_memcpy_h2d(g_st.d_lcp, lcp.data(), n * 4))) return rc;
+    if ((rc = ps5_merge_device(g_ctx, g_st.d_arena, tot, g_st.d_offs, run_first.data(), nruns, g_st.d_perm,
+                               g_st.d_lcp, g_st.d_out_idx, g_st.d_out_lcp)))
+        return rc;
+    if ((rc = ps5_memcpy_d2h(idx.data(), g_st.d_out_idx, n * 4))) return rc;
+    for (uint64_t i = 0; i < n; ++i) output[i] = const_cast<unsigned char*>(str[idx[i]]);
+    if (out_lcp) {
+        if ((rc = ps5_memcpy_d2h(lcp.data(), g_st.d_out_lcp, n * 4))) return rc;
+        for (uint64_t i = 1; i < n; ++i) out_lcp[i] = lcp[i];
+    }
+    return PS5_OK;
 }

@@ -981,6 +981,113 @@ __global__ __launch_bounds__(kWG) void k_widen(const uint32_t* offs32, const uin
     offs64[i] = base[lo] + offs32[i];
 }
 
+// ---- LCP-aware merge of sorted runs ---------------------------------------
+
+// Compare strings a and b (arena offsets) that share their first h
+// characters: returns true if a <= b, and their LCP in *lcp.
+__device__ __forceinline__ bool str_leq_from(const uint8_t* arena, uint64_t alen, uint64_t a, uint64_t b,
+                                             uint32_t h, uint32_t* lcp)
+{
+    for (;;) {
+        const key64 ka = load_key(arena, alen, a + h), kb = load_key(arena, alen, b + h);
+        if (ka != kb) {
+            *lcp = h + lcp_key(ka, kb);
+            return ka < kb;
+        }
+        if ((ka & 0xFFull) == 0) {   // both end inside this key: equal strings
+            *lcp = h + key_depth(ka);
+            return true;
+        }
+        h += 8;
+    }
+}
+
+// One round of pairwise merges.  Pair p merges runs A = [first[p], mid[p])
+// and B = [mid[p], first[p+1]) of (idx, lcp) into the same positions of the
+// output; lcp[q] is the LCP of the strings at q-1 and q within a run (its
+// first entry unused).  Thread t of pair p produces outputs
+// [first[p] + t*E, +E): a co-rank search places them in A and B (string
+// comparisons; equal strings take A first, so ties keep run order), then the
+// binary LCP merge (lcp_merge_binary and lcp_compare,
+// bingmann-lcp_mergesort_binary.hpp) emits them, comparing characters only
+// where the LCPs to the previous output do not decide.  tstart[p] is the
+// first global thread of pair p.
+constexpr uint32_t kMergeE = 8;
+__global__ __launch_bounds__(kWG) void k_merge_round(Bufs B, const uint32_t* src_idx, const uint32_t* src_lcp,
+                                                     uint32_t* dst_idx, uint32_t* dst_lcp, const uint64_t* first,
+                                                     const uint64_t* mid, const uint64_t* tstart, uint32_t npairs)
+{
+    const uint64_t gt = (uint64_t)blockIdx.x * kWG + threadIdx.x;
+    if (gt >= tstart[npairs]) return;
+    uint32_t lo = 0, hi = npairs;   // the pair holding thread gt: largest p with tstart[p] <= gt
+    while (lo + 1 < hi) {
+        const uint32_t m = (lo + hi) / 2;
+        if (tstart[m] <= gt) lo = m;
+        else hi = m;
+    }
+    const uint32_t p = lo;
+    const uint64_t a0 = first[p], b0 = mid[p], e0 = first[p + 1];
+    const uint64_t la = b0 - a0, lb = e0 - b0;
+    const uint64_t diag = (gt - tstart[p]) * kMergeE;
+    if (diag >= la + lb) return;
+    const uint32_t* IA = src_idx + a0;
+    const uint32_t* IB = src_idx + b0;
+    auto off = [&](uint32_t id) { return str_off(B, id); };
+    // co-rank: i strings of A and diag - i of B precede this thread's outputs
+    uint64_t i0 = diag > lb ? diag - lb : 0, i1 = min(diag, la);
+    while (i0 < i1) {
+        const uint64_t mi = (i0 + i1) / 2;
+        uint32_t l;
+        if (str_leq_from(B.arena, B.alen, off(IA[mi]), off(IB[diag - 1 - mi]), 0, &l)) i0 = mi + 1;
+        else i1 = mi;
+    }
+    uint64_t i = i0, j = diag - i0;
+    // LCPs of the next A and B string with the previous output
+    uint32_t lcpa = 0, lcpb = 0;
+    if (diag > 0) {
+        bool prev_is_a;
+        if (i == 0) prev_is_a = false;
+        else if (j == 0) prev_is_a = true;
+        else {
+            uint32_t l;
+            prev_is_a = !str_leq_from(B.arena, B.alen, off(IA[i - 1]), off(IB[j - 1]), 0, &l);
+        }
+        const uint64_t prev = prev_is_a ? off(IA[i - 1]) : off(IB[j - 1]);
+        if (i < la) {
+            if (prev_is_a) lcpa = src_lcp[a0 + i];
+            else str_leq_from(B.arena, B.alen, prev, off(IA[i]), 0, &lcpa);
+        }
+        if (j < lb) {
+            if (!prev_is_a) lcpb = src_lcp[b0 + j];
+            else str_leq_from(B.arena, B.alen, prev, off(IB[j]), 0, &lcpb);
+        }
+    }
+    const uint64_t nout = min((uint64_t)kMergeE, la + lb - diag);
+    for (uint64_t q = 0; q < nout; ++q) {
+        const uint64_t o = a0 + diag + q;
+        bool take_a;
+        uint32_t lab = 0;     // LCP of the two heads
+        if (j >= lb) take_a = true;
+        else if (i >= la) take_a = false;
+        else if (lcpa == lcpb) take_a = str_leq_from(B.arena, B.alen, off(IA[i]), off(IB[j]), lcpa, &lab);
+        else if (lcpa > lcpb) { take_a = true; lab = lcpb; }
+        else { take_a = false; lab = lcpa; }
+        if (take_a) {
+            dst_idx[o] = IA[i];
+            dst_lcp[o] = lcpa;
+            ++i;
+            lcpa = i < la ? src_lcp[a0 + i] : 0u;
+            lcpb = lab;
+        } else {
+            dst_idx[o] = IB[j];
+            dst_lcp[o] = lcpb;
+            ++j;
+            lcpb = j < lb ? src_lcp[b0 + j] : 0u;
+            lcpa = lab;
+        }
+    }
+}
+
 // plain copy of 16-byte words (calibrates the HBM byte counters)
 __global__ __launch_bounds__(kWG) void k_copy16(const uint4* src, uint4* dst, uint64_t n16)
 {
@@ -1088,6 +1195,8 @@ struct ps5_ctx {
     uint64_t hist_cap = 0;
     uint32_t* part = nullptr;
     uint64_t part_cap = 0;
+    uint64_t* d_merge = nullptr;     // plan of a merge round
+    uint64_t merge_cap = 0;
     uint64_t* d_sets = nullptr;      // set boundaries of a batch
     uint64_t sets_cap = 0;
     unsigned long long* d_sums = nullptr;   // profiling: elements per kernel class
@@ -1644,6 +1753,7 @@ void ps5_ctx_destroy(ps5_ctx* c)
     if (c->d_cnt) (void)hipFree(c->d_cnt);
     if (c->d_sums) (void)hipFree(c->d_sums);
     if (c->d_sets) (void)hipFree(c->d_sets);
+    if (c->d_merge) (void)hipFree(c->d_merge);
     if (c->h_cnt) (void)hipHostFree(c->h_cnt);
     if (c->h_upcnt) (void)hipHostFree(c->h_upcnt);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -1682,6 +1792,86 @@ int ps5_sort_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, co
         if (c->profile) collect_events(c);
     }
     return rc;
+}
+
+int ps5_merge_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, const uint32_t* d_offs,
+                     const uint64_t* run_first, uint32_t nruns, const uint32_t* d_idx, const uint32_t* d_lcp,
+                     uint32_t* d_idx_out, uint32_t* d_lcp_out)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
+    if (nruns == 0 || !run_first) return set_err(PS5_ERR_ARG, "no runs");
+    const uint64_t n = run_first[nruns];
+    if (run_first[0] != 0) return set_err(PS5_ERR_ARG, "run_first[0] must be 0");
+    for (uint32_t r = 0; r < nruns; ++r)
+        if (run_first[r + 1] < run_first[r]) return set_err(PS5_ERR_ARG, "run boundaries must be non-decreasing");
+    if (n == 0) return PS5_OK;
+    if (!d_arena || !d_offs || !d_idx || !d_lcp || !d_idx_out || !d_lcp_out)
+        return set_err(PS5_ERR_ARG, "NULL device pointer");
+    if (n >= 0xFFFFFF00ull) return set_err(PS5_ERR_ARG, "n too large (limit 2^32-256 strings)");
+    if (arena_bytes == 0 || arena_bytes > 0xFFFFFFFFull) return set_err(PS5_ERR_ARG, "arena_bytes must be in [1, 2^32)");
+    if (((uintptr_t)d_arena & 7u) != 0) return set_err(PS5_ERR_ARG, "arena must be 8-byte aligned");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    if (nruns == 1) {
+        HIPCHK(hipMemcpyAsync(d_idx_out, d_idx, n * 4, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(d_lcp_out, d_lcp, n * 4, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return PS5_OK;
+    }
+    int rc = ensure_capacity(c, n);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(c->d_cnt, 0, C_NCNT * sizeof(uint32_t), st));
+    Bufs B{};
+    B.n = (uint32_t)n;
+    B.cnt = c->d_cnt;
+    B.arena = d_arena;
+    B.alen = arena_bytes;
+    B.offs = d_offs;
+    // ping-pong buffers: indices in idx[0]/idx[1], LCPs in key[0]/key[1]
+    uint32_t* ti[2] = { c->idx[0], c->idx[1] };
+    uint32_t* tl[2] = { reinterpret_cast<uint32_t*>(c->key[0]), reinterpret_cast<uint32_t*>(c->key[1]) };
+    std::vector<uint64_t> bounds(run_first, run_first + nruns + 1);
+    const uint32_t* si = d_idx;
+    const uint32_t* sl = d_lcp;
+    int cur = 0;
+    while (bounds.size() > 2) {
+        const uint32_t nr = (uint32_t)bounds.size() - 1, np = (nr + 1) / 2;
+        std::vector<uint64_t> plan(3 * (np + 1), 0);   // first[np+1], mid[np], tstart[np+1]
+        uint64_t* f = plan.data();
+        uint64_t* mdl = f + (np + 1);
+        uint64_t* ts = mdl + (np + 1);
+        for (uint32_t p = 0; p < np; ++p) {
+            f[p] = bounds[2 * p];
+            mdl[p] = bounds[2 * p + 1];
+            const uint64_t e = 2 * p + 2 <= nr ? bounds[2 * p + 2] : bounds[2 * p + 1];
+            ts[p + 1] = ts[p] + (e - f[p] + kMergeE - 1) / kMergeE;
+        }
+        f[np] = bounds[nr];
+        const bool last = np == 1;
+        uint32_t* di = last ? d_idx_out : ti[cur];
+        uint32_t* dl = last ? d_lcp_out : tl[cur];
+        if (plan.size() > c->merge_cap) {
+            if (c->d_merge) (void)hipFree(c->d_merge);
+            c->d_merge = nullptr;
+            c->merge_cap = 0;
+            HIPCHK(hipMalloc(&c->d_merge, plan.size() * sizeof(uint64_t)));
+            c->merge_cap = plan.size();
+        }
+        HIPCHK(hipMemcpyAsync(c->d_merge, plan.data(), plan.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+        if (ts[np]) {
+            hipLaunchKernelGGL(k_merge_round, dim3(div_up(ts[np], kWG)), dim3(kWG), 0, st, B, si, sl, di, dl,
+                               c->d_merge, c->d_merge + (np + 1), c->d_merge + 2 * (np + 1), np);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(st));     // the plan buffer is rewritten next round
+        std::vector<uint64_t> nb(np + 1);
+        for (uint32_t p = 0; p <= np; ++p) nb[p] = f[p];
+        bounds.swap(nb);
+        si = di;
+        sl = dl;
+        cur ^= 1;
+    }
+    return readback(c, nullptr, 0);
 }
 
 int ps5_dist_chars_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, const uint32_t* d_offs,

@@ -30,6 +30,7 @@ EXPORTS = [
     "ps5_sort_batch_device", "ps5_gen_batch_plan", "ps5_gen_batch_fill",
     "ps5_server_create", "ps5_server_destroy", "ps5_server_submit", "ps5_server_wait", "ps5_server_stats",
     "ps5_ctx_set_tuning", "ps5_ctx_get_tuning", "ps5_autotune_device", "ps5_autotune_batch_device",
+    "ps5_merge_device", "ps5_lcp_merge_strings",
 ]
 
 
@@ -105,6 +106,8 @@ def load():
          + [ctypes.POINTER(ctypes.c_double)])
     _sig(lib, "ps5_autotune_batch_device", _I, [_P, _P, _U64, _P, _P, _U32, _P, _P] + [ctypes.POINTER(_U32)] * 3
          + [ctypes.POINTER(ctypes.c_double)])
+    _sig(lib, "ps5_merge_device", _I, [_P, _P, _U64, _P, _P, _U32, _P, _P, _P, _P])
+    _sig(lib, "ps5_lcp_merge_strings", _I, [_P, _P, _P, ctypes.c_uint, _P, _P])
     _lib = lib
     return lib
 
@@ -312,6 +315,27 @@ class Context:
             for b in bufs:
                 b.free()
 
+    def merge_device(self, d_arena, arena_bytes, d_offs, run_first, d_idx, d_lcp, d_idx_out, d_lcp_out):
+        rf = np.ascontiguousarray(run_first, np.uint64)
+        _check(load().ps5_merge_device(self.h, d_arena.ptr, arena_bytes, d_offs.ptr, _ptr(rf), len(rf) - 1,
+                                       d_idx.ptr, d_lcp.ptr, d_idx_out.ptr, d_lcp_out.ptr), "ps5_merge_device")
+
+    def merge(self, arena, offs, run_first, idx, lcp):
+        """Merge sorted runs (idx/lcp arrays, run r at [run_first[r], run_first[r+1])): (idx, lcp)."""
+        n = int(run_first[-1])
+        bufs = [DeviceBuffer(arena.size)] + [DeviceBuffer(max(4, 4 * n)) for _ in range(5)]
+        da, do, di, dl, dio, dlo = bufs
+        try:
+            da.upload(arena)
+            do.upload(np.ascontiguousarray(offs, np.uint32))
+            di.upload(np.ascontiguousarray(idx, np.uint32))
+            dl.upload(np.ascontiguousarray(lcp, np.uint32))
+            self.merge_device(da, int(arena.size), do, run_first, di, dl, dio, dlo)
+            return dio.download(np.uint32, n), dlo.download(np.uint32, n)
+        finally:
+            for b in bufs:
+                b.free()
+
     def sort_batch_device(self, d_arena, arena_bytes, d_offs64, set_first, d_perm, d_lcp=None):
         sf = np.ascontiguousarray(set_first, np.uint64)
         _check(load().ps5_sort_batch_device(self.h, d_arena.ptr, arena_bytes, d_offs64.ptr, _ptr(sf), len(sf) - 1,
@@ -465,3 +489,26 @@ def sort_suffixes(text, with_lcp=True):
     _check(lib.ps5_sort_strings(arr, n, _ptr(lcp) if with_lcp else None, None), "ps5_sort_strings")
     order = np.array([arr[i] - base for i in range(n)], np.int64)
     return order, (lcp[:n] if with_lcp else None)
+
+
+def lcp_merge_strings(runs, lcps):
+    """Host-form merge (ps5_lcp_merge_strings) of sorted runs of bytes objects
+    with their LCP lists: returns (merged list, lcp array)."""
+    lib = load()
+    nr = len(runs)
+    bufs = [[ctypes.create_string_buffer(x, len(x) + 1) for x in run] for run in runs]
+    arrs = [(ctypes.c_void_p * max(1, len(r)))(*[ctypes.addressof(b) for b in r]) for r in bufs]
+    lcp_arrs = [np.ascontiguousarray(np.asarray(l if len(l) else [0], np.uint64)) for l in lcps]
+    run_ptrs = (ctypes.c_void_p * max(1, nr))(*[ctypes.addressof(a) for a in arrs])
+    lcp_ptrs = (ctypes.c_void_p * max(1, nr))(*[a.ctypes.data for a in lcp_arrs])
+    sizes = np.array([len(r) for r in runs], np.uint64)
+    n = int(sizes.sum())
+    out = (ctypes.c_void_p * max(1, n))()
+    out_lcp = np.full(max(1, n), 42, np.uint64)
+    _check(lib.ps5_lcp_merge_strings(run_ptrs, lcp_ptrs, _ptr(sizes), nr, out, _ptr(out_lcp)),
+           "ps5_lcp_merge_strings")
+    addr = {}
+    for r, run in enumerate(bufs):
+        for i, b in enumerate(run):
+            addr[ctypes.addressof(b)] = runs[r][i]
+    return [addr[out[i]] for i in range(n)], out_lcp[:n]

@@ -340,3 +340,80 @@ def test_serve_mix_equals_alone(ps5, ctx, oracle, budget_mb, slots, max_batch):
         assert np.array_equal(pa, pm)
         assert np.array_equal(la[1:], lm[1:])
         assert oracle.verify(a, o, pm, lm)[0] == 0
+
+
+# ---- LCP-aware merge of sorted runs ------------------------------------------
+
+def _runs_of(oracle, arena, offs, bounds):
+    """sort each index range with the brute-force oracle: global idx + run-local LCPs"""
+    idx = np.zeros(len(offs), np.uint32)
+    lcp = np.zeros(len(offs), np.uint32)
+    for b, e in zip(bounds[:-1], bounds[1:]):
+        if e > b:
+            p, l = oracle.brute_sort(arena, offs[b:e])
+            idx[b:e] = p + b
+            lcp[b:e] = l
+    return idx, lcp
+
+
+@pytest.mark.parametrize("nruns,letters,hi", [(1, b"ab", 10), (2, b"ab", 12), (3, b"abc", 20), (5, b"ACGT", 40),
+                                               (8, bytes(range(1, 256)), 6), (16, b"ab", 30)])
+def test_merge_runs_equals_full_sort(ps5, ctx, oracle, nruns, letters, hi):
+    """runs that are sorted index ranges merge to the full sort (ties in index order)"""
+    n = 40000
+    strs = random_strings(n, 0, hi, letters, nruns * 7 + hi)
+    arena, offs = pack(strs)
+    rng = np.random.default_rng(nruns)
+    cuts = np.sort(rng.integers(0, n + 1, nruns - 1))
+    bounds = np.concatenate([[0], cuts, [n]]).astype(np.uint64)
+    idx, lcp = _runs_of(oracle, arena, offs, bounds)
+    mi, ml = ctx.merge(arena, offs, bounds, idx, lcp)
+    bp, bl = oracle.brute_sort(arena, offs)
+    assert np.array_equal(mi, bp)
+    assert np.array_equal(ml[1:], bl[1:])
+
+
+def test_merge_deep_prefixes_and_empty_runs(ps5, ctx, oracle):
+    """long shared prefixes (the LCP rules decide most comparisons) and empty runs"""
+    base = b"x" * 200
+    strs = [base[:int(i % 190)] + s for i, s in enumerate(random_strings(20000, 0, 30, b"ab", 3))]
+    arena, offs = pack(strs)
+    n = len(strs)
+    bounds = np.array([0, 0, 5000, 5000, 12000, n, n], np.uint64)
+    idx, lcp = _runs_of(oracle, arena, offs, bounds)
+    mi, ml = ctx.merge(arena, offs, bounds, idx, lcp)
+    bp, bl = oracle.brute_sort(arena, offs)
+    assert np.array_equal(mi, bp)
+    assert np.array_equal(ml[1:], bl[1:])
+
+
+def test_merge_of_device_sorted_shards(ps5, ctx, oracle):
+    """the multi-GPU shape: shards sorted by ps5_sort_device, then merged"""
+    arena, offs = ps5.workload("ascii2gb", scale=1 / 512)
+    n = len(offs)
+    bounds = np.linspace(0, n, 9).astype(np.uint64)
+    idx = np.zeros(n, np.uint32)
+    lcp = np.zeros(n, np.uint32)
+    for b, e in zip(bounds[:-1], bounds[1:]):
+        p, l = ctx.sort(arena, offs[b:e])
+        idx[b:e] = p + b
+        lcp[b:e] = l
+    mi, ml = ctx.merge(arena, offs, bounds, idx, lcp)
+    assert oracle.verify(arena, offs, mi, ml)[0] == 0
+
+
+def test_host_lcp_merge_strings(ps5):
+    runs = []
+    lcps = []
+    for r in range(4):
+        strs = sorted(random_strings(3000, 0, 12, b"abc", 40 + r))
+        l = [0] + [next((h for h in range(min(len(a), len(b))) if a[h] != b[h]), min(len(a), len(b)))
+                   for a, b in zip(strs, strs[1:])]
+        runs.append(strs)
+        lcps.append(l)
+    out, lcp = ps5.lcp_merge_strings(runs, lcps)
+    flat = [s for run in runs for s in run]
+    assert out == sorted(flat)
+    el = [next((h for h in range(min(len(a), len(b))) if a[h] != b[h]), min(len(a), len(b)))
+          for a, b in zip(out, out[1:])]
+    assert lcp[0] == 42 and list(lcp[1:]) == el
