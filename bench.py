@@ -14,6 +14,7 @@ Two further comparisons print their own JSON line when asked for:
   --mode serve   Poisson arrivals served by the continuous-batching server vs a
                  FIFO of plain sorts (log-uniform 10 KB..256 MB requests of
                  prefix-heavy strings, seed 0x5EED0005)
+  --mode merge   LCP merge of --shards sorted index ranges of --workload
 """
 import argparse
 import importlib.util
@@ -153,6 +154,57 @@ def bench_batch(ps5, args):
            "config": {"workload": f"{args.sets} sets, strings log-uniform [{args.count_lo}, {args.count_hi}], "
                                   "length U[4,48], alphabet ACGT/digits/bytes 33..126, seed 0x5EED0004"}}
     print(json.dumps(out), flush=True)
+
+
+def bench_merge(ps5, args):
+    """LCP merge of K sorted shards (contiguous index ranges) of a workload."""
+    import ctypes
+    lib = ps5.load()
+    arena, offs = ps5.workload(args.workload)
+    n = len(offs)
+    K = args.shards
+    bounds = np.linspace(0, n, K + 1).astype(np.uint64)
+    ctx = ps5.Context(0)
+    assert lib.ps5_ctx_reserve(ctx.h, n) == 0
+    d_arena, d_offs = ps5.DeviceBuffer(arena.size), ps5.DeviceBuffer(4 * n)
+    d_idx, d_lcp = ps5.DeviceBuffer(4 * n), ps5.DeviceBuffer(4 * n)
+    d_io, d_lo = ps5.DeviceBuffer(4 * n), ps5.DeviceBuffer(4 * n)
+    d_arena.upload(arena)
+    d_offs.upload(offs)
+    P = ctypes.c_void_p
+    t0 = time.perf_counter()
+    for b, e in zip(bounds[:-1], bounds[1:]):      # each shard sorted on its own (not timed below)
+        b, e = int(b), int(e)
+        ps5._check(lib.ps5_sort_device(ctx.h, d_arena.ptr, int(arena.size), P(d_offs.ptr.value + 4 * b), e - b,
+                                       P(d_idx.ptr.value + 4 * b), P(d_lcp.ptr.value + 4 * b)), "ps5_sort_device")
+    lib.ps5_device_synchronize()
+    shard_ms = (time.perf_counter() - t0) * 1e3
+    idx = d_idx.download(np.uint32, n)
+    for b, e in zip(bounds[:-1], bounds[1:]):      # shard-local indices -> global
+        idx[int(b):int(e)] += np.uint32(int(b))
+    d_idx.upload(idx)
+    for _ in range(args.warmup):
+        ctx.merge_device(d_arena, int(arena.size), d_offs, bounds, d_idx, d_lcp, d_io, d_lo)
+    lib.ps5_device_synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.merge_device(d_arena, int(arena.size), d_offs, bounds, d_idx, d_lcp, d_io, d_lo)
+    lib.ps5_device_synchronize()
+    merge_ms = (time.perf_counter() - t) * 1e3 / args.steps
+    mi, ml = d_io.download(np.uint32, n), d_lo.download(np.uint32, n)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ps5_oracle
+    vr, vmsg = ps5_oracle.verify(arena, offs, mi, ml)
+    chars = int(arena.size) - 16
+    out = {"metric": "LCP merge of sorted shards, merged chars/s (GB/s)", "value": round(chars / merge_ms / 1e6, 3),
+           "unit": "GB/s", "merge_ms": round(merge_ms, 3), "shards": K, "shard_sorts_ms": round(shard_ms, 3),
+           "rounds": int(np.ceil(np.log2(K))) if K > 1 else 0, "verified": vr == 0,
+           "config": {"workload": WORKLOADS[args.workload], "strings": n, "arena_bytes": chars,
+                      "shards": "contiguous index ranges, each sorted by ps5_sort_device"}}
+    if vr != 0:
+        print(f"[bench] MERGE OUTPUT WRONG: {vmsg}", file=sys.stderr)
+    print(json.dumps(out), flush=True)
+    return 0 if vr == 0 else 1
 
 
 def bench_serve(ps5, args):
@@ -348,7 +400,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--rocprof-summary", metavar="DIR",
                     help="print the per-kernel table of a rocprofv3 trace + FETCH_SIZE + WRITE_SIZE run and exit")
-    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "batch", "serve", "merge"])
+    ap.add_argument("--shards", type=int, default=8, help="sorted shards merged in --mode merge")
     ap.add_argument("--sets", type=int, default=4096)
     ap.add_argument("--count-lo", type=int, default=1000)
     ap.add_argument("--count-hi", type=int, default=2_000_000)
@@ -385,6 +438,8 @@ def main():
         return bench_batch(ps5, args)
     if args.mode == "serve":
         return bench_serve(ps5, args)
+    if args.mode == "merge":
+        return bench_merge(ps5, args)
     t0 = time.time()
     arena, offs = ps5.workload(args.workload)
     n = len(offs)

@@ -1012,7 +1012,7 @@ __device__ __forceinline__ bool str_leq_from(const uint8_t* arena, uint64_t alen
 // bingmann-lcp_mergesort_binary.hpp) emits them, comparing characters only
 // where the LCPs to the previous output do not decide.  tstart[p] is the
 // first global thread of pair p.
-constexpr uint32_t kMergeE = 8;
+constexpr uint32_t kMergeE = 64;
 __global__ __launch_bounds__(kWG) void k_merge_round(Bufs B, const uint32_t* src_idx, const uint32_t* src_lcp,
                                                      uint32_t* dst_idx, uint32_t* dst_lcp, const uint64_t* first,
                                                      const uint64_t* mid, const uint64_t* tstart, uint32_t npairs)

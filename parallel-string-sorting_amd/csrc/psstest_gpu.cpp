@@ -48,6 +48,33 @@ static void mi355x_parallel_sample_sortBTCU2_out(unsigned char** strings, size_t
     memcpy(strings, out.data(), n * sizeof(unsigned char*));   // copy back for verification
 }
 
+// LCP merge sort in the shape of the reference's eberle/parallel_lcp_mergesort
+// contestants (src/parallel/eberle-parallel-lcp-mergesort.hpp:99-126): eight
+// index ranges sorted with their LCP arrays, then merged by the LCP-aware
+// device merge -- the path a sharded (multi-GPU) sort takes.
+static void mi355x_lcp_mergesort_8way(unsigned char** strings, size_t n)
+{
+    const unsigned K = n >= 8 ? 8 : 1;
+    std::vector<unsigned char*> runs_v[8];
+    std::vector<uintptr_t> lcps_v[8];
+    unsigned char* const* runs[8];
+    const uintptr_t* lcps[8];
+    size_t sizes[8];
+    for (unsigned r = 0; r < K; ++r) {
+        const size_t b = n * r / K, e = n * (r + 1) / K;
+        runs_v[r].assign(strings + b, strings + e);
+        lcps_v[r].assign(e - b + 1, 0);
+        mi355x_ps5_sort_lcp(runs_v[r].data(), lcps_v[r].data(), e - b);
+        runs[r] = runs_v[r].data();
+        lcps[r] = lcps_v[r].data();
+        sizes[r] = e - b;
+    }
+    if (int rc = ps5_lcp_merge_strings(runs, lcps, sizes, K, strings, nullptr)) {
+        fprintf(stderr, "ps5_lcp_merge_strings: error %d: %s\n", rc, ps5_last_error());
+        abort();
+    }
+}
+
 static void std_sort_strcmp(unsigned char** strings, size_t n)
 {
     std::stable_sort(strings, strings + n, [](const unsigned char* a, const unsigned char* b) {
@@ -63,6 +90,8 @@ PSS_CONTESTANT_PARALLEL(mi355x_ps5_sort_lcp_cache, "mi355x/parallel_sample_sortB
                         "pS5 on MI355X with LCP array and distinguishing characters (gfx950)")
 PSS_CONTESTANT_PARALLEL(mi355x_parallel_sample_sortBTCU2_out, "mi355x/parallel_sample_sortBTCU2_out",
                         "pS5 on MI355X, separate output array (gfx950)")
+PSS_CONTESTANT_PARALLEL(mi355x_lcp_mergesort_8way, "mi355x/lcp_mergesort_8way",
+                        "8 sorted index ranges merged by the device LCP merge (gfx950)")
 PSS_CONTESTANT(std_sort_strcmp, "std/stable_sort_strcmp", "std::stable_sort with strcmp (host, one thread)")
 
 // ---- input -----------------------------------------------------------------
