@@ -1005,86 +1005,186 @@ __device__ __forceinline__ bool str_leq_from(const uint8_t* arena, uint64_t alen
 // One round of pairwise merges.  Pair p merges runs A = [first[p], mid[p])
 // and B = [mid[p], first[p+1]) of (idx, lcp) into the same positions of the
 // output; lcp[q] is the LCP of the strings at q-1 and q within a run (its
-// first entry unused).  Thread t of pair p produces outputs
-// [first[p] + t*E, +E): a co-rank search places them in A and B (string
-// comparisons; equal strings take A first, so ties keep run order), then the
-// binary LCP merge (lcp_merge_binary and lcp_compare,
-// bingmann-lcp_mergesort_binary.hpp) emits them, comparing characters only
-// where the LCPs to the previous output do not decide.  tstart[p] is the
-// first global thread of pair p.
-constexpr uint32_t kMergeE = 64;
-__global__ __launch_bounds__(kWG) void k_merge_round(Bufs B, const uint32_t* src_idx, const uint32_t* src_lcp,
-                                                     uint32_t* dst_idx, uint32_t* dst_lcp, const uint64_t* first,
-                                                     const uint64_t* mid, const uint64_t* tstart, uint32_t npairs)
+// first entry unused).  The output of a pair is cut into tiles of kMergeT
+// positions, one workgroup each.  k_merge_part finds, for the first output
+// of every tile, how many strings of A precede it (a merge-path co-rank,
+// by string comparisons; equal strings take A first, so ties keep run
+// order).  k_merge_tile stages the tile's strings of A and B in LDS (index,
+// LCP and their first 8 characters), places each thread's kMergeE outputs by
+// a co-rank inside the tile, and emits them with the binary LCP merge
+// (lcp_merge_binary and lcp_compare, bingmann-lcp_mergesort_binary.hpp),
+// which compares characters only where the LCPs to the previous output do
+// not decide; the staged characters decide most of those compares without
+// touching the arena.  The first 8 characters of every output are written
+// beside it, so only the first round reads them from the arena.
+constexpr uint32_t kMergeE = 8;                    // outputs per thread
+constexpr uint32_t kMergeT = kWG * kMergeE;        // outputs per tile
+
+// a <= b for strings a, b (input indices) whose first 8 characters are ka, kb
+__device__ __forceinline__ bool merge_leq(const Bufs& B, uint32_t a, uint32_t b, key64 ka, key64 kb,
+                                          uint32_t* lcp)
 {
-    const uint64_t gt = (uint64_t)blockIdx.x * kWG + threadIdx.x;
-    if (gt >= tstart[npairs]) return;
-    uint32_t lo = 0, hi = npairs;   // the pair holding thread gt: largest p with tstart[p] <= gt
+    if (ka != kb) {
+        *lcp = lcp_key(ka, kb);
+        return ka < kb;
+    }
+    if ((ka & 0xFFull) == 0) {
+        *lcp = key_depth(ka);
+        return true;
+    }
+    return str_leq_from(B.arena, B.alen, str_off(B, a), str_off(B, b), 8, lcp);
+}
+
+// first 8 characters of the string at position pos of the source
+__device__ __forceinline__ key64 merge_key(const Bufs& B, const uint32_t* src_idx, const key64* src_key,
+                                           uint64_t pos)
+{
+    return src_key ? src_key[pos] : load_key(B.arena, B.alen, str_off(B, src_idx[pos]));
+}
+
+// the pair holding tile g: the largest p with tstart[p] <= g
+__device__ __forceinline__ uint32_t merge_pair(const uint64_t* tstart, uint32_t npairs, uint64_t g)
+{
+    uint32_t lo = 0, hi = npairs;
     while (lo + 1 < hi) {
         const uint32_t m = (lo + hi) / 2;
-        if (tstart[m] <= gt) lo = m;
+        if (tstart[m] <= g) lo = m;
         else hi = m;
     }
-    const uint32_t p = lo;
-    const uint64_t a0 = first[p], b0 = mid[p], e0 = first[p + 1];
-    const uint64_t la = b0 - a0, lb = e0 - b0;
-    const uint64_t diag = (gt - tstart[p]) * kMergeE;
-    if (diag >= la + lb) return;
-    const uint32_t* IA = src_idx + a0;
-    const uint32_t* IB = src_idx + b0;
-    auto off = [&](uint32_t id) { return str_off(B, id); };
-    // co-rank: i strings of A and diag - i of B precede this thread's outputs
+    return lo;
+}
+
+// co[g] = strings of A (of g's pair) that precede the first output of tile g
+__global__ __launch_bounds__(kWG) void k_merge_part(Bufs B, const uint32_t* src_idx, const key64* src_key,
+                                                    const uint64_t* first, const uint64_t* mid,
+                                                    const uint64_t* tstart, uint32_t npairs, uint32_t* co)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * kWG + threadIdx.x;
+    if (g >= tstart[npairs]) return;
+    const uint32_t p = merge_pair(tstart, npairs, g);
+    const uint64_t a0 = first[p], b0 = mid[p];
+    const uint64_t la = b0 - a0, lb = first[p + 1] - b0;
+    const uint64_t diag = (g - tstart[p]) * kMergeT;
     uint64_t i0 = diag > lb ? diag - lb : 0, i1 = min(diag, la);
     while (i0 < i1) {
         const uint64_t mi = (i0 + i1) / 2;
+        const uint64_t pa = a0 + mi, pb = b0 + diag - 1 - mi;
         uint32_t l;
-        if (str_leq_from(B.arena, B.alen, off(IA[mi]), off(IB[diag - 1 - mi]), 0, &l)) i0 = mi + 1;
-        else i1 = mi;
+        if (merge_leq(B, src_idx[pa], src_idx[pb], merge_key(B, src_idx, src_key, pa),
+                      merge_key(B, src_idx, src_key, pb), &l))
+            i0 = mi + 1;
+        else
+            i1 = mi;
     }
-    uint64_t i = i0, j = diag - i0;
-    // LCPs of the next A and B string with the previous output
-    uint32_t lcpa = 0, lcpb = 0;
-    if (diag > 0) {
-        bool prev_is_a;
-        if (i == 0) prev_is_a = false;
-        else if (j == 0) prev_is_a = true;
-        else {
-            uint32_t l;
-            prev_is_a = !str_leq_from(B.arena, B.alen, off(IA[i - 1]), off(IB[j - 1]), 0, &l);
-        }
-        const uint64_t prev = prev_is_a ? off(IA[i - 1]) : off(IB[j - 1]);
-        if (i < la) {
-            if (prev_is_a) lcpa = src_lcp[a0 + i];
-            else str_leq_from(B.arena, B.alen, prev, off(IA[i]), 0, &lcpa);
-        }
-        if (j < lb) {
-            if (!prev_is_a) lcpb = src_lcp[b0 + j];
-            else str_leq_from(B.arena, B.alen, prev, off(IB[j]), 0, &lcpb);
-        }
-    }
-    const uint64_t nout = min((uint64_t)kMergeE, la + lb - diag);
-    for (uint64_t q = 0; q < nout; ++q) {
-        const uint64_t o = a0 + diag + q;
-        bool take_a;
-        uint32_t lab = 0;     // LCP of the two heads
-        if (j >= lb) take_a = true;
-        else if (i >= la) take_a = false;
-        else if (lcpa == lcpb) take_a = str_leq_from(B.arena, B.alen, off(IA[i]), off(IB[j]), lcpa, &lab);
-        else if (lcpa > lcpb) { take_a = true; lab = lcpb; }
-        else { take_a = false; lab = lcpa; }
-        if (take_a) {
-            dst_idx[o] = IA[i];
-            dst_lcp[o] = lcpa;
-            ++i;
-            lcpa = i < la ? src_lcp[a0 + i] : 0u;
-            lcpb = lab;
+    co[g] = (uint32_t)i0;
+}
+
+__global__ __launch_bounds__(kWG) void k_merge_tile(Bufs B, const uint32_t* src_idx, const uint32_t* src_lcp,
+                                                    const key64* src_key, uint32_t* dst_idx, uint32_t* dst_lcp,
+                                                    key64* dst_key, const uint64_t* first, const uint64_t* mid,
+                                                    const uint64_t* tstart, uint32_t npairs, const uint32_t* co)
+{
+    // slots [0, na+1): A[ia0-1 .. ia1), slots [na+1, na+nb+2): B[jb0-1 .. jb1);
+    // slot 0 of a part is unused when the part starts its run
+    __shared__ key64 s_key[kMergeT + 2];
+    __shared__ uint32_t s_idx[kMergeT + 2];
+    __shared__ uint32_t s_lcp[kMergeT + 2];
+    __shared__ uint16_t s_oslot[kMergeT];
+    __shared__ uint32_t s_olcp[kMergeT];
+    const uint64_t g = blockIdx.x;
+    const uint32_t p = merge_pair(tstart, npairs, g);
+    const uint64_t a0 = first[p], b0 = mid[p];
+    const uint64_t la = b0 - a0, lb = first[p + 1] - b0;
+    const uint64_t diag0 = (g - tstart[p]) * kMergeT;
+    const uint32_t len = (uint32_t)min((uint64_t)kMergeT, la + lb - diag0);
+    const uint64_t ia0 = co[g], ia1 = g + 1 < tstart[p + 1] ? co[g + 1] : la;
+    const uint64_t jb0 = diag0 - ia0;
+    const uint32_t na = (uint32_t)(ia1 - ia0), nb = len - na;
+    const uint32_t sb = na + 1;        // first slot of B
+    for (uint32_t k = threadIdx.x; k < len + 2; k += kWG) {
+        uint64_t pos;
+        bool valid;
+        if (k < sb) {
+            valid = k > 0 || ia0 > 0;
+            pos = a0 + ia0 + k - 1;
         } else {
-            dst_idx[o] = IB[j];
-            dst_lcp[o] = lcpb;
-            ++j;
-            lcpb = j < lb ? src_lcp[b0 + j] : 0u;
-            lcpa = lab;
+            valid = k > sb || jb0 > 0;
+            pos = b0 + jb0 + (k - sb) - 1;
         }
+        if (valid) {
+            s_idx[k] = src_idx[pos];
+            s_lcp[k] = src_lcp[pos];
+            s_key[k] = merge_key(B, src_idx, src_key, pos);
+        }
+    }
+    __syncthreads();
+    const uint32_t d = threadIdx.x * kMergeE;
+    if (d < len) {
+        auto leq = [&](uint32_t x, uint32_t y, uint32_t* l) {
+            return merge_leq(B, s_idx[x], s_idx[y], s_key[x], s_key[y], l);
+        };
+        uint32_t i0 = d > nb ? d - nb : 0, i1 = min(d, na);
+        while (i0 < i1) {
+            const uint32_t mi = (i0 + i1) / 2;
+            uint32_t l;
+            if (leq(1 + mi, sb + d - mi, &l)) i0 = mi + 1;     // A[mi] <= B[d-1-mi]
+            else i1 = mi;
+        }
+        uint32_t i = i0, j = d - i0;
+        // LCPs of the next A and B string with the previous output
+        uint32_t lcpa = 0, lcpb = 0;
+        const bool hasa = ia0 + i > 0, hasb = jb0 + j > 0;
+        if (hasa || hasb) {
+            bool prev_is_a;
+            uint32_t l;
+            if (!hasb) prev_is_a = true;
+            else if (!hasa) prev_is_a = false;
+            else prev_is_a = !leq(i, sb + j, &l);
+            const uint32_t prev = prev_is_a ? i : sb + j;
+            if (i < na) {
+                if (prev_is_a) lcpa = s_lcp[1 + i];
+                else leq(prev, 1 + i, &lcpa);
+            }
+            if (j < nb) {
+                if (!prev_is_a) lcpb = s_lcp[sb + 1 + j];
+                else leq(prev, sb + 1 + j, &lcpb);
+            }
+        }
+        const uint32_t nout = min(kMergeE, len - d);
+        for (uint32_t q = 0; q < nout; ++q) {
+            bool take_a;
+            uint32_t lab = 0;     // LCP of the two heads
+            if (j >= nb) take_a = true;
+            else if (i >= na) take_a = false;
+            else if (lcpa == lcpb) {
+                const uint32_t x = 1 + i, y = sb + 1 + j;
+                if (lcpa < 8) take_a = leq(x, y, &lab);
+                else take_a = str_leq_from(B.arena, B.alen, str_off(B, s_idx[x]), str_off(B, s_idx[y]), lcpa, &lab);
+            }
+            else if (lcpa > lcpb) { take_a = true; lab = lcpb; }
+            else { take_a = false; lab = lcpa; }
+            if (take_a) {
+                s_oslot[d + q] = (uint16_t)(1 + i);
+                s_olcp[d + q] = lcpa;
+                ++i;
+                lcpa = i < na ? s_lcp[1 + i] : 0u;
+                lcpb = lab;
+            } else {
+                s_oslot[d + q] = (uint16_t)(sb + 1 + j);
+                s_olcp[d + q] = lcpb;
+                ++j;
+                lcpb = j < nb ? s_lcp[sb + 1 + j] : 0u;
+                lcpa = lab;
+            }
+        }
+    }
+    __syncthreads();
+    const uint64_t o0 = a0 + diag0;
+    for (uint32_t k = threadIdx.x; k < len; k += kWG) {
+        const uint32_t s = s_oslot[k];
+        dst_idx[o0 + k] = s_idx[s];
+        dst_lcp[o0 + k] = s_olcp[k];
+        if (dst_key) dst_key[o0 + k] = s_key[s];
     }
 }
 
@@ -1827,12 +1927,15 @@ int ps5_merge_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, c
     B.arena = d_arena;
     B.alen = arena_bytes;
     B.offs = d_offs;
-    // ping-pong buffers: indices in idx[0]/idx[1], LCPs in key[0]/key[1]
+    // ping-pong buffers: indices in idx[0]/idx[1], first 8 characters in
+    // key[0]/key[1], LCPs in the two halves of bkey
     uint32_t* ti[2] = { c->idx[0], c->idx[1] };
-    uint32_t* tl[2] = { reinterpret_cast<uint32_t*>(c->key[0]), reinterpret_cast<uint32_t*>(c->key[1]) };
+    key64* tk[2] = { c->key[0], c->key[1] };
+    uint32_t* tl[2] = { reinterpret_cast<uint32_t*>(c->bkey), reinterpret_cast<uint32_t*>(c->bkey) + c->cap };
     std::vector<uint64_t> bounds(run_first, run_first + nruns + 1);
     const uint32_t* si = d_idx;
     const uint32_t* sl = d_lcp;
+    const key64* sk = nullptr;     // the first round reads characters from the arena
     int cur = 0;
     while (bounds.size() > 2) {
         const uint32_t nr = (uint32_t)bounds.size() - 1, np = (nr + 1) / 2;
@@ -1844,23 +1947,33 @@ int ps5_merge_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, c
             f[p] = bounds[2 * p];
             mdl[p] = bounds[2 * p + 1];
             const uint64_t e = 2 * p + 2 <= nr ? bounds[2 * p + 2] : bounds[2 * p + 1];
-            ts[p + 1] = ts[p] + (e - f[p] + kMergeE - 1) / kMergeE;
+            ts[p + 1] = ts[p] + (e - f[p] + kMergeT - 1) / kMergeT;
         }
         f[np] = bounds[nr];
+        const uint64_t tiles = ts[np];
         const bool last = np == 1;
         uint32_t* di = last ? d_idx_out : ti[cur];
         uint32_t* dl = last ? d_lcp_out : tl[cur];
-        if (plan.size() > c->merge_cap) {
+        key64* dk = last ? nullptr : tk[cur];
+        const uint64_t words = plan.size() + (tiles + 1) / 2;    // plan, then co[tiles] as uint32
+        if (words > c->merge_cap) {
             if (c->d_merge) (void)hipFree(c->d_merge);
             c->d_merge = nullptr;
             c->merge_cap = 0;
-            HIPCHK(hipMalloc(&c->d_merge, plan.size() * sizeof(uint64_t)));
-            c->merge_cap = plan.size();
+            HIPCHK(hipMalloc(&c->d_merge, words * sizeof(uint64_t)));
+            c->merge_cap = words;
         }
         HIPCHK(hipMemcpyAsync(c->d_merge, plan.data(), plan.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
-        if (ts[np]) {
-            hipLaunchKernelGGL(k_merge_round, dim3(div_up(ts[np], kWG)), dim3(kWG), 0, st, B, si, sl, di, dl,
-                               c->d_merge, c->d_merge + (np + 1), c->d_merge + 2 * (np + 1), np);
+        if (tiles) {
+            const uint64_t* pf = c->d_merge;
+            const uint64_t* pm = pf + (np + 1);
+            const uint64_t* pt = pm + (np + 1);
+            uint32_t* co = reinterpret_cast<uint32_t*>(c->d_merge + plan.size());
+            hipLaunchKernelGGL(k_merge_part, dim3(div_up(tiles, kWG)), dim3(kWG), 0, st, B, si, sk, pf, pm, pt, np,
+                               co);
+            HIPCHK(hipGetLastError());
+            hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kWG), 0, st, B, si, sl, sk, di, dl, dk, pf, pm, pt,
+                               np, co);
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipStreamSynchronize(st));     // the plan buffer is rewritten next round
@@ -1869,6 +1982,7 @@ int ps5_merge_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, c
         bounds.swap(nb);
         si = di;
         sl = dl;
+        sk = dk;
         cur ^= 1;
     }
     return readback(c, nullptr, 0);
