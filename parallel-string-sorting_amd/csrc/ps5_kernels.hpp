@@ -38,12 +38,16 @@ constexpr uint32_t kTreeSlots = 2048;        // per segment: level-order tree [0
 constexpr int kWaveChunk = 8192;             // keys ranked by one wave (multi-workgroup step)
 constexpr int kTile = kWaveChunk * kWaves;   // keys per count workgroup
 constexpr uint32_t kTinyT = 64;              // leaves sorted by one wave
-constexpr uint32_t kBlkT = 1024;             // leaves sorted on chip by one workgroup
+constexpr uint32_t kBlkT = 2048;             // leaves sorted on chip by one workgroup
+constexpr uint32_t kMaxLeafTarget = 1024;    // largest leaf size the fan-out aims at
 constexpr uint32_t kRunChunk = 4096;         // equal-key leaves are processed in chunks
-// Blocks are kept in three size classes, each sorted by a kernel whose LDS
-// fits its largest block: (64, 256], (256, 512], (512, 1024].
-constexpr int kBlkClasses = 3;
-__host__ __device__ constexpr int block_class(uint32_t m) { return m <= 256 ? 0 : m <= 512 ? 1 : 2; }
+// Blocks are kept in four size classes, each sorted by a kernel whose LDS
+// fits its largest block: (64, 256], (256, 512], (512, 1024], (1024, 2048].
+constexpr int kBlkClasses = 4;
+__host__ __device__ constexpr int block_class(uint32_t m)
+{
+    return m <= 256 ? 0 : m <= 512 ? 1 : m <= 1024 ? 2 : 3;
+}
 constexpr int kRankBits = 21;
 constexpr uint32_t kRankMask = (1u << kRankBits) - 1;
 constexpr uint32_t kNoBucket = 0x7FF;        // bucket id of inactive lanes
@@ -64,10 +68,10 @@ enum : int {
     C_LROUND = 0,  // [2] large segments starting a round (round parity)
     C_LLVL = 2,    // [2] large segments of the next level (level parity)
     C_TINY = 4,    // [2] leaf lists (round parity)
-    C_BLOCK = 6,   // [3][2] blocks: C_BLOCK + 2 * size class + round parity
-    C_RUN = 12,    // [2]
-    C_ERR = 14,    // error flags
-    C_NCNT = 16
+    C_BLOCK = 6,   // [4][2] blocks: C_BLOCK + 2 * size class + round parity
+    C_RUN = 14,    // [2]
+    C_ERR = 16,    // error flags
+    C_NCNT = 18
 };
 constexpr uint32_t ERR_CAP = 1u, ERR_LCP = 2u, ERR_BOUNDS = 4u;
 

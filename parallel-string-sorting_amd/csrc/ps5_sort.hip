@@ -60,7 +60,7 @@ __global__ void k_setup(uint32_t* cnt, Item* list, int c_index, uint32_t n)
 // single copy brings both to the host: out[0, C_NCNT) = counters, then
 // min(cnt[ci], kPeek) items from byte kStageHead on.
 constexpr uint32_t kPeek = 2048;
-constexpr uint32_t kStageHead = 64;
+constexpr uint32_t kStageHead = 128;
 static_assert(C_NCNT * 4 <= kStageHead, "counters must fit the stage head");
 __global__ __launch_bounds__(kWG) void k_stage(const uint32_t* cnt, const Item* list, int ci, unsigned char* out)
 {
@@ -454,7 +454,7 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
 {
     constexpr int kPer = (kMaxBkt + kWG - 1) / kWG;   // 8
     const uint32_t dstbuf = (seg.flags & F_BUFB) ^ F_BUFB;
-    uint32_t c_t = 0, c_l = 0, c_r = 0, c_b[kBlkClasses] = { 0, 0, 0 };
+    uint32_t c_t = 0, c_l = 0, c_r = 0, c_b[kBlkClasses] = {};
     uint32_t st[kPer], sz[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
@@ -1735,6 +1735,9 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                     else if (cls == 1)
                         hipLaunchKernelGGL((k_block<512, 8>), dim3(nblk[cls]), dim3(512 / 8), 0, st, list,
                                            blk_done[cls], cnt, B, nsink, depth);
+                    else if (cls == 2)
+                        hipLaunchKernelGGL((k_block<1024, 8>), dim3(nblk[cls]), dim3(1024 / 8), 0, st, list,
+                                           blk_done[cls], cnt, B, nsink, depth);
                     else
                         hipLaunchKernelGGL((k_block<kBlkT, 8>), dim3(nblk[cls]), dim3(kBlkT / 8), 0, st, list,
                                            blk_done[cls], cnt, B, nsink, depth);
@@ -1743,6 +1746,9 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                 blk_done[cls] += nblk[cls];
             }
             HIPCHK(hipGetLastError());
+            // blocks only append to the next round's lists: after a level
+            // without large segments this round has no more levels
+            if (nlarge == 0) break;
             if ((rc = readback(c, nx_large, nx_ci))) return rc;
             cur_ci = nx_ci;
         }
@@ -1752,9 +1758,9 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
         for (int cls = 0; cls < kBlkClasses; ++cls) nb += c->h_cnt[C_BLOCK + 2 * cls + rp];
         c->st_leaves += nt + nb + nr;
         if (c->trace) {
-            fprintf(stderr, "[ps5] round %u depth %u: tiny %u, blocks %u/%u/%u, equal chunks %u, levels %llu\n", round,
-                    depth, nt, c->h_cnt[C_BLOCK + rp], c->h_cnt[C_BLOCK + 2 + rp], c->h_cnt[C_BLOCK + 4 + rp], nr,
-                    (unsigned long long)c->st_levels);
+            fprintf(stderr, "[ps5] round %u depth %u: tiny %u, blocks %u/%u/%u/%u, equal chunks %u, levels %llu\n",
+                    round, depth, nt, c->h_cnt[C_BLOCK + rp], c->h_cnt[C_BLOCK + 2 + rp], c->h_cnt[C_BLOCK + 4 + rp],
+                    c->h_cnt[C_BLOCK + 6 + rp], nr, (unsigned long long)c->st_levels);
         }
         if (nt) {
             {
@@ -2062,7 +2068,8 @@ int ps5_sort_device_timed(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_byt
 int ps5_ctx_set_tuning(ps5_ctx* c, uint32_t leaf_target, uint32_t max_treebits, uint32_t max_oversample)
 {
     if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
-    if (leaf_target < kTinyT || leaf_target > kBlkT) return set_err(PS5_ERR_ARG, "leaf_target must be in [64, 1024]");
+    if (leaf_target < kTinyT || leaf_target > kMaxLeafTarget)
+        return set_err(PS5_ERR_ARG, "leaf_target must be in [64, 1024]");
     if (max_treebits < 1 || max_treebits > (uint32_t)kMaxTreebits)
         return set_err(PS5_ERR_ARG, "max_treebits must be in [1, 10]");
     if (max_oversample < 2 || max_oversample > 32 || (max_oversample & 1))

@@ -192,7 +192,11 @@ def test_runner_suffix_and_compressed(tmp_path):
         say(r.stdout[-2000:], r.stderr[-2000:])
         assert r.returncode == 0
         res = [ln for ln in r.stdout.splitlines() if ln.startswith("RESULT")]
-        assert len(res) == 4 and all("status=ok" in ln for ln in res)
+        algos = sorted(ln.split("algo=")[1].split("\t")[0] for ln in res)
+        assert algos == ["mi355x/lcp_mergesort_8way", "mi355x/parallel_sample_sortBTCU2",
+                         "mi355x/parallel_sample_sortBTCU2_lcp", "mi355x/parallel_sample_sortBTCU2_lcp_cache",
+                         "mi355x/parallel_sample_sortBTCU2_out"]
+        assert all("status=ok" in ln for ln in res)
         count = len(raw) if extra else raw.count(b"\n")
         assert all(f"string_count={count}" in ln for ln in res)
 
