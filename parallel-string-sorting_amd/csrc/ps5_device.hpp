@@ -20,28 +20,33 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z)
 // first NUL on (the reference's get_char_uint64_simple, stringset.hpp:201-231).
 // Two aligned 8-byte loads and a funnel shift; near the arena end the bytes
 // are fetched one by one so nothing outside arena[0, alen) is read.
+// The arena is global memory: loads through address-space-1 pointers are
+// global loads, which wait on the vector memory counter alone (a generic
+// pointer would compile to flat loads that also wait for LDS operations).
+typedef __attribute__((address_space(1))) const uint64_t g_u64;
+typedef __attribute__((address_space(1))) const uint8_t g_u8;
 __device__ __forceinline__ key64 load_key(const uint8_t* arena, uint64_t alen, uint64_t off)
 {
-    const uint64_t base = (uint64_t)(uintptr_t)arena;
-    const uint64_t a = base + off;
-    const uint64_t a8 = a & ~7ull;
+    const uint64_t a8 = off & ~7ull;
     uint64_t lo, hi;
-    if (a8 + 16 <= base + alen) {
-        lo = *(const uint64_t*)(uintptr_t)a8;
-        hi = *(const uint64_t*)(uintptr_t)(a8 + 8);
+    if (a8 + 16 <= alen) {
+        const g_u64* p = (const g_u64*)(arena + a8);
+        lo = p[0];
+        hi = p[1];
     } else {
         lo = 0;
         hi = 0;
+        const g_u8* b = (const g_u8*)arena;
         for (int i = 0; i < 16; ++i) {
-            uint64_t p = a8 + i;
-            if (p >= base && p < base + alen) {
-                uint64_t c = *(const uint8_t*)(uintptr_t)p;
+            const uint64_t q = a8 + i;
+            if (q < alen) {
+                const uint64_t c = b[q];
                 if (i < 8) lo |= c << (8 * i);
                 else hi |= c << (8 * (i - 8));
             }
         }
     }
-    const uint32_t sh = (uint32_t)(a & 7) * 8;
+    const uint32_t sh = (uint32_t)(off & 7) * 8;
     uint64_t v = sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
     v = __builtin_bswap64(v);
     // high bit of each byte of t is set iff that byte is non-zero (no carries)
@@ -153,14 +158,18 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t bid, uint32_t G)
 // lanes holding the same 11-bit bucket id (all 64 lanes must execute this)
 __device__ __forceinline__ uint64_t match_bucket(uint32_t b)
 {
-    uint64_t peers = __ballot(1);
+    const uint64_t act = __ballot(1);
+    uint32_t lo = (uint32_t)act, hi = (uint32_t)(act >> 32);
 #pragma unroll
     for (int i = 0; i < 11; ++i) {
-        const bool bit = (b >> i) & 1u;
-        const uint64_t bal = __ballot(bit);
-        peers &= bit ? bal : ~bal;
+        // m: all ones if bit i of b is set, else zero; lanes agreeing with
+        // this lane on bit i are ~(ballot ^ m), computed as (~ballot) ^ m
+        const uint32_t m = (uint32_t)((int32_t)(b << (31 - i)) >> 31);
+        const uint64_t nbal = ~(uint64_t)__ballot(m != 0u);
+        lo &= (uint32_t)nbal ^ m;
+        hi &= (uint32_t)(nbal >> 32) ^ m;
     }
-    return peers;
+    return ((uint64_t)hi << 32) | lo;
 }
 
 __device__ __forceinline__ void flag_error(uint32_t* cnt, uint32_t e) { atomicOr(&cnt[C_ERR], e); }
@@ -283,6 +292,27 @@ __device__ __forceinline__ uint32_t classify_key(key64 key, const key64* s_tree,
     i -= 1u << k;
     const uint32_t nspl = (1u << k) - 1;
     return 2 * i + ((i < nspl && s_sorted[i] == key) ? 1u : 0u);
+}
+
+// classify_key for G keys at once: the G descents advance level by level
+// together, so their LDS reads are in flight at the same time.
+template <int G>
+__device__ __forceinline__ void classify_keys(const key64 (&key)[G], const key64* s_tree, const key64* s_sorted,
+                                              int k, uint32_t (&b)[G])
+{
+    uint32_t i[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) i[j] = 1;
+    for (int l = 0; l < k; ++l) {
+#pragma unroll
+        for (int j = 0; j < G; ++j) i[j] = 2 * i[j] + (key[j] > s_tree[i[j]] ? 1u : 0u);
+    }
+    const uint32_t nspl = (1u << k) - 1;
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        const uint32_t x = i[j] - (1u << k);
+        b[j] = 2 * x + ((x < nspl && s_sorted[x] == key[j]) ? 1u : 0u);
+    }
 }
 
 }  // namespace ps5

@@ -240,10 +240,11 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
             }
         }
         uint32_t bv[kGroups];
+        classify_keys<kGroups>(kv, s_tree, s_sorted, k, bv);
 #pragma unroll
         for (int j = 0; j < kGroups; ++j) {
             const uint32_t p = g + 64 * j + lane;
-            bv[j] = p < ce ? classify_key(kv[j], s_tree, s_sorted, k) : kNoBucket;
+            if (p >= ce) bv[j] = kNoBucket;
         }
 #pragma unroll
         for (int j = 0; j < kGroups; ++j) {
