@@ -339,109 +339,174 @@ constexpr uint32_t kScatBaseOff = kScatStartOff + 2048 * 4;
 constexpr uint32_t kScatTmpOff = kScatBaseOff + 2048 * 4;
 constexpr uint32_t kScatLds = kScatTmpOff + 64 * 4;
 
-// Move one wave chunk (kWaveChunk keys) to its buckets (distribute,
-// .hpp:1464-1490, but stable).  The chunk is first reordered by bucket in
+// Move wave chunks (kWaveChunk keys each) to their buckets (distribute,
+// .hpp:1464-1490, but stable).  A chunk is first reordered by bucket in
 // LDS, then written out position by position, so consecutive lanes store
 // consecutive addresses inside each bucket: full cache lines instead of the
 // scattered 8-byte stores of a direct scatter.
-__global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* desc, const TileRef* tiles, Bufs B,
-                                                     const uint32_t* hscan)
+//
+// The chunk's LDS image takes most of a CU's LDS, so one workgroup runs per
+// CU; to keep memory busy, each workgroup loops over chunks and loads the
+// next chunk (keys, indices, ranks and histogram entries) into registers
+// before it reorders and writes the current one.  Chunks are split into
+// kXcds contiguous ranges, one per XCD (workgroup g runs on XCD g % kXcds),
+// so the chunks in flight on an XCD are neighbours that share histogram
+// lines and bucket output lines in its L2.
+constexpr int kScatPer = kWaveChunk / kBigWG;
+static_assert(kWaveChunk % kBigWG == 0, "a chunk is a whole number of workgroup passes");
+
+// threadIdx.x through an opaque move: addresses derived from it are
+// recomputed per chunk instead of being hoisted out of the chunk loop,
+// where they would hold registers the two chunk buffers need
+__device__ __forceinline__ uint32_t scat_tid()
 {
-    extern __shared__ __align__(16) unsigned char smem[];
+    uint32_t t;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((uint32_t)threadIdx.x));
+    return t;
+}
+
+struct ScatChunk {       // one chunk's description and its prefetched inputs
+    LargeDesc D;
+    uint32_t t, w, cb, m;
+    uint32_t br[kScatPer], iv[kScatPer];
+    key64 kv[kScatPer];
+    uint32_t h0, h0n, h1, h1n, hseg;   // histogram entries of buckets tid, tid + kBigWG; segment base
+};
+
+// Issue the loads of chunk c.  Every load is unconditional (indices are
+// clamped into the arrays and invalid lanes discard their values when the
+// chunk is processed): the same number of loads every time lets the
+// compiler wait for exactly the older loads when the chunk is used, while
+// the next chunk's loads stay in flight.
+__device__ __forceinline__ void scat_fetch(const LargeDesc* __restrict__ desc, const TileRef* __restrict__ tiles,
+                                           const Bufs& B, const uint32_t* __restrict__ hscan, uint32_t c,
+                                           ScatChunk& S)
+{
+    const uint32_t tid = scat_tid();
+    const TileRef tr = tiles[c / kWaves];
+    S.w = c % kWaves;
+    S.t = tr.t;
+    S.D = desc[tr.seg];
+    const Item seg = S.D.seg;
+    S.cb = S.t * kTile + S.w * kWaveChunk;
+    const bool ok = item_ok(seg, B) && S.t < S.D.T && S.cb < seg.size;
+    S.m = ok ? min(seg.size - S.cb, (uint32_t)kWaveChunk) : 0u;
+    const key64* K = (seg.flags & F_BUFB) ? B.key1 : B.key0;
+    // a root's indices are implicit: its third load repeats the rank load
+    const uint32_t* I = (seg.flags & F_IDENT) ? B.bktrank : (seg.flags & F_BUFB) ? B.idx1 : B.idx0;
+    const uint32_t qmax = B.n - 1;
+#pragma unroll
+    for (int e = 0; e < kScatPer; ++e) {
+        const uint32_t q = min(seg.begin + S.cb + e * kBigWG + tid, qmax);
+        S.br[e] = B.bktrank[q];
+        S.kv[e] = K[q];
+        S.iv[e] = I[q];
+    }
+    // histogram entries; entry e's count is scan[e + 1] - scan[e], and the
+    // entry after the segment's last one is segbase + size (fixed up on use)
+    const uint32_t* h = hscan + S.D.hist_base;
+    const uint32_t T = max(S.D.T, 1u), nbkt = (2u << S.D.k) - 1;
+    const uint32_t last = ((nbkt - 1) * T + (T - 1)) * kWaves + (kWaves - 1);
+    const uint32_t t = min(S.t, T - 1);
+    const uint32_t e0 = (min(tid, nbkt - 1) * T + t) * kWaves + S.w;
+    const uint32_t e1 = (min(tid + kBigWG, nbkt - 1) * T + t) * kWaves + S.w;
+    S.hseg = h[0];
+    S.h0 = h[e0];
+    S.h0n = h[min(e0 + 1, last)];
+    S.h1 = h[e1];
+    S.h1n = h[min(e1 + 1, last)];
+}
+
+// Reorder one prefetched chunk by bucket in LDS and write it out.
+__device__ __forceinline__ void scat_chunk(const ScatChunk& S, const Bufs& B, unsigned char* smem)
+{
+    const uint32_t tid = scat_tid();
     key64* s_key = reinterpret_cast<key64*>(smem + kScatKeyOff);
     uint32_t* s_idx = reinterpret_cast<uint32_t*>(smem + kScatIdxOff);
     uint16_t* s_bkt = reinterpret_cast<uint16_t*>(smem + kScatBktOff);
     uint32_t* s_start = reinterpret_cast<uint32_t*>(smem + kScatStartOff);
     uint32_t* s_base = reinterpret_cast<uint32_t*>(smem + kScatBaseOff);
     uint32_t* s_tmp = reinterpret_cast<uint32_t*>(smem + kScatTmpOff);
-
-    const uint32_t chunk = xcd_contiguous(blockIdx.x, gridDim.x);
-    const TileRef tr = tiles[chunk / kWaves];
-    const uint32_t w = chunk % kWaves;
-    const LargeDesc D = desc[tr.seg];
-    const Item seg = D.seg;
-    const int k = (int)D.k, nbkt = (2 << k) - 1;
-    const uint32_t t = tr.t;
-    const uint32_t cb = t * kTile + w * kWaveChunk;
-    if (!item_ok(seg, B) || t >= D.T || cb >= seg.size) return;
-    const uint32_t m = min(seg.size - cb, (uint32_t)kWaveChunk);
-    const int src = seg.flags & F_BUFB;
-    const key64* K = src ? B.key1 : B.key0;
-    const uint32_t* I = src ? B.idx1 : B.idx0;
-    key64* KD = src ? B.key0 : B.key1;
-    uint32_t* ID = src ? B.idx0 : B.idx1;
-    // the chunk's loads go out first: every thread issues its kPer elements'
-    // loads, which then overlap the histogram prologue below
-    constexpr int kPer = kWaveChunk / kBigWG;
-    static_assert(kWaveChunk % kBigWG == 0, "a chunk is a whole number of workgroup passes");
-    const bool ident = seg.flags & F_IDENT;
-    uint32_t br[kPer], iv[kPer];
-    key64 kv[kPer];
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-        const uint32_t p = e * kBigWG + threadIdx.x;
-        const uint32_t q = seg.begin + cb + p;
-        br[e] = 0xFFFFFFFFu;
-        if (p < m) {
-            br[e] = B.bktrank[q];
-            kv[e] = K[q];
-            iv[e] = ident ? q : I[q];
-        }
-    }
-    const uint32_t* h = hscan + D.hist_base;
-    const uint32_t segbase = h[0];
-    // counts and global bases of this chunk per bucket; entry e's count is
-    // scan[e + 1] - scan[e] (the entry after the segment's last one holds
-    // segbase + size)
-    const uint32_t last = ((uint32_t)(nbkt - 1) * D.T + (D.T - 1)) * kWaves + (kWaves - 1);
+    __syncthreads();     // the previous chunk's LDS image has been written out
+    const uint32_t m = S.m;
+    if (m == 0) return;      // (uniform) an empty or malformed chunk
+    const Item seg = S.D.seg;
+    const uint32_t nbkt = (2u << S.D.k) - 1;
+    key64* KD = (seg.flags & F_BUFB) ? B.key0 : B.key1;
+    uint32_t* ID = (seg.flags & F_BUFB) ? B.idx0 : B.idx1;
+    const uint32_t segend = S.hseg + seg.size;
+    const uint32_t last = ((nbkt - 1) * S.D.T + (S.D.T - 1)) * kWaves + (kWaves - 1);
+    const uint32_t b0 = tid, b1 = tid + kBigWG;
     uint32_t c0 = 0, c1 = 0;
-    {
-        const int b0 = threadIdx.x, b1 = threadIdx.x + kBigWG;
-        if (b0 < nbkt) {
-            const uint32_t e = ((uint32_t)b0 * D.T + t) * kWaves + w;
-            const uint32_t v = h[e];
-            c0 = (e == last ? segbase + seg.size : h[e + 1]) - v;
-            s_base[b0] = v - segbase;
-        }
-        if (b1 < nbkt) {
-            const uint32_t e = ((uint32_t)b1 * D.T + t) * kWaves + w;
-            const uint32_t v = h[e];
-            c1 = (e == last ? segbase + seg.size : h[e + 1]) - v;
-            s_base[b1] = v - segbase;
-        }
+    if (b0 < nbkt) {
+        const uint32_t e = (b0 * S.D.T + S.t) * kWaves + S.w;
+        c0 = (e == last ? segend : S.h0n) - S.h0;
+        s_base[b0] = S.h0 - S.hseg;
+    }
+    if (b1 < nbkt) {
+        const uint32_t e = (b1 * S.D.T + S.t) * kWaves + S.w;
+        c1 = (e == last ? segend : S.h1n) - S.h1;
+        s_base[b1] = S.h1 - S.hseg;
     }
     // local bucket starts: buckets 0..1023 first, then 1024..2046
     uint32_t tot0, tot1;
     const uint32_t ex0 = block_excl_scan<kBigWaves>(c0, &tot0, s_tmp);
     const uint32_t ex1 = block_excl_scan<kBigWaves>(c1, &tot1, s_tmp);
-    if (threadIdx.x < (uint32_t)nbkt) s_start[threadIdx.x] = ex0;
-    if (threadIdx.x + kBigWG < (uint32_t)nbkt) s_start[threadIdx.x + kBigWG] = tot0 + ex1;
+    if (b0 < nbkt) s_start[b0] = ex0;
+    if (b1 < nbkt) s_start[b1] = tot0 + ex1;
     __syncthreads();
     if (tot0 + tot1 != m) {   // the histogram must account for every key of the chunk
-        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
+        if (tid == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-        const uint32_t p = e * kBigWG + threadIdx.x;
+    for (int e = 0; e < kScatPer; ++e) {
+        const uint32_t p = e * kBigWG + tid;
         if (p >= m) continue;
-        const uint32_t b = br[e] >> kRankBits;
-        const uint32_t l = b < (uint32_t)nbkt ? s_start[b] + (br[e] & kRankMask) : 0xFFFFFFFFu;
+        const uint32_t b = S.br[e] >> kRankBits;
+        const uint32_t l = b < nbkt ? s_start[b] + (S.br[e] & kRankMask) : 0xFFFFFFFFu;
         if (l >= m) { flag_error(B.cnt, ERR_BOUNDS); continue; }
-        s_key[l] = kv[e];
-        s_idx[l] = iv[e];
+        s_key[l] = S.kv[e];
+        s_idx[l] = (seg.flags & F_IDENT) ? seg.begin + S.cb + p : S.iv[e];
         s_bkt[l] = (uint16_t)b;
     }
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-        const uint32_t j = e * kBigWG + threadIdx.x;
+    for (int e = 0; e < kScatPer; ++e) {
+        const uint32_t j = e * kBigWG + tid;
         if (j >= m) continue;
         const uint32_t b = s_bkt[j];
         const uint32_t rel = s_base[b] + (j - s_start[b]);
         if (rel >= seg.size) { flag_error(B.cnt, ERR_BOUNDS); continue; }
         KD[seg.begin + rel] = s_key[j];
         ID[seg.begin + rel] = s_idx[j];
+    }
+}
+
+__global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* __restrict__ desc,
+                                                     const TileRef* __restrict__ tiles, Bufs B,
+                                                     const uint32_t* __restrict__ hscan, uint32_t nchunks)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const uint32_t per = gridDim.x / kXcds;      // workgroups per XCD (the grid is a multiple of kXcds)
+    const uint32_t xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+    const uint32_t chi = (uint32_t)((uint64_t)nchunks * (xcd + 1) / kXcds);
+    uint32_t c = (uint32_t)((uint64_t)nchunks * xcd / kXcds) + slot;
+    if (slot >= per || c >= chi) return;
+    // two register sets: one chunk is reordered while the next one loads
+    // (past the end of its range a workgroup fetches its last chunk again,
+    // unused, so that every iteration issues the same loads)
+    ScatChunk A, Bn;
+    scat_fetch(desc, tiles, B, hscan, c, A);
+    for (;;) {
+        scat_fetch(desc, tiles, B, hscan, min(c + per, chi - 1), Bn);
+        scat_chunk(A, B, smem);
+        c += per;
+        if (c >= chi) break;
+        scat_fetch(desc, tiles, B, hscan, min(c + per, chi - 1), A);
+        scat_chunk(Bn, B, smem);
+        c += per;
+        if (c >= chi) break;
     }
 }
 
@@ -1298,6 +1363,7 @@ struct ps5_ctx {
     uint64_t part_cap = 0;
     uint64_t* d_merge = nullptr;     // plan of a merge round
     uint64_t merge_cap = 0;
+    uint32_t scatter_wgs = 256;         // k_scatter workgroups: one per CU, a multiple of kXcds
     uint64_t* d_sets = nullptr;      // set boundaries of a batch
     uint64_t sets_cap = 0;
     unsigned long long* d_sums = nullptr;   // profiling: elements per kernel class
@@ -1574,8 +1640,10 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     {
         // bktrank 4 + key 8 + idx 4 R (no idx at a root), key 8 + idx 4 W
         Launch L(c, K_SCATTER, elems * 28 - elems_ident * 4 + hl * 8);
-        hipLaunchKernelGGL(k_scatter, dim3((uint32_t)ntiles * kWaves), dim3(kBigWG), kScatLds, st, c->d_desc,
-                           d_tiles, B, c->hist);
+        const uint32_t nchunks = (uint32_t)ntiles * kWaves;
+        const uint32_t wgs = std::min(c->scatter_wgs, div_up(nchunks, kXcds) * kXcds);
+        hipLaunchKernelGGL(k_scatter, dim3(wgs), dim3(kBigWG), kScatLds, st, c->d_desc, d_tiles, B, c->hist,
+                           nchunks);
     }
     {
         Launch L(c, K_CHILDREN, hl * 4 / kWaves);
@@ -1833,7 +1901,10 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     ps5_ctx* c = new ps5_ctx;
     c->device = device;
     c->trace = getenv("PS5_TRACE") != nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    int ncu = 0;
+    hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+    c->scatter_wgs = std::max<uint32_t>(kXcds, (uint32_t)std::max(ncu, 0) / kXcds * kXcds);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->d_cnt, C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_cnt), C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_upcnt), C_NCNT * sizeof(uint32_t));
