@@ -181,7 +181,7 @@ int ps5_sort_device_timed(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_b
 
 /* Shape parameters of a context's sorts (they never change the output):
  *   leaf_target    strings per bucket the multi-workgroup levels aim for,
- *                  in [64, 1024] (default 512); sets how many levels run and
+ *                  in [64, 1024] (default 768); sets how many levels run and
  *                  how large the on-chip blocks are;
  *   max_treebits   splitter-tree depth limit of one level, in [1, 10]
  *                  (default 10: up to 1023 splitters, 2047 buckets, the

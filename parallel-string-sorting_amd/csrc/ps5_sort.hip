@@ -1369,7 +1369,7 @@ struct ps5_ctx {
     unsigned long long* d_sums = nullptr;   // profiling: elements per kernel class
     unsigned long long* h_sums = nullptr;
     // tuning
-    uint32_t leaf_target = 512;      // strings per block the multi-workgroup levels aim for
+    uint32_t leaf_target = 768;      // strings per block the multi-workgroup levels aim for
     uint32_t max_treebits = kMaxTreebits;   // splitter-tree depth limit of one level
     uint32_t max_oversample = 32;    // sample keys per splitter (limited by kMaxSamples)
     uint64_t seed = 0x5EED5EEDull;

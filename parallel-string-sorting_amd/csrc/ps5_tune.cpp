@@ -43,13 +43,13 @@ int autotune(ps5_ctx* ctx, const std::function<int()>& sort_once, uint32_t* leaf
     };
     double best_t = 0;
     if ((rc = time_of(best, &best_t))) return rc;
-    static const uint32_t leafs[] = { 96, 128, 192, 256, 384, 512, 768 };
+    static const uint32_t leafs[] = { 96, 128, 192, 256, 384, 512, 768, 1024 };
     static const uint32_t bits[] = { 7, 8, 9, 10 };
     static const uint32_t overs[] = { 4, 8, 16, 32 };
     for (int pass = 0; pass < 2; ++pass) {
         for (int param = 0; param < 3; ++param) {
             const uint32_t* cand = param == 0 ? leafs : param == 1 ? bits : overs;
-            const int nc = param == 0 ? 7 : 4;
+            const int nc = param == 0 ? (int)(sizeof(leafs) / sizeof(leafs[0])) : 4;
             Tuning round_best = best;
             for (int i = 0; i < nc; ++i) {
                 Tuning t = best;

@@ -361,7 +361,7 @@ class Context:
                     b.free()
         return perm, lcp
 
-    def set_tuning(self, leaf_target=512, max_treebits=10, max_oversample=32):
+    def set_tuning(self, leaf_target=768, max_treebits=10, max_oversample=32):
         _check(load().ps5_ctx_set_tuning(self.h, leaf_target, max_treebits, max_oversample), "ps5_ctx_set_tuning")
 
     def get_tuning(self):
