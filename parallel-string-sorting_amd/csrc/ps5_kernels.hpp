@@ -35,8 +35,12 @@ constexpr int kMaxSpl = (1 << kMaxTreebits) - 1;    // 1023 splitters
 constexpr int kMaxBkt = 2 * kMaxSpl + 1;            // 2047 buckets
 constexpr int kMaxSamples = 4096;            // sample keys of one splitter tree
 constexpr uint32_t kTreeSlots = 2048;        // per segment: level-order tree [0,1024), sorted splitters [1024,2048)
-constexpr int kWaveChunk = 8192;             // keys ranked by one wave (multi-workgroup step)
-constexpr int kTile = kWaveChunk * kWaves;   // keys per count workgroup
+// A large segment is cut into T tiles of kWaves equal wave chunks; a wave
+// chunk (the unit of stable ranks and of the scatter's LDS reorder) holds at
+// most kChunkMax keys, and T is the fewest tiles that allows, so a level's
+// tiles are as few and as full as possible.
+constexpr int kChunkMax = 9216;              // keys ranked by one wave (multi-workgroup step)
+constexpr int kTileMax = kChunkMax * kWaves; // keys per count workgroup
 constexpr uint32_t kTinyT = 64;              // leaves sorted by one wave
 constexpr uint32_t kBlkT = 2048;             // leaves sorted on chip by one workgroup
 constexpr uint32_t kMaxLeafTarget = 1024;    // largest leaf size the fan-out aims at
@@ -77,7 +81,9 @@ constexpr uint32_t ERR_CAP = 1u, ERR_LCP = 2u, ERR_BOUNDS = 4u;
 
 struct LargeDesc {       // one segment of the multi-workgroup sample-sort step
     Item seg;
-    uint32_t T;          // tiles of kTile keys
+    uint32_t T;          // tiles of kWaves * chunk keys (the last one may be short)
+    uint32_t chunk;      // keys per wave chunk, <= kChunkMax
+    uint32_t pad;
     uint32_t hist_base;  // first histogram entry of this segment
     uint32_t k;          // treebits
     uint32_t oversample; // sample keys per splitter

@@ -158,8 +158,8 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
     for (int i = threadIdx.x; i < nspl; i += kBigWG) tree[1024 + i] = sorted[a / 2 + a * i];
 }
 
-// Classify one tile of kTile keys.  Each wave ranks its own chunk of
-// kWaveChunk keys stably (ballot matching, ranks follow input order) and the
+// Classify one tile of kWaves wave chunks.  Each wave ranks its own chunk
+// of D.chunk keys stably (ballot matching, ranks follow input order) and the
 // chunk counts go to hist[hist_base + (b * T + t) * kWaves + w]: bucket-major,
 // then tile, then wave -- the order of the reference's prefix sum
 // (.hpp:1437-1460) with wave chunks as the parts -- so an exclusive scan of
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     key64* s_tree = reinterpret_cast<key64*>(smem);
     key64* s_sorted = s_tree + (nb2 >> 1);
     uint16_t* s_hist_base = reinterpret_cast<uint16_t*>(s_sorted + (nb2 >> 1));
-    static_assert(kWaveChunk <= 65535, "per-wave bucket counts must fit 16 bits");
+    static_assert(kChunkMax <= 65535, "per-wave bucket counts must fit 16 bits");
     const TileRef tr = tiles[xcd_contiguous(blockIdx.x, gridDim.x)];
     const LargeDesc D = desc[tr.seg];
     const Item seg = D.seg;
@@ -201,8 +201,8 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     const uint32_t t = tr.t;
     const uint32_t w = wave_id(), lane = lane_id();
     uint16_t* s_hist = s_hist_base + w * nb2;      // this wave's counts
-    const uint32_t cb = t * kTile + w * kWaveChunk;
-    const uint32_t ce = (item_ok(seg, B) && t < D.T) ? min(seg.size, cb + kWaveChunk) : 0u;
+    const uint32_t cb = (t * kWaves + w) * D.chunk;
+    const uint32_t ce = (item_ok(seg, B) && t < D.T && D.chunk <= kChunkMax) ? min(seg.size, cb + D.chunk) : 0u;
     // kGroups 64-key groups per iteration: all loads are in flight before the
     // tree descents, then the groups are ranked in order (stable)
     constexpr int kGroups = 4;
@@ -332,14 +332,14 @@ __global__ __launch_bounds__(kWG) void k_scan_down(uint32_t* data, uint32_t len,
 
 // LDS layout of k_scatter (dynamic shared memory)
 constexpr uint32_t kScatKeyOff = 0;
-constexpr uint32_t kScatIdxOff = kScatKeyOff + kWaveChunk * 8;
-constexpr uint32_t kScatBktOff = kScatIdxOff + kWaveChunk * 4;
-constexpr uint32_t kScatStartOff = kScatBktOff + kWaveChunk * 2;
+constexpr uint32_t kScatIdxOff = kScatKeyOff + kChunkMax * 8;
+constexpr uint32_t kScatBktOff = kScatIdxOff + kChunkMax * 4;
+constexpr uint32_t kScatStartOff = kScatBktOff + kChunkMax * 2;
 constexpr uint32_t kScatBaseOff = kScatStartOff + 2048 * 4;
 constexpr uint32_t kScatTmpOff = kScatBaseOff + 2048 * 4;
 constexpr uint32_t kScatLds = kScatTmpOff + 64 * 4;
 
-// Move wave chunks (kWaveChunk keys each) to their buckets (distribute,
+// Move wave chunks (at most kChunkMax keys each) to their buckets (distribute,
 // .hpp:1464-1490, but stable).  A chunk is first reordered by bucket in
 // LDS, then written out position by position, so consecutive lanes store
 // consecutive addresses inside each bucket: full cache lines instead of the
@@ -352,8 +352,8 @@ constexpr uint32_t kScatLds = kScatTmpOff + 64 * 4;
 // kXcds contiguous ranges, one per XCD (workgroup g runs on XCD g % kXcds),
 // so the chunks in flight on an XCD are neighbours that share histogram
 // lines and bucket output lines in its L2.
-constexpr int kScatPer = kWaveChunk / kBigWG;
-static_assert(kWaveChunk % kBigWG == 0, "a chunk is a whole number of workgroup passes");
+constexpr int kScatPer = kChunkMax / kBigWG;
+static_assert(kChunkMax % kBigWG == 0, "a chunk fits a whole number of workgroup passes");
 
 // threadIdx.x through an opaque move: addresses derived from it are
 // recomputed per chunk instead of being hoisted out of the chunk loop,
@@ -388,9 +388,9 @@ __device__ __forceinline__ void scat_fetch(const LargeDesc* __restrict__ desc, c
     S.t = tr.t;
     S.D = desc[tr.seg];
     const Item seg = S.D.seg;
-    S.cb = S.t * kTile + S.w * kWaveChunk;
-    const bool ok = item_ok(seg, B) && S.t < S.D.T && S.cb < seg.size;
-    S.m = ok ? min(seg.size - S.cb, (uint32_t)kWaveChunk) : 0u;
+    S.cb = (S.t * kWaves + S.w) * S.D.chunk;
+    const bool ok = item_ok(seg, B) && S.t < S.D.T && S.D.chunk <= kChunkMax && S.cb < seg.size;
+    S.m = ok ? min(seg.size - S.cb, S.D.chunk) : 0u;
     const key64* K = (seg.flags & F_BUFB) ? B.key1 : B.key0;
     // a root's indices are implicit: its third load repeats the rank load
     const uint32_t* I = (seg.flags & F_IDENT) ? B.bktrank : (seg.flags & F_BUFB) ? B.idx1 : B.idx0;
@@ -417,8 +417,10 @@ __device__ __forceinline__ void scat_fetch(const LargeDesc* __restrict__ desc, c
     S.h1n = h[min(e1 + 1, last)];
 }
 
-// Reorder one prefetched chunk by bucket in LDS and write it out.
-__device__ __forceinline__ void scat_chunk(const ScatChunk& S, const Bufs& B, unsigned char* smem)
+// Stage one prefetched chunk in LDS, reordered by bucket (after the
+// previous chunk's image has been written out).  Returns false (uniformly)
+// for an empty or inconsistent chunk.
+__device__ __forceinline__ bool scat_stage(const ScatChunk& S, const Bufs& B, unsigned char* smem)
 {
     const uint32_t tid = scat_tid();
     key64* s_key = reinterpret_cast<key64*>(smem + kScatKeyOff);
@@ -429,11 +431,9 @@ __device__ __forceinline__ void scat_chunk(const ScatChunk& S, const Bufs& B, un
     uint32_t* s_tmp = reinterpret_cast<uint32_t*>(smem + kScatTmpOff);
     __syncthreads();     // the previous chunk's LDS image has been written out
     const uint32_t m = S.m;
-    if (m == 0) return;      // (uniform) an empty or malformed chunk
+    if (m == 0) return false;
     const Item seg = S.D.seg;
     const uint32_t nbkt = (2u << S.D.k) - 1;
-    key64* KD = (seg.flags & F_BUFB) ? B.key0 : B.key1;
-    uint32_t* ID = (seg.flags & F_BUFB) ? B.idx0 : B.idx1;
     const uint32_t segend = S.hseg + seg.size;
     const uint32_t last = ((nbkt - 1) * S.D.T + (S.D.T - 1)) * kWaves + (kWaves - 1);
     const uint32_t b0 = tid, b1 = tid + kBigWG;
@@ -457,7 +457,7 @@ __device__ __forceinline__ void scat_chunk(const ScatChunk& S, const Bufs& B, un
     __syncthreads();
     if (tot0 + tot1 != m) {   // the histogram must account for every key of the chunk
         if (tid == 0) flag_error(B.cnt, ERR_BOUNDS);
-        return;
+        return false;
     }
 #pragma unroll
     for (int e = 0; e < kScatPer; ++e) {
@@ -471,6 +471,20 @@ __device__ __forceinline__ void scat_chunk(const ScatChunk& S, const Bufs& B, un
         s_bkt[l] = (uint16_t)b;
     }
     __syncthreads();
+    return true;
+}
+
+// Write a staged chunk of m keys of segment seg out, position by position.
+__device__ __forceinline__ void scat_write(const Item& seg, uint32_t m, const Bufs& B, unsigned char* smem)
+{
+    const uint32_t tid = scat_tid();
+    const key64* s_key = reinterpret_cast<const key64*>(smem + kScatKeyOff);
+    const uint32_t* s_idx = reinterpret_cast<const uint32_t*>(smem + kScatIdxOff);
+    const uint16_t* s_bkt = reinterpret_cast<const uint16_t*>(smem + kScatBktOff);
+    const uint32_t* s_start = reinterpret_cast<const uint32_t*>(smem + kScatStartOff);
+    const uint32_t* s_base = reinterpret_cast<const uint32_t*>(smem + kScatBaseOff);
+    key64* KD = (seg.flags & F_BUFB) ? B.key0 : B.key1;
+    uint32_t* ID = (seg.flags & F_BUFB) ? B.idx0 : B.idx1;
 #pragma unroll
     for (int e = 0; e < kScatPer; ++e) {
         const uint32_t j = e * kBigWG + tid;
@@ -493,18 +507,18 @@ __global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* __restrict_
     const uint32_t chi = (uint32_t)((uint64_t)nchunks * (xcd + 1) / kXcds);
     uint32_t c = (uint32_t)((uint64_t)nchunks * xcd / kXcds) + slot;
     if (slot >= per || c >= chi) return;
-    // two register sets: one chunk is reordered while the next one loads
-    // (past the end of its range a workgroup fetches its last chunk again,
-    // unused, so that every iteration issues the same loads)
-    ScatChunk A, Bn;
-    scat_fetch(desc, tiles, B, hscan, c, A);
+    // one register set: once a chunk is staged in LDS its registers take the
+    // next chunk's loads, which overlap the write-out (past the end of its
+    // range a workgroup fetches its last chunk again, unused, so that every
+    // iteration issues the same loads)
+    ScatChunk R;
+    scat_fetch(desc, tiles, B, hscan, c, R);
     for (;;) {
-        scat_fetch(desc, tiles, B, hscan, min(c + per, chi - 1), Bn);
-        scat_chunk(A, B, smem);
-        c += per;
-        if (c >= chi) break;
-        scat_fetch(desc, tiles, B, hscan, min(c + per, chi - 1), A);
-        scat_chunk(Bn, B, smem);
+        const Item seg = R.D.seg;
+        const uint32_t m = R.m;
+        const bool staged = scat_stage(R, B, smem);
+        scat_fetch(desc, tiles, B, hscan, min(c + per, chi - 1), R);
+        if (staged) scat_write(seg, m, B, smem);
         c += per;
         if (c >= chi) break;
     }
@@ -1461,9 +1475,9 @@ int ensure_capacity(ps5_ctx* c, uint64_t n)
         HIPCHK(hipMalloc(&c->l_block[i], (uint64_t)kBlkClasses * c->cap_block * sizeof(Item)));
         HIPCHK(hipMalloc(&c->l_run[i], (uint64_t)c->cap_run * sizeof(Item)));
     }
-    // every large segment has at most size/kTile + 1 tiles; the splitter
+    // every large segment has at most size/kTileMax + 1 tiles; the splitter
     // trees and histograms of a level grow on demand (grow_level)
-    c->tiles_cap = cap / kTile + c->cap_large + 1;
+    c->tiles_cap = cap / kTileMax + c->cap_large + 1;
     const uint64_t plan_bytes = (uint64_t)c->cap_large * sizeof(LargeDesc) + c->tiles_cap * sizeof(TileRef);
     static_assert(sizeof(LargeDesc) % alignof(TileRef) == 0, "tile table must stay aligned after the descriptors");
     static_assert(sizeof(LargeDesc) % 8 == 0 && sizeof(TileRef) % 8 == 0, "the plan is fetched in 8-byte words");
@@ -1595,10 +1609,12 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     const TileRef* d_tiles = reinterpret_cast<const TileRef*>(c->d_desc + nlarge);
     for (uint32_t i = 0; i < nlarge; ++i) {
         const Item seg = c->h_large[i];
-        const uint32_t T = div_up(seg.size, kTile);
+        const uint32_t T = div_up(seg.size, kTileMax);
         LargeDesc& D = c->h_desc[i];
         D.seg = seg;
         D.T = T;
+        D.chunk = div_up(seg.size, (uint64_t)T * kWaves);   // <= kChunkMax: equal chunks, the last one short
+        D.pad = 0;
         D.hist_base = (uint32_t)hl;
         D.k = (uint32_t)large_treebits(seg.size, c->leaf_target, (int)c->max_treebits);
         const uint32_t nspl = (1u << D.k) - 1;
