@@ -156,6 +156,20 @@ def bench_batch(ps5, args):
     print(json.dumps(out), flush=True)
 
 
+def calibration_copy(ps5):
+    """Two plain 1 GiB device copies (k_copy16): the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    cb = 1 << 30
+    src = ps5.DeviceBuffer(cb)
+    dst = ps5.DeviceBuffer(cb)
+    for _ in range(2):
+        ps5._check(ps5.load().ps5_dev_copy(dst.ptr, src.ptr, cb), "ps5_dev_copy")
+    src.free()
+    dst.free()
+    print(f"[bench] calibration copy: k_copy16 moved {cb} bytes read + {cb} bytes written per launch",
+          file=sys.stderr, flush=True)
+
+
 def bench_merge(ps5, args):
     """LCP merge of K sorted shards (contiguous index ranges) of a workload."""
     import ctypes
@@ -171,6 +185,8 @@ def bench_merge(ps5, args):
     d_io, d_lo = ps5.DeviceBuffer(4 * n), ps5.DeviceBuffer(4 * n)
     d_arena.upload(arena)
     d_offs.upload(offs)
+    if args.calibrate:
+        calibration_copy(ps5)
     P = ctypes.c_void_p
     t0 = time.perf_counter()
     for b, e in zip(bounds[:-1], bounds[1:]):      # each shard sorted on its own (not timed below)
@@ -459,15 +475,7 @@ def main():
     alen = int(arena.size)
 
     if args.calibrate:
-        cb = 1 << 30
-        src = ps5.DeviceBuffer(cb)
-        dst = ps5.DeviceBuffer(cb)
-        for _ in range(2):
-            ps5._check(ps5.load().ps5_dev_copy(dst.ptr, src.ptr, cb), "ps5_dev_copy")
-        src.free()
-        dst.free()
-        print(f"[bench] calibration copy: k_copy16 moved {cb} bytes read + {cb} bytes written per launch",
-              file=sys.stderr, flush=True)
+        calibration_copy(ps5)
     tuning = ctx.get_tuning()
     if args.autotune:
         t0 = time.time()
