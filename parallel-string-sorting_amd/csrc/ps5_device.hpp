@@ -59,6 +59,38 @@ __device__ __forceinline__ key64 load_key(const uint8_t* arena, uint64_t alen, u
     return v;
 }
 
+// The keys at depths off, off+8, ..., off+8(KB-1) of one string: KB+1
+// aligned 8-byte loads issued together (load_key near the arena end).  Keys
+// past the string's NUL hold whatever follows it; callers use a key only
+// while every key before it was NUL-free.
+template <int KB>
+__device__ __forceinline__ void load_keys(const uint8_t* arena, uint64_t alen, uint64_t off, key64 (&k)[KB])
+{
+    const uint64_t a8 = off & ~7ull;
+    if (a8 + 8 * (KB + 1) > alen) {
+#pragma unroll
+        for (int i = 0; i < KB; ++i) k[i] = load_key(arena, alen, off + 8 * i);
+        return;
+    }
+    const g_u64* p = (const g_u64*)(arena + a8);
+    uint64_t w[KB + 1];
+#pragma unroll
+    for (int i = 0; i <= KB; ++i) w[i] = p[i];
+    const uint32_t sh = (uint32_t)(off & 7) * 8;
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+        uint64_t v = sh ? ((w[i] >> sh) | (w[i + 1] << (64 - sh))) : w[i];
+        v = __builtin_bswap64(v);
+        const uint64_t t = ((v & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | v;
+        const uint64_t z = ~t & 0x8080808080808080ull;
+        if (z) {
+            const uint32_t q = (uint32_t)__clzll((long long)z);
+            v = q ? (v & (~0ull << (64 - q))) : 0ull;
+        }
+        k[i] = v;
+    }
+}
+
 // characters the two keys share (lcpKeyType, bingmann-parallel_sample_sort.hpp:225-230)
 __device__ __forceinline__ uint32_t lcp_key(key64 a, key64 b)
 {

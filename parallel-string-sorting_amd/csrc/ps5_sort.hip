@@ -663,6 +663,10 @@ __device__ void wave_finish_leaf(const Bufs& B, uint32_t begin, uint32_t m, key6
         s_r[lane] = 0;
     }
     bool first_pass = write_bkey;
+    constexpr int kLeafKeys = 8;
+    key64 kbuf[kLeafKeys] = {};     // string bid's next keys (depths cur+8, cur+16, ...)
+    uint32_t bid = 0xFFFFFFFFu;
+    int nbuf = 0;
     for (;;) {
         // re-sort every live run by (key, slot); finished positions keep their slot
         wave_lds_sync();
@@ -717,14 +721,26 @@ __device__ void wave_finish_leaf(const Bufs& B, uint32_t begin, uint32_t m, key6
         }
         wave_lds_sync();
         if (act) s_r[lane] = live ? run : 0xFFFFFFFFu;
-        // the next 8 characters of every string still in a run
+        // the next 8 characters of every string still in a run: fetched
+        // kLeafKeys keys at a time, so deep shared prefixes cost one memory
+        // round trip per 8 * kLeafKeys characters.  A lane's buffer belongs
+        // to the string it was loaded for (bid); when a re-sort moved any
+        // string to another lane, the wave loads afresh.
         first_pass = false;
         cur += 8;
-        if (live) {
-            if (id < B.n) k = load_key(B.arena, B.alen, str_off(B, id) + cur);
-            else { flag_error(B.cnt, ERR_BOUNDS); k = 0; }
-            s_k[lane] = k;
+        if (nbuf == 0 || __ballot(live && id != bid) != 0) {     // (uniform)
+            if (live) {
+                if (id < B.n) load_keys<kLeafKeys>(B.arena, B.alen, str_off(B, id) + cur, kbuf);
+                else { flag_error(B.cnt, ERR_BOUNDS); for (int i = 0; i < kLeafKeys; ++i) kbuf[i] = 0; }
+                bid = id;
+            }
+            nbuf = kLeafKeys;
         }
+        k = kbuf[0];
+#pragma unroll
+        for (int i = 0; i + 1 < kLeafKeys; ++i) kbuf[i] = kbuf[i + 1];
+        --nbuf;
+        if (live) s_k[lane] = k;
     }
 }
 
