@@ -932,9 +932,14 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
         const bool nr = act && !eqp && eqn && (kk & 0xFFull) != 0;
         if (__ballot(nr) == 0) continue;     // (uniform) nothing to append in this wave
         uint32_t rl = 0;
-        if (nr) {
-            rl = 1;
-            while (i + rl < m && s_k[i + rl] == kk) ++rl;
+        if (nr) {     // the run's last position: binary search in the sorted keys
+            uint32_t lo = i + 1, hi = m;     // s_k[lo] == kk, s_k[hi] != kk (or hi == m)
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_k[mid] == kk) lo = mid;
+                else hi = mid;
+            }
+            rl = lo - i + 1;
         }
         const bool small = nr && rl <= kTinyT;
         const uint32_t slot = small ? atomicAdd(&s_nruns, 1u) : kMaxRuns;
