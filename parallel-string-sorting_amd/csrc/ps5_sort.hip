@@ -1116,8 +1116,10 @@ __device__ __forceinline__ bool str_leq_from(const uint8_t* arena, uint64_t alen
 // (lcp_merge_binary and lcp_compare, bingmann-lcp_mergesort_binary.hpp),
 // which compares characters only where the LCPs to the previous output do
 // not decide; the staged characters decide most of those compares without
-// touching the arena.  The first 8 characters of every output are written
-// beside it, so only the first round reads them from the arena.
+// touching the arena.  The first round takes the 8 characters from keys
+// extracted in input order by one streaming pass (a gather of 8 bytes per
+// string instead of an offset and an arena window); every round writes them
+// beside its outputs for the next.
 constexpr uint32_t kMergeE = 8;                    // outputs per thread
 constexpr uint32_t kMergeT = kWG * kMergeE;        // outputs per tile
 
@@ -1136,11 +1138,16 @@ __device__ __forceinline__ bool merge_leq(const Bufs& B, uint32_t a, uint32_t b,
     return str_leq_from(B.arena, B.alen, str_off(B, a), str_off(B, b), 8, lcp);
 }
 
-// first 8 characters of the string at position pos of the source
+// first 8 characters of the string at position pos of the source: from the
+// previous round's output, else from in_key (indexed by input index)
 __device__ __forceinline__ key64 merge_key(const Bufs& B, const uint32_t* src_idx, const key64* src_key,
-                                           uint64_t pos)
+                                           const key64* in_key, uint64_t pos)
 {
-    return src_key ? src_key[pos] : load_key(B.arena, B.alen, str_off(B, src_idx[pos]));
+    if (src_key) return src_key[pos];
+    const uint32_t id = src_idx[pos];
+    if (id < B.n) return in_key[id];
+    flag_error(B.cnt, ERR_BOUNDS);
+    return 0;
 }
 
 // the pair holding tile g: the largest p with tstart[p] <= g
@@ -1157,6 +1164,7 @@ __device__ __forceinline__ uint32_t merge_pair(const uint64_t* tstart, uint32_t 
 
 // co[g] = strings of A (of g's pair) that precede the first output of tile g
 __global__ __launch_bounds__(kWG) void k_merge_part(Bufs B, const uint32_t* src_idx, const key64* src_key,
+                                                    const key64* in_key,
                                                     const uint64_t* first, const uint64_t* mid,
                                                     const uint64_t* tstart, uint32_t npairs, uint32_t* co)
 {
@@ -1171,8 +1179,8 @@ __global__ __launch_bounds__(kWG) void k_merge_part(Bufs B, const uint32_t* src_
         const uint64_t mi = (i0 + i1) / 2;
         const uint64_t pa = a0 + mi, pb = b0 + diag - 1 - mi;
         uint32_t l;
-        if (merge_leq(B, src_idx[pa], src_idx[pb], merge_key(B, src_idx, src_key, pa),
-                      merge_key(B, src_idx, src_key, pb), &l))
+        if (merge_leq(B, src_idx[pa], src_idx[pb], merge_key(B, src_idx, src_key, in_key, pa),
+                      merge_key(B, src_idx, src_key, in_key, pb), &l))
             i0 = mi + 1;
         else
             i1 = mi;
@@ -1181,7 +1189,8 @@ __global__ __launch_bounds__(kWG) void k_merge_part(Bufs B, const uint32_t* src_
 }
 
 __global__ __launch_bounds__(kWG) void k_merge_tile(Bufs B, const uint32_t* src_idx, const uint32_t* src_lcp,
-                                                    const key64* src_key, uint32_t* dst_idx, uint32_t* dst_lcp,
+                                                    const key64* src_key, const key64* in_key,
+                                                    uint32_t* dst_idx, uint32_t* dst_lcp,
                                                     key64* dst_key, const uint64_t* first, const uint64_t* mid,
                                                     const uint64_t* tstart, uint32_t npairs, const uint32_t* co)
 {
@@ -1215,7 +1224,7 @@ __global__ __launch_bounds__(kWG) void k_merge_tile(Bufs B, const uint32_t* src_
         if (valid) {
             s_idx[k] = src_idx[pos];
             s_lcp[k] = src_lcp[pos];
-            s_key[k] = merge_key(B, src_idx, src_key, pos);
+            s_key[k] = merge_key(B, src_idx, src_key, in_key, pos);
         }
     }
     __syncthreads();
@@ -2050,7 +2059,17 @@ int ps5_merge_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, c
     std::vector<uint64_t> bounds(run_first, run_first + nruns + 1);
     const uint32_t* si = d_idx;
     const uint32_t* sl = d_lcp;
-    const key64* sk = nullptr;     // the first round reads characters from the arena
+    // the first round reads every string's first 8 characters from ik, keys
+    // extracted in input order by one streaming pass (the previous round's
+    // output carries them afterwards); key[1] is free until round 2 writes it
+    key64* ik = c->key[1];
+    {
+        Bufs BI = B;
+        BI.key0 = ik;
+        hipLaunchKernelGGL(k_init, dim3(div_up(n, kWG)), dim3(kWG), 0, st, BI, (uint32_t)n);
+        HIPCHK(hipGetLastError());
+    }
+    const key64* sk = nullptr;
     int cur = 0;
     while (bounds.size() > 2) {
         const uint32_t nr = (uint32_t)bounds.size() - 1, np = (nr + 1) / 2;
@@ -2084,10 +2103,10 @@ int ps5_merge_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, c
             const uint64_t* pm = pf + (np + 1);
             const uint64_t* pt = pm + (np + 1);
             uint32_t* co = reinterpret_cast<uint32_t*>(c->d_merge + plan.size());
-            hipLaunchKernelGGL(k_merge_part, dim3(div_up(tiles, kWG)), dim3(kWG), 0, st, B, si, sk, pf, pm, pt, np,
+            hipLaunchKernelGGL(k_merge_part, dim3(div_up(tiles, kWG)), dim3(kWG), 0, st, B, si, sk, ik, pf, pm, pt, np,
                                co);
             HIPCHK(hipGetLastError());
-            hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kWG), 0, st, B, si, sl, sk, di, dl, dk, pf, pm, pt,
+            hipLaunchKernelGGL(k_merge_tile, dim3(tiles), dim3(kWG), 0, st, B, si, sl, sk, ik, di, dl, dk, pf, pm, pt,
                                np, co);
             HIPCHK(hipGetLastError());
         }
