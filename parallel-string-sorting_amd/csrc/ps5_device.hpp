@@ -188,12 +188,13 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t bid, uint32_t G)
 }
 
 // lanes holding the same 11-bit bucket id (all 64 lanes must execute this)
+template <int NB = 11>
 __device__ __forceinline__ uint64_t match_bucket(uint32_t b)
 {
     const uint64_t act = __ballot(1);
     uint32_t lo = (uint32_t)act, hi = (uint32_t)(act >> 32);
 #pragma unroll
-    for (int i = 0; i < 11; ++i) {
+    for (int i = 0; i < NB; ++i) {
         // m: all ones if bit i of b is set, else zero; lanes agreeing with
         // this lane on bit i are ~(ballot ^ m), computed as (~ballot) ^ m
         const uint32_t m = (uint32_t)((int32_t)(b << (31 - i)) >> 31);

@@ -54,7 +54,6 @@ __host__ __device__ constexpr int block_class(uint32_t m)
 }
 constexpr int kRankBits = 21;
 constexpr uint32_t kRankMask = (1u << kRankBits) - 1;
-constexpr uint32_t kNoBucket = 0x7FF;        // bucket id of inactive lanes
 
 struct Item {            // a segment or a leaf
     uint32_t begin;      // first position

@@ -174,9 +174,14 @@ __host__ __device__ constexpr uint32_t count_lds_bytes(int kmax)
     return (2u << kmax) * 8u + kWaves * (2u << kmax) * 2u;
 }
 
+// NB: bits of a bucket id in this launch (kmax + 1, at least 8): the ballot
+// matching of ranks costs a few vector instructions per bit.
+template <int NB>
 __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const TileRef* tiles, Bufs B,
                                                 const key64* trees, uint32_t* hist, int from_arena, int kmax)
 {
+    static_assert(NB >= 2 && NB <= kMaxTreebits + 1, "bucket ids of up to 2047 buckets");
+    constexpr uint32_t kNoBkt = (1u << NB) - 1;   // id of inactive lanes, above every bucket of the launch
     extern __shared__ __align__(16) unsigned char smem[];
     const uint32_t nb2 = 2u << kmax;               // bucket slots per wave
     key64* s_tree = reinterpret_cast<key64*>(smem);
@@ -187,7 +192,7 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     const LargeDesc D = desc[tr.seg];
     const Item seg = D.seg;
     const int k = (int)D.k, nbkt = (2 << k) - 1;
-    if (k > kmax) {   // the launch must size LDS for every tree of the level
+    if (k > kmax || k + 1 > NB) {   // the launch must size LDS and ids for every tree of the level
         if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
@@ -244,12 +249,12 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
 #pragma unroll
         for (int j = 0; j < kGroups; ++j) {
             const uint32_t p = g + 64 * j + lane;
-            if (p >= ce) bv[j] = kNoBucket;
+            if (p >= ce) bv[j] = kNoBkt;
         }
 #pragma unroll
         for (int j = 0; j < kGroups; ++j) {
             const uint32_t p = g + 64 * j + lane;
-            const uint64_t peers = match_bucket(bv[j]);
+            const uint64_t peers = match_bucket<NB>(bv[j]);
             const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
             if (p < ce) {
                 const uint32_t old = s_hist[bv[j]];
@@ -1673,8 +1678,16 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     {
         // key 8 R + bktrank 4 W; extracting at the root: offs 4 R + 8 characters R + key 8 W + bktrank 4 W
         Launch L(c, K_COUNT, (from_arena ? elems * 24 : elems * 12) + hl * 4);
-        hipLaunchKernelGGL(k_count, dim3((uint32_t)ntiles), dim3(kWG), count_lds_bytes(kmax), st, c->d_desc,
-                           d_tiles, B, c->trees, c->hist, (int)from_arena, kmax);
+        const dim3 g((uint32_t)ntiles), b(kWG);
+        const uint32_t lds = count_lds_bytes(kmax);
+        if (kmax <= 7)
+            hipLaunchKernelGGL(k_count<8>, g, b, lds, st, c->d_desc, d_tiles, B, c->trees, c->hist, (int)from_arena, kmax);
+        else if (kmax == 8)
+            hipLaunchKernelGGL(k_count<9>, g, b, lds, st, c->d_desc, d_tiles, B, c->trees, c->hist, (int)from_arena, kmax);
+        else if (kmax == 9)
+            hipLaunchKernelGGL(k_count<10>, g, b, lds, st, c->d_desc, d_tiles, B, c->trees, c->hist, (int)from_arena, kmax);
+        else
+            hipLaunchKernelGGL(k_count<11>, g, b, lds, st, c->d_desc, d_tiles, B, c->trees, c->hist, (int)from_arena, kmax);
     }
     {
         const uint32_t nb = div_up(hl32, kScanBlock);
@@ -1942,8 +1955,9 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     *out = nullptr;
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipFuncSetAttribute((const void*)k_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, kScatLds));
-    HIPCHK(hipFuncSetAttribute((const void*)k_count, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               count_lds_bytes(kMaxTreebits)));
+    for (const void* f : { (const void*)k_count<8>, (const void*)k_count<9>, (const void*)k_count<10>,
+                           (const void*)k_count<11> })
+        HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, count_lds_bytes(kMaxTreebits)));
     ps5_ctx* c = new ps5_ctx;
     c->device = device;
     c->trace = getenv("PS5_TRACE") != nullptr;
