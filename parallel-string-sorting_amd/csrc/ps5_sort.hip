@@ -654,18 +654,24 @@ __device__ __forceinline__ void wave_lds_sync()
 // (lanes >= m ignored).  s_k, s_i, s_r: this wave's 64-entry LDS arrays.
 // write_bkey: record the round-depth keys of the two end positions for the
 // leaf-boundary pass.  All 64 lanes must call this.
-__device__ void wave_finish_leaf(const Bufs& B, uint32_t begin, uint32_t m, key64 k, uint32_t id, uint32_t cur,
-                                 bool write_bkey, key64* s_k, uint32_t* s_i, uint32_t* s_r)
+//
+// The wave may also finish several independent runs at once (packed runs of
+// a block): slots [run0, run0 + length) of each run are contiguous, `pos` is
+// the position this lane's slot stands for (consecutive within a run), and
+// every lane of a run passes the run's first slot as run0.  A single leaf is
+// run0 = 0, pos = begin + lane.
+__device__ void wave_finish_leaf(const Bufs& B, uint32_t pos, uint32_t m, key64 k, uint32_t id, uint32_t cur,
+                                 bool write_bkey, uint32_t run0, key64* s_k, uint32_t* s_i, uint32_t* s_r)
 {
     const uint32_t lane = lane_id();
     const bool act = lane < m;
-    // every string of the leaf starts in run 0; (key, slot) orders a run stably
-    uint32_t run = 0;
+    // every string starts in its run; (key, slot) orders a run stably
+    uint32_t run = run0;
     bool live = act;         // position still has an undecided successor relation
     wave_lds_sync();
     if (act) {
         s_k[lane] = k;
-        s_r[lane] = 0;
+        s_r[lane] = run0;
     }
     bool first_pass = write_bkey;
     constexpr int kLeafKeys = 8;
@@ -707,7 +713,6 @@ __device__ void wave_finish_leaf(const Bufs& B, uint32_t begin, uint32_t m, key6
         }
         const bool term = (k & 0xFFull) == 0;
         if (act && live) {
-            const uint32_t pos = begin + lane;
             if (first_pass && (lane == 0 || lane == m - 1)) B.bkey[pos] = k;
             if (same_prev && !eqp && B.lcp) B.lcp[pos] = cur + lcp_key(pk, k);
             if (eqp && term && B.lcp) B.lcp[pos] = cur + key_depth(k);
@@ -777,7 +782,7 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
         k = src ? B.key1[L.begin + lane] : B.key0[L.begin + lane];
         id = (L.flags & F_IDENT) ? L.begin + lane : src ? B.idx1[L.begin + lane] : B.idx0[L.begin + lane];
     }
-    wave_finish_leaf(B, L.begin, m, k, id, depth, true, s_k[w], s_i[w], s_r[w]);
+    wave_finish_leaf(B, L.begin + lane, m, k, id, depth, true, 0u, s_k[w], s_i[w], s_r[w]);
 }
 
 // Sorting network over E (key, position) pairs in one thread's registers
@@ -805,6 +810,67 @@ __device__ __forceinline__ void thread_sort(key64 (&k)[E], uint32_t (&p)[E])
     }
 }
 
+constexpr uint32_t kBlockRuns = 64;     // short runs a block finishes itself
+
+// Finish a block's short runs of equal keys (each at most 64 strings, at
+// sorted positions beg + (s_runs[r] >> 16)) at depth + 8: the runs are
+// packed into batches of at most 64 strings, and a wave finishes all runs of
+// a batch in one pass (duplicates come in many short runs).  The batch
+// table lives in s_k after the waves' leaf arrays.
+__device__ __forceinline__ void block_finish_runs(const Bufs& B, key64* s_k, const uint32_t* s_runs,
+                                                  uint32_t nruns, uint32_t beg, uint32_t depth, uint32_t nwaves)
+{
+    const uint32_t tid = threadIdx.x, w = wave_id(), lane = lane_id();
+    // the sorted block is no longer needed: each wave's leaf arrays live in
+    // s_k, and after them the run batches
+    uint32_t* s_rslot = reinterpret_cast<uint32_t*>(s_k + nwaves * 128);      // first slot of a run in its batch
+    uint32_t* s_bfirst = s_rslot + kBlockRuns;                                 // first run of each batch
+    uint32_t* s_nbatch_p = s_bfirst + kBlockRuns + 1;
+    // pack the runs into batches of at most 64 strings: a wave finishes all
+    // runs of a batch in one pass (duplicates come in many short runs)
+    if (tid == 0) {
+        uint32_t nb = 0, fill = 64;
+        for (uint32_t r = 0; r < nruns; ++r) {
+            const uint32_t len = s_runs[r] & 0xFFFFu;
+            if (fill + len > 64) {
+                s_bfirst[nb++] = r;
+                fill = 0;
+            }
+            s_rslot[r] = fill;
+            fill += len;
+        }
+        s_bfirst[nb] = nruns;
+        *s_nbatch_p = nb;
+    }
+    __syncthreads();
+    key64* wk = s_k + w * 128;
+    uint32_t* wi = reinterpret_cast<uint32_t*>(wk + 64);
+    uint32_t* wr = wi + 64;
+    const uint32_t nbatch = *s_nbatch_p;
+    for (uint32_t bt = w; bt < nbatch; bt += nwaves) {
+        const uint32_t r0 = s_bfirst[bt], r1 = s_bfirst[bt + 1];
+        uint32_t slot0 = 0, total = 0, rs = 0;
+        bool mine = false;
+        for (uint32_t r = r0; r < r1; ++r) {     // the run whose slots hold this lane
+            const uint32_t so = s_rslot[r], len = s_runs[r] & 0xFFFFu;
+            if (lane >= so && lane < so + len) {
+                mine = true;
+                slot0 = so;
+                rs = s_runs[r] >> 16;
+            }
+            total = max(total, so + len);
+        }
+        key64 kk = 0;
+        uint32_t id = 0, pos = 0;
+        if (mine) {
+            pos = beg + rs + (lane - slot0);
+            kk = B.key0[pos];
+            id = B.idx0[pos];
+        }
+        wave_finish_leaf(B, pos, total, kk, id, depth + 8, false, slot0, wk, wi, wr);
+    }
+}
+
 // One workgroup of MAXM / E threads sorts a block of 65..MAXM strings
 // completely on chip and finishes every position at this depth (the
 // reference's insertion-sort base case, lcp_inssort.hpp:36-187, and the
@@ -824,10 +890,11 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
     static_assert(T % 64 == 0 && MAXM <= 65536, "whole waves; positions fit 16 bits");
     __shared__ key64 s_k[MAXM];
     __shared__ uint16_t s_p[MAXM];
-    constexpr uint32_t kMaxRuns = 64;            // short equal runs finished here (more go to the next round)
+    constexpr uint32_t kMaxRuns = kBlockRuns;    // short equal runs finished here (more go to the next round)
     __shared__ uint32_t s_runs[kMaxRuns];        // (start << 16) | length
     __shared__ uint32_t s_nruns;
-    static_assert(MAXM * 8 >= 1024 * (T / 64), "the per-wave leaf arrays (1 KB each) reuse s_k");
+    static_assert(MAXM * 8 >= 1024 * (T / 64) + 4 * (2 * kMaxRuns + 2),
+                  "the per-wave leaf arrays (1 KB each) and the run batches reuse s_k");
     const uint32_t idx = first + blockIdx.x;
     if (idx >= *cnt_items) return;
     const Item L = items[idx];
@@ -954,21 +1021,9 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
         wave_append_segment(nr && !keep, ni, nxt);
     }
     __syncthreads();     // the runs' keys and indices written above are visible to the whole group
-    const uint32_t nruns = min(s_nruns, kMaxRuns), w = wave_id(), lane = lane_id();
-    // the sorted block is no longer needed: each wave's leaf arrays live in s_k
-    key64* wk = s_k + w * 128;
-    uint32_t* wi = reinterpret_cast<uint32_t*>(wk + 64);
-    uint32_t* wr = wi + 64;
-    for (uint32_t r = w; r < nruns; r += T / 64) {
-        const uint32_t rs = s_runs[r] >> 16, rl = s_runs[r] & 0xFFFFu;
-        key64 kk = 0;
-        uint32_t id = 0;
-        if (lane < rl) {
-            kk = B.key0[beg + rs + lane];
-            id = B.idx0[beg + rs + lane];
-        }
-        wave_finish_leaf(B, beg + rs, rl, kk, id, depth + 8, false, wk, wi, wr);
-    }
+    const uint32_t nruns = min(s_nruns, kMaxRuns);
+    if (nruns == 0) return;     // (uniform)
+    block_finish_runs(B, s_k, s_runs, nruns, beg, depth, T / 64);
 }
 
 // One workgroup per chunk of an equal-key leaf (all keys equal).
