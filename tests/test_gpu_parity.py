@@ -112,6 +112,26 @@ def test_highlcp_scaled(ps5, ctx, oracle):
     assert lcp.max() == 100
 
 
+@pytest.mark.parametrize("copies", [2, 3, 7, 40, 70])
+def test_duplicate_groups_in_blocks(ctx, oracle, copies):
+    """many small groups of identical long strings that share 8-character
+    prefixes with others: blocks hold many short runs of equal keys (packed
+    into shared wave passes, up to 64 per block) and runs longer than a wave;
+    identical strings must keep input order"""
+    say("duplicate groups", copies)
+    rng = np.random.default_rng(100 + copies)
+    base = [bytes(rng.choice(list(b"ACGT"), int(rng.integers(24, 90))).tolist()) for _ in range(120000 // copies)]
+    strs = [b for b in base for _ in range(copies)]
+    order = rng.permutation(len(strs))
+    strs = [strs[i] for i in order]
+    arena, offs = pack(strs)
+    perm, lcp = ctx.sort(arena, offs)
+    r, msg = oracle.verify(arena, offs, perm, lcp)
+    assert r == 0, msg
+    op, ol = oracle.ps5_sort(arena, offs)
+    assert np.array_equal(perm, op) and np.array_equal(lcp[1:], ol[1:])
+
+
 def test_repeated_calls_shrinking(ctx, oracle):
     """buffers are reused across calls; results must not depend on history"""
     for n, seed in [(300000, 1), (5000, 2), (70000, 3), (10, 4)]:
