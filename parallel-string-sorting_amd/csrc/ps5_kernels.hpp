@@ -37,10 +37,14 @@ constexpr int kMaxSamples = 4096;            // sample keys of one splitter tree
 constexpr uint32_t kTreeSlots = 2048;        // per segment: level-order tree [0,1024), sorted splitters [1024,2048)
 // A large segment is cut into T tiles of kWaves equal wave chunks; a wave
 // chunk (the unit of stable ranks and of the scatter's LDS reorder) holds at
-// most kChunkMax keys, and T is the fewest tiles that allows, so a level's
-// tiles are as few and as full as possible.
+// most a per-level limit of keys, and T is the fewest tiles that allows, so
+// a level's tiles are as few and as full as possible.  The limit is
+// kChunkMax unless the level is too small to give kMinTiles tiles that way;
+// then it shrinks (not below kChunkMin) so small sorts still spread over
+// the GPU.
 constexpr int kChunkMax = 9216;              // keys ranked by one wave (multi-workgroup step)
-constexpr int kTileMax = kChunkMax * kWaves; // keys per count workgroup
+constexpr int kChunkMin = 2048;
+constexpr uint32_t kMinTiles = 512;
 constexpr uint32_t kTinyT = 64;              // leaves sorted by one wave
 constexpr uint32_t kBlkT = 2048;             // leaves sorted on chip by one workgroup
 constexpr uint32_t kMaxLeafTarget = 1024;    // largest leaf size the fan-out aims at

@@ -1565,9 +1565,9 @@ int ensure_capacity(ps5_ctx* c, uint64_t n)
         HIPCHK(hipMalloc(&c->l_block[i], (uint64_t)kBlkClasses * c->cap_block * sizeof(Item)));
         HIPCHK(hipMalloc(&c->l_run[i], (uint64_t)c->cap_run * sizeof(Item)));
     }
-    // every large segment has at most size/kTileMax + 1 tiles; the splitter
-    // trees and histograms of a level grow on demand (grow_level)
-    c->tiles_cap = cap / kTileMax + c->cap_large + 1;
+    // every large segment has at most size/(kWaves * kChunkMin) + 1 tiles;
+    // the splitter trees and histograms of a level grow on demand (grow_level)
+    c->tiles_cap = cap / ((uint64_t)kWaves * kChunkMin) + c->cap_large + 1;
     const uint64_t plan_bytes = (uint64_t)c->cap_large * sizeof(LargeDesc) + c->tiles_cap * sizeof(TileRef);
     static_assert(sizeof(LargeDesc) % alignof(TileRef) == 0, "tile table must stay aligned after the descriptors");
     static_assert(sizeof(LargeDesc) % 8 == 0 && sizeof(TileRef) % 8 == 0, "the plan is fetched in 8-byte words");
@@ -1697,13 +1697,16 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     if (nlarge > c->cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
     TileRef* h_tiles = reinterpret_cast<TileRef*>(c->h_desc + nlarge);
     const TileRef* d_tiles = reinterpret_cast<const TileRef*>(c->d_desc + nlarge);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < nlarge; ++i) total += c->h_large[i].size;
+    const uint64_t climit = std::min<uint64_t>(kChunkMax, std::max<uint64_t>(kChunkMin, div_up(total, kWaves * kMinTiles)));
     for (uint32_t i = 0; i < nlarge; ++i) {
         const Item seg = c->h_large[i];
-        const uint32_t T = div_up(seg.size, kTileMax);
+        const uint32_t T = div_up(seg.size, climit * kWaves);
         LargeDesc& D = c->h_desc[i];
         D.seg = seg;
         D.T = T;
-        D.chunk = div_up(seg.size, (uint64_t)T * kWaves);   // <= kChunkMax: equal chunks, the last one short
+        D.chunk = div_up(seg.size, (uint64_t)T * kWaves);   // <= climit: equal chunks, the last one short
         D.pad = 0;
         D.hist_base = (uint32_t)hl;
         D.k = (uint32_t)large_treebits(seg.size, c->leaf_target, (int)c->max_treebits);
