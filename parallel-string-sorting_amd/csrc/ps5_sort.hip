@@ -745,6 +745,23 @@ __device__ void wave_finish_leaf(const Bufs& B, uint32_t pos, uint32_t m, key64 
                 bid = id;
             }
             nbuf = kLeafKeys;
+            // depths where, in every run, all strings agree with the run's
+            // first one and none ends decide nothing: skip them together
+            // (identical strings then cost one pass per 56 characters)
+            int skip = kLeafKeys - 1;
+#pragma unroll
+            for (int i = 0; i < kLeafKeys - 1; ++i) {
+                const uint32_t lo = __shfl((uint32_t)kbuf[i], (int)run, 64);
+                const uint32_t hi = __shfl((uint32_t)(kbuf[i] >> 32), (int)run, 64);
+                const bool same = kbuf[i] == (((uint64_t)hi << 32) | lo) && (kbuf[i] & 0xFFull) != 0;
+                if (skip == kLeafKeys - 1 && __ballot(live && !same) != 0) skip = i;     // (uniform)
+            }
+            for (int s = 0; s < skip; ++s) {     // (uniform)
+#pragma unroll
+                for (int i = 0; i + 1 < kLeafKeys; ++i) kbuf[i] = kbuf[i + 1];
+            }
+            nbuf -= skip;
+            cur += 8 * (uint32_t)skip;
         }
         k = kbuf[0];
 #pragma unroll
