@@ -313,22 +313,13 @@ __device__ __forceinline__ int inorder_of(int t, int k)
     return ((2 * j + 1) << (k - l - 1)) - 1;
 }
 
-// Bucket of a key: b = 2 * lower_bound + [splitter == key]
+// Buckets of G keys: b = 2 * lower_bound + [splitter == key]
 // (ClassifyTreeSimple::find_bkt, bingmann-sample_sortBTC.hpp:45-72).  The
 // descent i = 2i + (key > tree[i]) over the level-order tree ends at the
 // number of splitters below the key; the equality test reads that splitter
 // from the sorted array (one load instead of tracking a candidate per level).
-__device__ __forceinline__ uint32_t classify_key(key64 key, const key64* s_tree, const key64* s_sorted, int k)
-{
-    uint32_t i = 1;
-    for (int l = 0; l < k; ++l) i = 2 * i + (key > s_tree[i] ? 1u : 0u);
-    i -= 1u << k;
-    const uint32_t nspl = (1u << k) - 1;
-    return 2 * i + ((i < nspl && s_sorted[i] == key) ? 1u : 0u);
-}
-
-// classify_key for G keys at once: the G descents advance level by level
-// together, so their LDS reads are in flight at the same time.
+// The G descents advance level by level together, so their LDS reads are in
+// flight at the same time.
 template <int G>
 __device__ __forceinline__ void classify_keys(const key64 (&key)[G], const key64* s_tree, const key64* s_sorted,
                                               int k, uint32_t (&b)[G])
