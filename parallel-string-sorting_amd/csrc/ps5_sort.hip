@@ -2150,7 +2150,7 @@ int ps5_merge_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, c
     const uint32_t* sl = d_lcp;
     // the first round reads every string's first 8 characters from ik, keys
     // extracted in input order by one streaming pass (the previous round's
-    // output carries them afterwards); key[1] is free until round 2 writes it
+    // output carries them afterwards); key[1] is free until the second merge round writes it
     key64* ik = c->key[1];
     {
         Bufs BI = B;
