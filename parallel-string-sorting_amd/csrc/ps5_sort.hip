@@ -1446,6 +1446,8 @@ enum { S_BLOCK, S_TINY, S_RUN, S_NUM };
 struct ps5_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;         // a round's leaves run here beside its last blocks
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint64_t cap = 0;
     key64* key[2] = { nullptr, nullptr };
     uint32_t* idx[2] = { nullptr, nullptr };
@@ -1852,6 +1854,8 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
     return PS5_OK;
 }
 
+constexpr uint32_t kForkLeafGroups = 8192;   // leaf workgroups worth a side stream
+
 int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* offs, const uint64_t* offs64,
              uint32_t n, uint32_t* perm, uint32_t* lcp, const uint64_t* set_first, uint32_t nsets)
 {
@@ -1904,6 +1908,27 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, mask);
         }
         const Sink nsink = make_sink(c, c->l_round[np], C_LROUND + np, np);
+        // tiny leaves and equal-key chunks of this round, on stream s
+        auto launch_leaves = [&](hipStream_t s) {
+            const uint32_t nt = c->h_cnt[C_TINY + rp], nr = c->h_cnt[C_RUN + rp];
+            if (nt) {
+                {
+                    Launch L(c, K_LEAF_TINY, 0);
+                    hipLaunchKernelGGL(k_leaf_tiny, dim3(div_up(nt, kWaves)), dim3(kWG), 0, s, c->l_tiny[rp],
+                                       c->d_cnt + C_TINY + rp, B, depth);
+                }
+                sum_sizes(c, c->l_tiny[rp], 0, c->d_cnt + C_TINY + rp, nt, S_TINY);
+            }
+            if (nr) {
+                {
+                    Launch L(c, K_LEAF_RUN, 0);
+                    hipLaunchKernelGGL(k_leaf_run, dim3(nr), dim3(kWG), 0, s, c->l_run[rp], c->d_cnt + C_RUN + rp,
+                                       B, nsink, depth);
+                }
+                sum_sizes(c, c->l_run[rp], 0, c->d_cnt + C_RUN + rp, nr, S_RUN);
+            }
+        };
+        bool leaves_forked = false;
         uint32_t blk_done[kBlkClasses] = {};
         int cur_ci = C_LROUND + rp;
         for (uint32_t level = 0;; ++level) {
@@ -1916,6 +1941,18 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
             }
             if (nlarge == 0 && nblk_all == 0) break;
             c->st_levels++;
+            // the last level (blocks only): the round's other leaves are all
+            // listed, and when they are many they run on the side stream
+            // beside the blocks (not when profiling, whose events time the
+            // main stream; for few leaves the fork costs more than it hides)
+            if (nlarge == 0 && !c->profile &&
+                div_up(c->h_cnt[C_TINY + rp], kWaves) + c->h_cnt[C_RUN + rp] >= kForkLeafGroups) {
+                HIPCHK(hipEventRecord(c->ev_fork, st));
+                HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+                launch_leaves(c->side);
+                HIPCHK(hipEventRecord(c->ev_join, c->side));
+                leaves_forked = true;
+            }
             Item* nx_large = c->l_lvl[level & 1];
             const int nx_ci = C_LLVL + (level & 1);
             {
@@ -1965,22 +2002,8 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                     round, depth, nt, c->h_cnt[C_BLOCK + rp], c->h_cnt[C_BLOCK + 2 + rp], c->h_cnt[C_BLOCK + 4 + rp],
                     c->h_cnt[C_BLOCK + 6 + rp], nr, (unsigned long long)c->st_levels);
         }
-        if (nt) {
-            {
-                Launch L(c, K_LEAF_TINY, 0);
-                hipLaunchKernelGGL(k_leaf_tiny, dim3(div_up(nt, kWaves)), dim3(kWG), 0, st, c->l_tiny[rp],
-                                   c->d_cnt + C_TINY + rp, B, depth);
-            }
-            sum_sizes(c, c->l_tiny[rp], 0, c->d_cnt + C_TINY + rp, nt, S_TINY);
-        }
-        if (nr) {
-            {
-                Launch L(c, K_LEAF_RUN, 0);
-                hipLaunchKernelGGL(k_leaf_run, dim3(nr), dim3(kWG), 0, st, c->l_run[rp], c->d_cnt + C_RUN + rp, B,
-                                   nsink, depth);
-            }
-            sum_sizes(c, c->l_run[rp], 0, c->d_cnt + C_RUN + rp, nr, S_RUN);
-        }
+        if (leaves_forked) HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
+        else launch_leaves(st);
         if (nt + nb + nr && lcp) {
             Launch L(c, K_LEAFBOUND, (uint64_t)(nt + nb + nr) * 36);
             LeafLists LL;
@@ -2040,6 +2063,9 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
     c->scatter_wgs = std::max<uint32_t>(kXcds, (uint32_t)std::max(ncu, 0) / kXcds * kXcds);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&c->d_cnt, C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_cnt), C_NCNT * sizeof(uint32_t));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_upcnt), C_NCNT * sizeof(uint32_t));
@@ -2074,7 +2100,11 @@ void ps5_ctx_destroy(ps5_ctx* c)
     if (c->h_sums) (void)hipHostFree(c->h_sums);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto& u : c->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
+    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     delete c;
 }
 
@@ -2102,6 +2132,7 @@ int ps5_sort_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, co
     int rc = run_sort(c, d_arena, arena_bytes, d_offs, nullptr, (uint32_t)n, d_perm, d_lcp, nullptr, 0);
     if (rc) {
         (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->side);
         if (c->profile) collect_events(c);
     }
     return rc;
@@ -2257,6 +2288,7 @@ int ps5_sort_batch_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_byt
         HIPCHK(hipStreamSynchronize(c->stream));
     } else {
         (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->side);
         if (c->profile) collect_events(c);
     }
     return rc;
