@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -24,6 +26,7 @@
 #include <vector>
 
 #include "../../include/ps5_mi355x.h"
+#include "ps5_internal.hpp"
 
 namespace {
 
@@ -37,11 +40,25 @@ struct Staging {
     uint32_t* d_perm = nullptr;
     uint32_t* d_lcp = nullptr;
     uint8_t* d_cache = nullptr;
+    uint64_t* d_wide = nullptr;      // sorted pointers / widened LCPs, copied straight to the caller
     uint64_t n_cap = 0;
     uint32_t* d_out_idx = nullptr;   // merge outputs
     uint32_t* d_out_lcp = nullptr;
     uint64_t out_cap = 0;
 } g_st;
+
+// PS5_TRACE set: phase times of a host-entry sort on stderr
+struct PhaseClock {
+    bool on = getenv("PS5_TRACE") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(const char* what)
+    {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        fprintf(stderr, "[ps5] host %-10s %8.2f ms\n", what, std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
 
 unsigned host_threads()
 {
@@ -84,6 +101,8 @@ int ensure_staging(uint64_t arena_bytes, uint64_t n)
         g_st.n_cap = 0;
         if (g_st.d_cache) (void)hipFree(g_st.d_cache);
         g_st.d_cache = nullptr;
+        if (g_st.d_wide) (void)hipFree(g_st.d_wide);
+        g_st.d_wide = nullptr;
         for (uint32_t** q : { &g_st.d_offs, &g_st.d_perm, &g_st.d_lcp }) {
             void* p = nullptr;
             if (ps5_dev_malloc(&p, n * sizeof(uint32_t))) return PS5_ERR_NOMEM;
@@ -92,6 +111,8 @@ int ensure_staging(uint64_t arena_bytes, uint64_t n)
         void* p = nullptr;
         if (ps5_dev_malloc(&p, n)) return PS5_ERR_NOMEM;
         g_st.d_cache = (uint8_t*)p;
+        if (ps5_dev_malloc(&p, n * sizeof(uint64_t))) return PS5_ERR_NOMEM;
+        g_st.d_wide = (uint64_t*)p;
         g_st.n_cap = n;
     }
     return PS5_OK;
@@ -109,31 +130,52 @@ extern "C" int ps5_sort_strings_cache(unsigned char** strings, size_t n, uintptr
     int rc;
     if (!g_ctx && (rc = ps5_ctx_create(&g_ctx, 0))) return rc;
 
-    // lengths and contiguity of the pointer array
-    std::vector<uint64_t> len(n);
-    parallel_for(n, [&](uint64_t b, uint64_t e) {
-        for (uint64_t i = b; i < e; ++i) len[i] = strlen((const char*)strings[i]);
-    });
-    // One span: pointers in non-decreasing order, each string starting inside
-    // or right after what the strings before it cover.  That is psstest's
-    // arena (one NUL-separated string after the other) and its --suffix mode
-    // (every position of one text); every byte of the span belongs to some
-    // string, so the whole span is readable and is copied as it is.
-    bool span = true;
-    const uint8_t* span_end = strings[0] + len[0] + 1;
-    for (uint64_t i = 1; i < n && span; ++i) {
-        span = strings[i] >= strings[i - 1] && strings[i] <= span_end;
-        span_end = std::max<const uint8_t*>(span_end, strings[i] + len[i] + 1);
+    PhaseClock clk;
+    // One span: pointers in non-decreasing order, each at most a page after
+    // the one before.  That is psstest's arena (one NUL-separated string
+    // after the other) and its --suffix mode (every position of one text).
+    // Every byte between two consecutive starts then lies on one of their two
+    // pages, so the whole span up to the last string's end is readable
+    // (no string ends after the last one: a string that reaches past the next
+    // start shares that string's end), and it is copied as it is, without a
+    // pass over the characters here.
+    constexpr uintptr_t kPage = 4096;
+    const unsigned T = host_threads();
+    std::vector<uint8_t> ok(T, 1);
+    {
+        std::vector<std::thread> th;
+        const uint64_t chunk = (n + T - 1) / T;
+        for (unsigned t = 0; t < T; ++t) {
+            th.emplace_back([&, t] {
+                const uint64_t b = std::max<uint64_t>(t * chunk, 1), e = std::min<uint64_t>(n, t * chunk + chunk);
+                bool good = true;
+                for (uint64_t i = b; i < e; ++i)
+                    good = good && strings[i] >= strings[i - 1] && (uintptr_t)(strings[i] - strings[i - 1]) <= kPage;
+                ok[t] = good;
+            });
+        }
+        for (auto& x : th) x.join();
     }
-    std::vector<uint32_t> offs(n);
+    bool span = true;
+    for (unsigned t = 0; t < T; ++t) span = span && ok[t];
+    uint64_t alen = 0;
+    if (span) {
+        alen = (uint64_t)(strings[n - 1] - strings[0]) + strlen((const char*)strings[n - 1]) + 1;
+        span = alen <= 0xFFFFFFF0ull;
+    }
+    std::unique_ptr<uint32_t[]> offs(new uint32_t[n]);
     std::vector<uint8_t> packed;
     const uint8_t* host_arena;
-    uint64_t alen;
-    if (span && (uint64_t)(span_end - strings[0]) <= 0xFFFFFFF0ull) {
+    if (span) {
         host_arena = strings[0];
-        alen = (uint64_t)(span_end - strings[0]);
-        for (uint64_t i = 0; i < n; ++i) offs[i] = (uint32_t)(strings[i] - strings[0]);
+        parallel_for(n, [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = b; i < e; ++i) offs[i] = (uint32_t)(strings[i] - strings[0]);
+        });
     } else {
+        std::vector<uint64_t> len(n);
+        parallel_for(n, [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = b; i < e; ++i) len[i] = strlen((const char*)strings[i]);
+        });
         uint64_t tot = 0;
         for (uint64_t i = 0; i < n; ++i) {
             if (tot > 0xFFFFFFF0ull) return PS5_ERR_ARG;
@@ -148,17 +190,18 @@ extern "C" int ps5_sort_strings_cache(unsigned char** strings, size_t n, uintptr
         });
         host_arena = packed.data();
     }
+    clk.mark("layout");
     if ((rc = ensure_staging(alen, n))) return rc;
     if ((rc = ps5_memcpy_h2d(g_st.d_arena, host_arena, alen))) return rc;
-    if ((rc = ps5_memcpy_h2d(g_st.d_offs, offs.data(), n * sizeof(uint32_t)))) return rc;
+    if ((rc = ps5_memcpy_h2d(g_st.d_offs, offs.get(), n * sizeof(uint32_t)))) return rc;
+    clk.mark("to device");
     if ((rc = ps5_sort_device(g_ctx, g_st.d_arena, alen, g_st.d_offs, n, g_st.d_perm, lcp ? g_st.d_lcp : nullptr)))
         return rc;
-    std::vector<uint32_t> perm(n);
-    if ((rc = ps5_memcpy_d2h(perm.data(), g_st.d_perm, n * sizeof(uint32_t)))) return rc;
-    if (lcp) {
-        std::vector<uint32_t> l(n);
-        if ((rc = ps5_memcpy_d2h(l.data(), g_st.d_lcp, n * sizeof(uint32_t)))) return rc;
-        for (uint64_t i = 1; i < n; ++i) lcp[i] = l[i];
+    clk.mark("sort");
+    // LCPs widened on the device and copied straight into lcp[1..n)
+    if (lcp && n > 1) {
+        if ((rc = ps5::widen_lcps(g_ctx, g_st.d_lcp, n, g_st.d_wide))) return rc;
+        if ((rc = ps5_memcpy_d2h(lcp + 1, g_st.d_wide + 1, (n - 1) * sizeof(uint64_t)))) return rc;
     }
     if (cache && n > 1) {
         if ((rc = ps5_dist_chars_device(g_ctx, g_st.d_arena, alen, g_st.d_offs, n, g_st.d_perm, g_st.d_lcp,
@@ -166,13 +209,30 @@ extern "C" int ps5_sort_strings_cache(unsigned char** strings, size_t n, uintptr
             return rc;
         if ((rc = ps5_memcpy_d2h(cache + 1, g_st.d_cache + 1, n - 1))) return rc;
     }
+    clk.mark("to host");
     unsigned char** out = output ? output : strings;
-    if (out == strings) {
-        std::vector<unsigned char*> orig(strings, strings + n);
-        for (uint64_t i = 0; i < n; ++i) out[i] = orig[perm[i]];
+    static_assert(sizeof(unsigned char*) == sizeof(uint64_t), "pointers are 64-bit");
+    if (span) {     // the sorted pointers, computed on the device: base + offset of each sorted string
+        if ((rc = ps5::gather_pointers(g_ctx, g_st.d_perm, g_st.d_offs, (uint64_t)(uintptr_t)strings[0], n,
+                                       g_st.d_wide)))
+            return rc;
+        if ((rc = ps5_memcpy_d2h(out, g_st.d_wide, n * sizeof(uint64_t)))) return rc;
     } else {
-        for (uint64_t i = 0; i < n; ++i) out[i] = strings[perm[i]];
+        std::unique_ptr<uint32_t[]> perm(new uint32_t[n]);
+        if ((rc = ps5_memcpy_d2h(perm.get(), g_st.d_perm, n * sizeof(uint32_t)))) return rc;
+        if (out == strings) {
+            std::vector<unsigned char*> orig(n);
+            parallel_for(n, [&](uint64_t b, uint64_t e) { std::copy(strings + b, strings + e, orig.begin() + b); });
+            parallel_for(n, [&](uint64_t b, uint64_t e) {
+                for (uint64_t i = b; i < e; ++i) out[i] = orig[perm[i]];
+            });
+        } else {
+            parallel_for(n, [&](uint64_t b, uint64_t e) {
+                for (uint64_t i = b; i < e; ++i) out[i] = strings[perm[i]];
+            });
+        }
     }
+    clk.mark("permute");
     return PS5_OK;
 }
 

@@ -14,4 +14,13 @@ namespace ps5 {
 int widen_offsets(ps5_ctx* c, const uint32_t* d_offs32, const uint64_t* d_first, const uint64_t* d_base,
                   uint32_t nsets, uint64_t n, uint64_t* d_offs64);
 
+// Host-entry results computed on the device, for a direct copy into the
+// caller's arrays: ptrs[i] = base + offs[perm[i]] (the sorted string
+// pointers of a caller whose strings lie in one span starting at host
+// address base), and wide[i] = lcp[i] (uintptr_t LCPs).  Both run on the
+// context's stream and synchronise it.
+int gather_pointers(ps5_ctx* c, const uint32_t* d_perm, const uint32_t* d_offs, uint64_t base, uint64_t n,
+                    uint64_t* d_ptrs);
+int widen_lcps(ps5_ctx* c, const uint32_t* d_lcp, uint64_t n, uint64_t* d_wide);
+
 }  // namespace ps5

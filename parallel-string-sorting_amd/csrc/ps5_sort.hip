@@ -1144,6 +1144,21 @@ __global__ __launch_bounds__(kWG) void k_dist_chars(const uint8_t* arena, const 
     cache[i] = arena[p];
 }
 
+// ptrs[i] = base + offs[perm[i]]: the sorted strings' host pointers
+__global__ __launch_bounds__(kWG) void k_gather_ptrs(const uint32_t* perm, const uint32_t* offs, uint64_t base,
+                                                     uint64_t n, uint64_t* ptrs)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x;
+    if (i < n) ptrs[i] = base + offs[perm[i] < n ? perm[i] : 0u];
+}
+
+// wide[i] = lcp[i] as uint64 (entry 0 is copied too; callers skip it)
+__global__ __launch_bounds__(kWG) void k_widen_lcp(const uint32_t* lcp, uint64_t n, uint64_t* wide)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x;
+    if (i < n) wide[i] = lcp[i];
+}
+
 // offs64[i] = base[s] + offs32[i] for the set s holding string i
 __global__ __launch_bounds__(kWG) void k_widen(const uint32_t* offs32, const uint64_t* first, const uint64_t* base,
                                                uint32_t nsets, uint64_t n, uint64_t* offs64)
@@ -2428,6 +2443,30 @@ int widen_offsets(ps5_ctx* c, const uint32_t* d_offs32, const uint64_t* d_first,
     hipLaunchKernelGGL(k_widen, dim3(div_up(n, kWG)), dim3(kWG), 0, c->stream, d_offs32, d_first, d_base, nsets, n,
                        d_offs64);
     HIPCHK(hipGetLastError());
+    return PS5_OK;
+}
+
+
+int gather_pointers(ps5_ctx* c, const uint32_t* d_perm, const uint32_t* d_offs, uint64_t base, uint64_t n,
+                    uint64_t* d_ptrs)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "gather_pointers: no context");
+    if (n == 0) return PS5_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_gather_ptrs, dim3(div_up(n, kWG)), dim3(kWG), 0, c->stream, d_perm, d_offs, base, n, d_ptrs);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PS5_OK;
+}
+
+int widen_lcps(ps5_ctx* c, const uint32_t* d_lcp, uint64_t n, uint64_t* d_wide)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "widen_lcps: no context");
+    if (n == 0) return PS5_OK;
+    HIPCHK(hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_widen_lcp, dim3(div_up(n, kWG)), dim3(kWG), 0, c->stream, d_lcp, n, d_wide);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
     return PS5_OK;
 }
 
