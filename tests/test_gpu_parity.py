@@ -150,6 +150,40 @@ def test_host_entry_contiguous_and_packed(ps5, oracle):
     assert np.array_equal(lcp[1:].astype(np.uint32), el[1:])
 
 
+@pytest.mark.parametrize("layout", ["every_other", "page_gaps", "far_gap", "descending"])
+def test_host_entry_pointer_layouts(ps5, oracle, layout):
+    """pointers into one buffer: with gaps of up to a page the span is copied
+    as is (gap bytes unread by the sort), otherwise the strings are packed;
+    both give the exact order and LCPs"""
+    import ctypes
+    rng = np.random.default_rng(77)
+    strs = random_strings(30000, 0, 12, b"pqr", 5)
+    gaps = {"every_other": lambda i: 1 + (i % 2) * 7, "page_gaps": lambda i: int(rng.integers(0, 4000)),
+            "far_gap": lambda i: 70000 if i == 15000 else 0, "descending": lambda i: 0}[layout]
+    blob, starts = bytearray(), []
+    for i, x in enumerate(strs):
+        blob += b"\xAB" * gaps(i)          # filler the sort must never look at
+        starts.append(len(blob))
+        blob += x + b"\0"
+    blob += b"\0" * 16
+    buf = ctypes.create_string_buffer(bytes(blob), len(blob))
+    base = ctypes.addressof(buf)
+    ptrs = [base + o for o in starts]
+    if layout == "descending":
+        ptrs = ptrs[::-1]
+        strs = strs[::-1]
+    n = len(ptrs)
+    arr = (ctypes.c_void_p * n)(*ptrs)
+    lcp = np.full(n, 42, np.uint64)
+    lib = ps5.load()
+    assert lib.ps5_sort_strings_cache(arr, n, lcp.ctypes.data_as(ctypes.c_void_p), None, None) == 0
+    pos = {p: i for i, p in enumerate(ptrs)}
+    order = [pos[arr[i]] for i in range(n)]
+    ep, el = expected(strs)
+    assert order == list(ep)
+    assert lcp[0] == 42 and np.array_equal(lcp[1:].astype(np.uint32), el[1:])
+
+
 @pytest.mark.parametrize("letters,hi,n", [(b"ab", 8, 20000), (b"ACGT", 30, 150000),
                                           (bytes(range(1, 256)), 3, 5000)], ids=["ab", "acgt", "bytes"])
 def test_dist_chars_device(ctx, oracle, letters, hi, n):
