@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <memory>
 #include <chrono>
 #include <cstdio>
@@ -291,7 +292,9 @@ extern "C" int ps5_lcp_merge_strings(unsigned char* const* const* runs, const ui
     // (q-th of the concatenation) gets index q
     std::vector<const unsigned char*> str(n);
     for (unsigned r = 0; r < nruns; ++r)
-        for (uint64_t i = 0; i < sizes[r]; ++i) str[run_first[r] + i] = runs[r][i];
+        parallel_for(sizes[r], [&](uint64_t b, uint64_t e) {
+            std::copy(runs[r] + b, runs[r] + e, str.begin() + run_first[r] + b);
+        });
     std::vector<uint64_t> len(n);
     parallel_for(n, [&](uint64_t b, uint64_t e) {
         for (uint64_t i = b; i < e; ++i) len[i] = strlen((const char*)str[i]);
@@ -301,15 +304,21 @@ extern "C" int ps5_lcp_merge_strings(unsigned char* const* const* runs, const ui
     for (uint64_t i = 0; i < n; ++i) {
         if (tot > 0xFFFFFFF0ull) return PS5_ERR_ARG;
         offs[i] = (uint32_t)tot;
-        idx[i] = (uint32_t)i;
         tot += len[i] + 1;
     }
     if (tot > 0xFFFFFFF0ull) return PS5_ERR_ARG;
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) idx[i] = (uint32_t)i;
+    });
+    std::atomic<bool> lcp_ok{ true };
     for (unsigned r = 0; r < nruns; ++r)
-        for (uint64_t i = 1; i < sizes[r]; ++i) {
-            if (lcps[r][i] > 0xFFFFFFFFull) return PS5_ERR_ARG;
-            lcp[run_first[r] + i] = (uint32_t)lcps[r][i];
-        }
+        parallel_for(sizes[r], [&](uint64_t b, uint64_t e) {
+            for (uint64_t i = std::max<uint64_t>(b, 1); i < e; ++i) {
+                if (lcps[r][i] > 0xFFFFFFFFull) lcp_ok = false;
+                lcp[run_first[r] + i] = (uint32_t)lcps[r][i];
+            }
+        });
+    if (!lcp_ok) return PS5_ERR_ARG;
     std::vector<uint8_t> packed(tot);
     parallel_for(n, [&](uint64_t b, uint64_t e) {
         for (uint64_t i = b; i < e; ++i) memcpy(&packed[offs[i]], str[i], len[i] + 1);
@@ -336,10 +345,12 @@ extern "C" int ps5_lcp_merge_strings(unsigned char* const* const* runs, const ui
                                g_st.d_lcp, g_st.d_out_idx, g_st.d_out_lcp)))
         return rc;
     if ((rc = ps5_memcpy_d2h(idx.data(), g_st.d_out_idx, n * 4))) return rc;
-    for (uint64_t i = 0; i < n; ++i) output[i] = const_cast<unsigned char*>(str[idx[i]]);
-    if (out_lcp) {
-        if ((rc = ps5_memcpy_d2h(lcp.data(), g_st.d_out_lcp, n * 4))) return rc;
-        for (uint64_t i = 1; i < n; ++i) out_lcp[i] = lcp[i];
+    parallel_for(n, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) output[i] = const_cast<unsigned char*>(str[idx[i]]);
+    });
+    if (out_lcp && n > 1) {     // widened on the device, copied straight into out_lcp[1..n)
+        if ((rc = ps5::widen_lcps(g_ctx, g_st.d_out_lcp, n, g_st.d_wide))) return rc;
+        if ((rc = ps5_memcpy_d2h(out_lcp + 1, g_st.d_wide + 1, (n - 1) * sizeof(uint64_t)))) return rc;
     }
     return PS5_OK;
 }
