@@ -150,8 +150,11 @@ extern "C" int ps5_sort_strings_cache(unsigned char** strings, size_t n, uintptr
             th.emplace_back([&, t] {
                 const uint64_t b = std::max<uint64_t>(t * chunk, 1), e = std::min<uint64_t>(n, t * chunk + chunk);
                 bool good = true;
-                for (uint64_t i = b; i < e; ++i)
-                    good = good && strings[i] >= strings[i - 1] && (uintptr_t)(strings[i] - strings[i - 1]) <= kPage;
+                // (addresses compared as integers: the pointers need not share an allocation)
+                for (uint64_t i = b; i < e; ++i) {
+                    const uintptr_t a = (uintptr_t)strings[i - 1], c = (uintptr_t)strings[i];
+                    good = good && c >= a && c - a <= kPage;
+                }
                 ok[t] = good;
             });
         }
@@ -161,7 +164,7 @@ extern "C" int ps5_sort_strings_cache(unsigned char** strings, size_t n, uintptr
     for (unsigned t = 0; t < T; ++t) span = span && ok[t];
     uint64_t alen = 0;
     if (span) {
-        alen = (uint64_t)(strings[n - 1] - strings[0]) + strlen((const char*)strings[n - 1]) + 1;
+        alen = (uint64_t)((uintptr_t)strings[n - 1] - (uintptr_t)strings[0]) + strlen((const char*)strings[n - 1]) + 1;
         span = alen <= 0xFFFFFFF0ull;
     }
     std::unique_ptr<uint32_t[]> offs(new uint32_t[n]);
@@ -170,7 +173,7 @@ extern "C" int ps5_sort_strings_cache(unsigned char** strings, size_t n, uintptr
     if (span) {
         host_arena = strings[0];
         parallel_for(n, [&](uint64_t b, uint64_t e) {
-            for (uint64_t i = b; i < e; ++i) offs[i] = (uint32_t)(strings[i] - strings[0]);
+            for (uint64_t i = b; i < e; ++i) offs[i] = (uint32_t)((uintptr_t)strings[i] - (uintptr_t)strings[0]);
         });
     } else {
         std::vector<uint64_t> len(n);
