@@ -313,25 +313,48 @@ __global__ __launch_bounds__(kWG) void k_scan_top(uint32_t* part, uint32_t npart
 }
 
 // exclusive scan inside each block, offset by its block prefix (in place)
+// (data is 16-byte aligned: a thread's kScanPer consecutive entries move as
+// 16-byte vectors, except in a block cut short by len)
 __global__ __launch_bounds__(kWG) void k_scan_down(uint32_t* data, uint32_t len, const uint32_t* part)
 {
+    static_assert(kScanPer % 4 == 0, "whole 16-byte vectors per thread");
     __shared__ uint32_t s_tmp[16];
     const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+    const bool full = base + kScanPer <= len;
     uint32_t v[kScanPer];
+    if (full) {
+        const uint4* p = reinterpret_cast<const uint4*>(data + base);
+#pragma unroll
+        for (int q = 0; q < kScanPer / 4; ++q) {
+            const uint4 x = p[q];
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j) v[j] = base + j < len ? data[base + j] : 0u;
+    }
     uint32_t sum = 0;
 #pragma unroll
-    for (int j = 0; j < kScanPer; ++j) {
-        const uint32_t i = base + j;
-        v[j] = i < len ? data[i] : 0u;
-        sum += v[j];
-    }
+    for (int j = 0; j < kScanPer; ++j) sum += v[j];
     uint32_t tot;
     uint32_t run = block_excl_scan<kWaves>(sum, &tot, s_tmp) + part[blockIdx.x];
+    uint32_t o[kScanPer];
 #pragma unroll
     for (int j = 0; j < kScanPer; ++j) {
-        const uint32_t i = base + j;
-        if (i < len) data[i] = run;
+        o[j] = run;
         run += v[j];
+    }
+    if (full) {
+        uint4* p = reinterpret_cast<uint4*>(data + base);
+#pragma unroll
+        for (int q = 0; q < kScanPer / 4; ++q) p[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanPer; ++j)
+            if (base + j < len) data[base + j] = o[j];
     }
 }
 
