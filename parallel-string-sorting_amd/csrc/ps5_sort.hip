@@ -99,7 +99,7 @@ __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, uint64_t seed, key64* trees,
                                                   int from_arena)
 {
-    __shared__ key64 s_samp[2][kMaxSamples];
+    __shared__ key64 s_samp[1][kMaxSamples];
     const LargeDesc D = desc[blockIdx.x];
     const Item seg = D.seg;
     if (!item_ok(seg, B)) {
@@ -115,14 +115,18 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
         if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
-    // every sample load is issued first (kPer per thread), then the samples
-    // are sorted in runs of 64 in the waves' registers (padding ~0 last)
+    // every sample load is issued first (kPer consecutive samples per
+    // thread), each thread sorts its kPer keys with a network, then runs are
+    // merged pairwise by merge path: a co-rank search places a thread's kPer
+    // outputs, then it merges them sequentially (padding ~0 sorts last; equal
+    // keys need no tie order, only the splitters are kept)
     constexpr int kPer = kMaxSamples / kBigWG;
-    static_assert(kMaxSamples % kBigWG == 0, "whole sample passes");
+    static_assert(kMaxSamples % kBigWG == 0 && kPer == 4, "four samples per thread");
+    const uint32_t o = threadIdx.x * kPer;
     key64 v[kPer];
 #pragma unroll
     for (int e = 0; e < kPer; ++e) {
-        const uint32_t j = e * kBigWG + threadIdx.x;
+        const uint32_t j = o + e;
         v[e] = ~0ull;
         if (j < ss) {
             const uint64_t h = mix64(seed ^ ((uint64_t)seg.begin << 21) ^ (uint64_t)j);
@@ -130,29 +134,52 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
             v[e] = from_arena ? load_key(B.arena, B.alen, str_off(B, pos)) : K[pos];
         }
     }
+    const bool act = o < P;
+    if (act) {
+        constexpr int net[][2] = { { 0, 1 }, { 2, 3 }, { 0, 2 }, { 1, 3 }, { 1, 2 } };
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-        const uint32_t j = e * kBigWG + threadIdx.x;    // run j / 64 belongs to this wave
-        if (e * (uint32_t)kBigWG >= P) break;           // (uniform) P is a multiple of 64
-        if (j >= P) continue;                           // (wave-uniform)
-        uint32_t p = j;
-        wave_sort64(v[e], p);
-        s_samp[0][j] = v[e];
+        for (int c = 0; c < 5; ++c) {
+            const key64 x = v[net[c][0]], y = v[net[c][1]];
+            v[net[c][0]] = min(x, y);
+            v[net[c][1]] = max(x, y);
+        }
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) s_samp[0][o + e] = v[e];
     }
     __syncthreads();
-    // pairwise merges by rank (equal keys of the left run first)
-    int cur = 0;
-    for (uint32_t L = 64; L < P; L <<= 1) {
-        for (uint32_t i = threadIdx.x; i < P; i += kBigWG) {
-            const uint32_t run = i / L, t = i % L, prt = run ^ 1u;
-            const key64 v = s_samp[cur][i];
-            const uint32_t c = run_rank(&s_samp[cur][prt * L], L, L, v, prt < run);
-            s_samp[cur ^ 1][min(run, prt) * L + t + c] = v;
+    key64* sa = s_samp[0];
+    for (uint32_t L = kPer; L < P; L <<= 1) {
+        if (act) {
+            const uint32_t a0 = (o / (2 * L)) * (2 * L), b0 = a0 + L, diag = o - a0;
+            uint32_t lo = diag > L ? diag - L : 0u, hi = min(diag, L);
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sa[a0 + mid] <= sa[b0 + diag - 1 - mid]) lo = mid + 1;
+                else hi = mid;
+            }
+            uint32_t i = lo, j = diag - lo;
+            key64 ka = i < L ? sa[a0 + i] : 0ull, kb = j < L ? sa[b0 + j] : 0ull;
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const bool take_a = j >= L || (i < L && ka <= kb);
+                v[e] = take_a ? ka : kb;
+                if (take_a) {
+                    ++i;
+                    ka = i < L ? sa[a0 + i] : 0ull;
+                } else {
+                    ++j;
+                    kb = j < L ? sa[b0 + j] : 0ull;
+                }
+            }
         }
         __syncthreads();
-        cur ^= 1;
+        if (act) {
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) sa[o + e] = v[e];
+        }
+        __syncthreads();
     }
-    const key64* sorted = s_samp[cur];
+    const key64* sorted = sa;
     key64* tree = trees + (uint64_t)blockIdx.x * kTreeSlots;
     for (int t = threadIdx.x + 1; t <= nspl; t += kBigWG) tree[t] = sorted[a / 2 + a * inorder_of(t, k)];
     for (int i = threadIdx.x; i < nspl; i += kBigWG) tree[1024 + i] = sorted[a / 2 + a * i];
