@@ -107,73 +107,11 @@ __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 __device__ __forceinline__ uint32_t wave_id() { return threadIdx.x >> 6; }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
-// Value of lane (lane ^ J) of the wave, without the LDS crossbar: DPP quad
-// permutes for 1 and 2, a half-row mirror then a quad mirror for 4, a row
-// rotate by 8 for 8, and the gfx950 permlane swaps for 16 and 32 (which
-// exchange rows 1<->0, 3<->2 of two copies, resp. the two 32-lane halves).
-template <int J>
-__device__ __forceinline__ uint32_t lane_xor(uint32_t v)
-{
-    if constexpr (J == 1) {
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
-    } else if constexpr (J == 2) {
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);   // quad_perm [2,3,0,1]
-    } else if constexpr (J == 4) {
-        const int h = __builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, true);      // row_half_mirror
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, h, 0x1B, 0xF, 0xF, true);        // quad_perm [3,2,1,0]
-    } else if constexpr (J == 8) {
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true);  // row_ror:8
-    } else if constexpr (J == 16) {
-        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-        return (lane_id() & 16u) ? r[0] : r[1];
-    } else {
-        static_assert(J == 32, "lane distance must be a power of two below 64");
-        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-        return (lane_id() & 32u) ? r[0] : r[1];
-    }
-}
-
-template <int J>
-__device__ __forceinline__ uint64_t lane_xor64(uint64_t v)
-{
-    const uint32_t lo = lane_xor<J>((uint32_t)v);
-    const uint32_t hi = lane_xor<J>((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // (key, position) order: positions are distinct, so the order is total and
 // equal keys keep their input order (stable).
 __device__ __forceinline__ bool kp_less(key64 ka, uint32_t pa, key64 kb, uint32_t pb)
 {
     return ka < kb || (ka == kb && pa < pb);
-}
-
-// One compare-exchange step of a bitonic network across the lanes of a wave:
-// partner lane ^ J, ascending where bit KK of the lane is clear.
-template <int KK, int J>
-__device__ __forceinline__ void wave_cx(key64& k, uint32_t& p)
-{
-    const uint32_t lane = lane_id();
-    const key64 ok = lane_xor64<J>(k);
-    const uint32_t op = lane_xor<J>(p);
-    const bool keep_min = ((lane & (uint32_t)J) == 0) == ((lane & (uint32_t)KK) == 0);
-    const bool take = kp_less(ok, op, k, p) == keep_min;
-    k = take ? ok : k;     // selects, not a branch around the moves
-    p = take ? op : p;
-}
-
-// Sort the 64 (key, position) pairs of a wave ascending, lane i ends with the
-// i-th smallest: a bitonic network entirely in registers (21 steps).
-__device__ __forceinline__ void wave_sort64(key64& k, uint32_t& p)
-{
-    wave_cx<2, 1>(k, p);
-    wave_cx<4, 2>(k, p);  wave_cx<4, 1>(k, p);
-    wave_cx<8, 4>(k, p);  wave_cx<8, 2>(k, p);  wave_cx<8, 1>(k, p);
-    wave_cx<16, 8>(k, p); wave_cx<16, 4>(k, p); wave_cx<16, 2>(k, p); wave_cx<16, 1>(k, p);
-    wave_cx<32, 16>(k, p); wave_cx<32, 8>(k, p); wave_cx<32, 4>(k, p); wave_cx<32, 2>(k, p);
-    wave_cx<32, 1>(k, p);
-    wave_cx<64, 32>(k, p); wave_cx<64, 16>(k, p); wave_cx<64, 8>(k, p); wave_cx<64, 4>(k, p);
-    wave_cx<64, 2>(k, p);  wave_cx<64, 1>(k, p);
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on
@@ -280,29 +218,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total,
     __syncthreads();
     *total = tot;
     return wpre + x - v;
-}
-
-// Number of elements of a sorted run a[0, len) that precede `key` of a run
-// further right (before_eq: elements equal to key precede it, i.e. the run is
-// to the left of the key's run).  Binary lifting over a power-of-two L >= len.
-__device__ __forceinline__ uint32_t run_rank(const key64* a, uint32_t len, uint32_t L, key64 key, bool before_eq)
-{
-    uint32_t c = 0;
-    if (len == L) {     // a full run: a[c + s - 1] with c + s <= L always exists, except in the first step
-        if (a[L - 1] < key || (before_eq && a[L - 1] == key)) return L;
-        for (uint32_t s = L >> 1; s > 0; s >>= 1) {
-            const key64 x = a[c + s - 1];
-            c += (x < key || (before_eq && x == key)) ? s : 0u;
-        }
-        return c;
-    }
-    for (uint32_t s = L; s > 0; s >>= 1) {
-        if (c + s <= len) {
-            const key64 x = a[c + s - 1];
-            if (x < key || (before_eq && x == key)) c += s;
-        }
-    }
-    return c;
 }
 
 // In-order index of level-order node t of a complete tree with k levels.

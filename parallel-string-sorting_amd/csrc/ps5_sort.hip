@@ -88,8 +88,8 @@ __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 // ---- multi-workgroup sample-sort step (segments > kBlkT) -------------------
 
 // Splitter trees of the large segments of a level, one workgroup each.
-// D.oversample * nspl random keys are sorted (64-key runs in registers, then
-// merged in LDS); splitter i is sample
+// D.oversample * nspl random keys are sorted (4-key runs per thread by a
+// sorting network, then merge-path rounds in LDS); splitter i is sample
 // oversample/2 + oversample*i (the reference's choice for oversample factor 2,
 // bingmann-sample_sort_tree_builder.hpp:44-58) and the tree is stored in
 // level order: trees[seg*2048 + t], t = 1..nspl, followed by the splitters in
@@ -99,7 +99,7 @@ __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, uint64_t seed, key64* trees,
                                                   int from_arena)
 {
-    __shared__ key64 s_samp[1][kMaxSamples];
+    __shared__ key64 s_samp[kMaxSamples];
     const LargeDesc D = desc[blockIdx.x];
     const Item seg = D.seg;
     if (!item_ok(seg, B)) {
@@ -144,10 +144,10 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
             v[net[c][1]] = max(x, y);
         }
 #pragma unroll
-        for (int e = 0; e < kPer; ++e) s_samp[0][o + e] = v[e];
+        for (int e = 0; e < kPer; ++e) s_samp[o + e] = v[e];
     }
     __syncthreads();
-    key64* sa = s_samp[0];
+    key64* sa = s_samp;
     for (uint32_t L = kPer; L < P; L <<= 1) {
         if (act) {
             const uint32_t a0 = (o / (2 * L)) * (2 * L), b0 = a0 + L, diag = o - a0;
