@@ -132,6 +132,42 @@ def test_duplicate_groups_in_blocks(ctx, oracle, copies):
     assert np.array_equal(perm, op) and np.array_equal(lcp[1:], ol[1:])
 
 
+def shaped_set(seed):
+    """A seeded random shape: string count (log-uniform in 30..300 000),
+    alphabet (1 to 255 symbols, high bytes included), length range, a common
+    prefix on part of the set and groups of identical copies."""
+    rng = np.random.default_rng(7000 + seed)
+    n = int(np.exp(rng.uniform(np.log(30), np.log(300000))))
+    alphabets = [b"a", b"ab", b"abc", b"ACGT", b"0123456789", bytes(range(97, 123)), bytes(range(33, 127)),
+                 bytes(range(1, 256))]
+    alpha = alphabets[int(rng.integers(len(alphabets)))]
+    lo = int(rng.integers(0, 20))
+    hi = lo + int(rng.integers(0, 120))
+    strs = random_strings(n, lo, hi, alpha, 7000 + seed)
+    if rng.random() < 0.5:      # a shared prefix (up to 47 characters) on a fraction of the set
+        pre = bytes(rng.choice(list(alpha), int(rng.integers(1, 48))).tolist())
+        frac = rng.random()
+        strs = [pre + s if rng.random() < frac else s for s in strs]
+    if rng.random() < 0.4:      # every string a copy of one of fewer base strings
+        base = strs[:max(1, n // int(rng.integers(2, 100)))]
+        strs = [base[i] for i in rng.integers(0, len(base), n)]
+    return strs
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_shapes(ctx, oracle, seed):
+    """seeded sweep over set shapes; the oracle's verifier fixes the output
+    uniquely (order, tie order, permutation, every LCP)"""
+    strs = shaped_set(seed)
+    say("shape", seed, len(strs), max(len(s) for s in strs))
+    arena, offs = pack(strs)
+    perm, lcp = ctx.sort(arena, offs)
+    r, msg = oracle.verify(arena, offs, perm, lcp)
+    assert r == 0, msg
+    if len(strs) <= 20000:
+        check_against_brute(ctx, oracle, arena, offs)
+
+
 def test_repeated_calls_shrinking(ctx, oracle):
     """buffers are reused across calls; results must not depend on history"""
     for n, seed in [(300000, 1), (5000, 2), (70000, 3), (10, 4)]:
