@@ -1966,7 +1966,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
         const uint32_t depth = round * 8;
         const int rp = round & 1, np = rp ^ 1;
         c->st_rounds++;
-        {   // the lists of round r+1 are empty again (consumed in round r-1)
+        if (round > 0) {   // the lists of round r+1 are empty again (consumed in round r-1; set up empty in round 0)
             Launch L(c, K_MISC, 0);
             uint32_t mask = (1u << (C_LROUND + np)) | (1u << (C_TINY + np)) | (1u << (C_RUN + np));
             for (int cls = 0; cls < kBlkClasses; ++cls) mask |= 1u << (C_BLOCK + 2 * cls + np);
@@ -2020,7 +2020,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
             }
             Item* nx_large = c->l_lvl[level & 1];
             const int nx_ci = C_LLVL + (level & 1);
-            {
+            if (round > 0 || level > 0) {     // (every counter starts at zero)
                 Launch L(c, K_MISC, 0);
                 hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, 1u << nx_ci);
             }
