@@ -161,15 +161,16 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
             key64 ka = i < L ? sa[a0 + i] : 0ull, kb = j < L ? sa[b0 + j] : 0ull;
 #pragma unroll
             for (int e = 0; e < kPer; ++e) {
+                // branch-free step, as in the block sort: one read of the
+                // consumed run's next head (unused past the run's end)
                 const bool take_a = j >= L || (i < L && ka <= kb);
+                const uint32_t src = take_a ? a0 + i : b0 + j;
                 v[e] = take_a ? ka : kb;
-                if (take_a) {
-                    ++i;
-                    ka = i < L ? sa[a0 + i] : 0ull;
-                } else {
-                    ++j;
-                    kb = j < L ? sa[b0 + j] : 0ull;
-                }
+                i += take_a ? 1u : 0u;
+                j += take_a ? 0u : 1u;
+                const key64 nx = sa[min(src + 1, P - 1)];
+                ka = take_a ? nx : ka;
+                kb = take_a ? kb : nx;
             }
         }
         __syncthreads();
@@ -1016,16 +1017,18 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
             key64 ka = i < la ? s_k[a0 + i] : 0ull, kb = j < lb ? s_k[b0 + j] : 0ull;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
+                // branch-free step: the consumed run's next head is the slot
+                // after the one taken (past a run's end it is never used:
+                // take_a tests i < la and j >= lb); clamped into the block
                 const bool take_a = j >= lb || (i < la && ka <= kb);
+                const uint32_t src = take_a ? a0 + i : b0 + j;
                 k[e] = take_a ? ka : kb;
-                p[e] = take_a ? s_p[a0 + i] : s_p[b0 + j];
-                if (take_a) {
-                    ++i;
-                    ka = i < la ? s_k[a0 + i] : 0ull;
-                } else {
-                    ++j;
-                    kb = j < lb ? s_k[b0 + j] : 0ull;
-                }
+                p[e] = s_p[src];
+                i += take_a ? 1u : 0u;
+                j += take_a ? 0u : 1u;
+                const key64 nx = s_k[min(src + 1, Pm - 1)];
+                ka = take_a ? nx : ka;
+                kb = take_a ? kb : nx;
             }
         }
         __syncthreads();
