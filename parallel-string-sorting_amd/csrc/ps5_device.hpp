@@ -239,17 +239,21 @@ template <int G>
 __device__ __forceinline__ void classify_keys(const key64 (&key)[G], const key64* s_tree, const key64* s_sorted,
                                               int k, uint32_t (&b)[G])
 {
-    uint32_t i[G];
+    // the descent tracks byte offsets into s_tree (8 * node): each level is
+    // one LDS read, one compare and one shift-add
+    uint32_t a[G];
 #pragma unroll
-    for (int j = 0; j < G; ++j) i[j] = 1;
+    for (int j = 0; j < G; ++j) a[j] = 8;
+    const unsigned char* tb = reinterpret_cast<const unsigned char*>(s_tree);
     for (int l = 0; l < k; ++l) {
 #pragma unroll
-        for (int j = 0; j < G; ++j) i[j] = 2 * i[j] + (key[j] > s_tree[i[j]] ? 1u : 0u);
+        for (int j = 0; j < G; ++j)
+            a[j] = 2 * a[j] + (key[j] > *reinterpret_cast<const key64*>(tb + a[j]) ? 8u : 0u);
     }
     const uint32_t nspl = (1u << k) - 1;
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-        const uint32_t x = i[j] - (1u << k);
+        const uint32_t x = (a[j] >> 3) - (1u << k);
         b[j] = 2 * x + ((x < nspl && s_sorted[x] == key[j]) ? 1u : 0u);
     }
 }

@@ -195,11 +195,13 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
 // rank inside the chunk.  With from_arena (the root of a sort, positions =
 // input indices) the keys are extracted from the arena here and stored: this
 // replaces the separate extraction pass and saves reading the keys back.
-// Dynamic LDS, sized by the largest tree of the level (kmax treebits):
-// tree[2^kmax], sorted splitters[2^kmax], per-wave 16-bit counts[kWaves][2^(kmax+1)].
+// The tree and the sorted splitters sit in static LDS (their addresses are
+// constants, so a descent step needs no address arithmetic); the per-wave
+// 16-bit counts[kWaves][2^(kmax+1)] in dynamic LDS, sized by the largest
+// tree of the level (kmax treebits).
 __host__ __device__ constexpr uint32_t count_lds_bytes(int kmax)
 {
-    return (2u << kmax) * 8u + kWaves * (2u << kmax) * 2u;
+    return kWaves * (2u << kmax) * 2u;
 }
 
 // NB: bits of a bucket id in this launch (kmax + 1, at least 8): the ballot
@@ -211,10 +213,10 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     static_assert(NB >= 2 && NB <= kMaxTreebits + 1, "bucket ids of up to 2047 buckets");
     constexpr uint32_t kNoBkt = (1u << NB) - 1;   // id of inactive lanes, above every bucket of the launch
     extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ key64 s_tree[1 << (NB - 1)];        // trees of up to NB - 1 levels
+    __shared__ key64 s_sorted[1 << (NB - 1)];
     const uint32_t nb2 = 2u << kmax;               // bucket slots per wave
-    key64* s_tree = reinterpret_cast<key64*>(smem);
-    key64* s_sorted = s_tree + (nb2 >> 1);
-    uint16_t* s_hist_base = reinterpret_cast<uint16_t*>(s_sorted + (nb2 >> 1));
+    uint16_t* s_hist_base = reinterpret_cast<uint16_t*>(smem);
     static_assert(kChunkMax <= 65535, "per-wave bucket counts must fit 16 bits");
     const TileRef tr = tiles[xcd_contiguous(blockIdx.x, gridDim.x)];
     const LargeDesc D = desc[tr.seg];
