@@ -958,7 +958,7 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
 {
     constexpr uint32_t T = MAXM / E;
     static_assert(T % 64 == 0 && MAXM <= 65536, "whole waves; positions fit 16 bits");
-    __shared__ key64 s_k[MAXM];
+    __shared__ key64 s_k[MAXM + 1];     // + 1: a merge step reads the slot after the one it takes
     __shared__ uint16_t s_p[MAXM];
     constexpr uint32_t kMaxRuns = kBlockRuns;    // short equal runs finished here (more go to the next round)
     __shared__ uint32_t s_runs[kMaxRuns];        // (start << 16) | length
@@ -1015,20 +1015,25 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
                 if (s_k[a0 + mid] <= s_k[b0 + diag - 1 - mid]) lo = mid + 1;
                 else hi = mid;
             }
-            uint32_t i = lo, j = diag - lo;
-            key64 ka = i < la ? s_k[a0 + i] : 0ull, kb = j < lb ? s_k[b0 + j] : 0ull;
+            // the heads as byte offsets into s_k, and the runs' ends
+            uint32_t pa = (a0 + lo) * 8u, pb = (b0 + diag - lo) * 8u;
+            const uint32_t ea = (a0 + la) * 8u, eb = (b0 + lb) * 8u;
+            const unsigned char* kbase = reinterpret_cast<const unsigned char*>(s_k);
+            const unsigned char* pbase = reinterpret_cast<const unsigned char*>(s_p);
+            key64 ka = pa < ea ? *reinterpret_cast<const key64*>(kbase + pa) : 0ull;
+            key64 kb = pb < eb ? *reinterpret_cast<const key64*>(kbase + pb) : 0ull;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 // branch-free step: the consumed run's next head is the slot
                 // after the one taken (past a run's end it is never used:
-                // take_a tests i < la and j >= lb); clamped into the block
-                const bool take_a = j >= lb || (i < la && ka <= kb);
-                const uint32_t src = take_a ? a0 + i : b0 + j;
+                // take_a tests pa < ea and pb >= eb; s_k has one spare slot)
+                const bool take_a = pb >= eb || (pa < ea && ka <= kb);
+                const uint32_t src = take_a ? pa : pb;
                 k[e] = take_a ? ka : kb;
-                p[e] = s_p[src];
-                i += take_a ? 1u : 0u;
-                j += take_a ? 0u : 1u;
-                const key64 nx = s_k[min(src + 1, Pm - 1)];
+                p[e] = *reinterpret_cast<const uint16_t*>(pbase + (src >> 2));
+                const key64 nx = *reinterpret_cast<const key64*>(kbase + src + 8u);
+                pa += take_a ? 8u : 0u;
+                pb += take_a ? 0u : 8u;
                 ka = take_a ? nx : ka;
                 kb = take_a ? kb : nx;
             }
