@@ -99,7 +99,7 @@ __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, uint64_t seed, key64* trees,
                                                   int from_arena)
 {
-    __shared__ key64 s_samp[kMaxSamples];
+    __shared__ key64 s_samp[kMaxSamples + 1];     // + 1: a merge step reads the slot after the one it takes
     const LargeDesc D = desc[blockIdx.x];
     const Item seg = D.seg;
     if (!item_ok(seg, B)) {
@@ -157,18 +157,22 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
                 if (sa[a0 + mid] <= sa[b0 + diag - 1 - mid]) lo = mid + 1;
                 else hi = mid;
             }
-            uint32_t i = lo, j = diag - lo;
-            key64 ka = i < L ? sa[a0 + i] : 0ull, kb = j < L ? sa[b0 + j] : 0ull;
+            // branch-free steps on byte offsets, as in the block sort: one
+            // read of the consumed run's next head (unused past the run's
+            // end; s_samp has one spare slot)
+            const unsigned char* sbase = reinterpret_cast<const unsigned char*>(sa);
+            uint32_t pa = (a0 + lo) * 8u, pb = (b0 + diag - lo) * 8u;
+            const uint32_t ea = (a0 + L) * 8u, eb = (b0 + L) * 8u;
+            key64 ka = pa < ea ? *reinterpret_cast<const key64*>(sbase + pa) : 0ull;
+            key64 kb = pb < eb ? *reinterpret_cast<const key64*>(sbase + pb) : 0ull;
 #pragma unroll
             for (int e = 0; e < kPer; ++e) {
-                // branch-free step, as in the block sort: one read of the
-                // consumed run's next head (unused past the run's end)
-                const bool take_a = j >= L || (i < L && ka <= kb);
-                const uint32_t src = take_a ? a0 + i : b0 + j;
+                const bool take_a = pb >= eb || (pa < ea && ka <= kb);
+                const uint32_t src = take_a ? pa : pb;
                 v[e] = take_a ? ka : kb;
-                i += take_a ? 1u : 0u;
-                j += take_a ? 0u : 1u;
-                const key64 nx = sa[min(src + 1, P - 1)];
+                const key64 nx = *reinterpret_cast<const key64*>(sbase + src + 8u);
+                pa += take_a ? 8u : 0u;
+                pb += take_a ? 0u : 8u;
                 ka = take_a ? nx : ka;
                 kb = take_a ? kb : nx;
             }
