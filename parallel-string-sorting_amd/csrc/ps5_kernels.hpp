@@ -39,10 +39,12 @@ constexpr uint32_t kTreeSlots = 2048;        // per segment: level-order tree [0
 // chunk (the unit of stable ranks and of the scatter's LDS reorder) holds at
 // most a per-level limit of keys, and T is the fewest tiles that allows, so
 // a level's tiles are as few and as full as possible.  The limit is
-// kChunkMax unless the level is too small to give kMinTiles tiles that way;
+// kChunkMax (kChunkMaxIdent in a level of roots, whose scatter stages
+// 2-byte positions) unless the level is too small to give kMinTiles tiles that way;
 // then it shrinks (not below kChunkMin) so small sorts still spread over
 // the GPU.
 constexpr int kChunkMax = 9216;              // keys ranked by one wave (multi-workgroup step)
+constexpr int kChunkMaxIdent = 12032;        // the same in a level of root segments (indices = positions)
 constexpr int kChunkMin = 2048;
 constexpr uint32_t kMinTiles = 512;
 constexpr uint32_t kTinyT = 64;              // leaves sorted by one wave
@@ -85,7 +87,7 @@ constexpr uint32_t ERR_CAP = 1u, ERR_LCP = 2u, ERR_BOUNDS = 4u;
 struct LargeDesc {       // one segment of the multi-workgroup sample-sort step
     Item seg;
     uint32_t T;          // tiles of kWaves * chunk keys (the last one may be short)
-    uint32_t chunk;      // keys per wave chunk, <= kChunkMax
+    uint32_t chunk;      // keys per wave chunk, <= kChunkMax (kChunkMaxIdent for a root)
     uint32_t pad;
     uint32_t hist_base;  // first histogram entry of this segment
     uint32_t k;          // treebits

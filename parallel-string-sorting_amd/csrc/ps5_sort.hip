@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     __shared__ key64 s_sorted[1 << (NB - 1)];
     const uint32_t nb2 = 2u << kmax;               // bucket slots per wave
     uint16_t* s_hist_base = reinterpret_cast<uint16_t*>(smem);
-    static_assert(kChunkMax <= 65535, "per-wave bucket counts must fit 16 bits");
+    static_assert(kChunkMax <= 65535 && kChunkMaxIdent <= 65535, "per-wave bucket counts must fit 16 bits");
     const TileRef tr = tiles[xcd_contiguous(blockIdx.x, gridDim.x)];
     const LargeDesc D = desc[tr.seg];
     const Item seg = D.seg;
@@ -241,7 +241,8 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     const uint32_t w = wave_id(), lane = lane_id();
     uint16_t* s_hist = s_hist_base + w * nb2;      // this wave's counts
     const uint32_t cb = (t * kWaves + w) * D.chunk;
-    const uint32_t ce = (item_ok(seg, B) && t < D.T && D.chunk <= kChunkMax) ? min(seg.size, cb + D.chunk) : 0u;
+    const uint32_t cmax = (seg.flags & F_IDENT) ? kChunkMaxIdent : kChunkMax;
+    const uint32_t ce = (item_ok(seg, B) && t < D.T && D.chunk <= cmax) ? min(seg.size, cb + D.chunk) : 0u;
     // kGroups 64-key groups per iteration: all loads are in flight before the
     // tree descents, then the groups are ranked in order (stable)
     constexpr int kGroups = 4;
@@ -392,16 +393,27 @@ __global__ __launch_bounds__(kWG) void k_scan_down(uint32_t* data, uint32_t len,
     }
 }
 
-// LDS layout of k_scatter (dynamic shared memory)
-constexpr uint32_t kScatKeyOff = 0;
-constexpr uint32_t kScatIdxOff = kScatKeyOff + kChunkMax * 8;
-constexpr uint32_t kScatBktOff = kScatIdxOff + kChunkMax * 4;
-constexpr uint32_t kScatStartOff = kScatBktOff + kChunkMax * 2;
-constexpr uint32_t kScatBaseOff = kScatStartOff + 2048 * 4;
-constexpr uint32_t kScatTmpOff = kScatBaseOff + 2048 * 4;
-constexpr uint32_t kScatLds = kScatTmpOff + 64 * 4;
+// LDS layout of k_scatter (dynamic shared memory).  In a level of root
+// segments (IDENT: indices = positions) a chunk stages 2-byte positions
+// instead of 4-byte indices, which leaves room for longer chunks: longer
+// runs per bucket, fewer partial cache lines written.
+template <bool IDENT>
+struct ScatLds {
+    static constexpr uint32_t kMax = IDENT ? kChunkMaxIdent : kChunkMax;
+    static constexpr int kPer = (kMax + kBigWG - 1) / kBigWG;   // keys per thread
+    static constexpr uint32_t kKeyOff = 0;
+    static constexpr uint32_t kIdxOff = kKeyOff + kMax * 8;
+    static constexpr uint32_t kBktOff = kIdxOff + kMax * (IDENT ? 2 : 4);
+    static constexpr uint32_t kStartOff = (kBktOff + kMax * 2 + 15) / 16 * 16;
+    static constexpr uint32_t kBaseOff = kStartOff + 2048 * 4;
+    static constexpr uint32_t kTmpOff = kBaseOff + 2048 * 4;
+    static constexpr uint32_t kBytes = kTmpOff + 64 * 4;
+};
+static_assert(ScatLds<false>::kBytes <= 160 * 1024 && ScatLds<true>::kBytes <= 160 * 1024,
+              "a chunk image fits a CU's LDS");
 
-// Move wave chunks (at most kChunkMax keys each) to their buckets (distribute,
+// Move wave chunks (at most kChunkMax keys each, kChunkMaxIdent in a level of
+// roots) to their buckets (distribute,
 // .hpp:1464-1490, but stable).  A chunk is first reordered by bucket in
 // LDS, then written out position by position, so consecutive lanes store
 // consecutive addresses inside each bucket: full cache lines instead of the
@@ -414,8 +426,6 @@ constexpr uint32_t kScatLds = kScatTmpOff + 64 * 4;
 // kXcds contiguous ranges, one per XCD (workgroup g runs on XCD g % kXcds),
 // so the chunks in flight on an XCD are neighbours that share histogram
 // lines and bucket output lines in its L2.
-constexpr int kScatPer = kChunkMax / kBigWG;
-static_assert(kChunkMax % kBigWG == 0, "a chunk fits a whole number of workgroup passes");
 
 // threadIdx.x through an opaque move: addresses derived from it are
 // recomputed per chunk instead of being hoisted out of the chunk loop,
@@ -427,11 +437,12 @@ __device__ __forceinline__ uint32_t scat_tid()
     return t;
 }
 
+template <int PER>
 struct ScatChunk {       // one chunk's description and its prefetched inputs
     LargeDesc D;
     uint32_t t, w, cb, m;
-    uint32_t br[kScatPer], iv[kScatPer];
-    key64 kv[kScatPer];
+    uint32_t br[PER], iv[PER];
+    key64 kv[PER];
     uint32_t h0, h0n, h1, h1n, hseg;   // histogram entries of buckets tid, tid + kBigWG; segment base
 };
 
@@ -440,9 +451,10 @@ struct ScatChunk {       // one chunk's description and its prefetched inputs
 // chunk is processed): the same number of loads every time lets the
 // compiler wait for exactly the older loads when the chunk is used, while
 // the next chunk's loads stay in flight.
+template <bool IDENT>
 __device__ __forceinline__ void scat_fetch(const LargeDesc* __restrict__ desc, const TileRef* __restrict__ tiles,
                                            const Bufs& B, const uint32_t* __restrict__ hscan, uint32_t c,
-                                           ScatChunk& S)
+                                           ScatChunk<ScatLds<IDENT>::kPer>& S)
 {
     const uint32_t tid = scat_tid();
     const TileRef tr = tiles[c / kWaves];
@@ -451,18 +463,18 @@ __device__ __forceinline__ void scat_fetch(const LargeDesc* __restrict__ desc, c
     S.D = desc[tr.seg];
     const Item seg = S.D.seg;
     S.cb = (S.t * kWaves + S.w) * S.D.chunk;
-    const bool ok = item_ok(seg, B) && S.t < S.D.T && S.D.chunk <= kChunkMax && S.cb < seg.size;
+    const bool ok = item_ok(seg, B) && S.t < S.D.T && S.D.chunk <= ScatLds<IDENT>::kMax && S.cb < seg.size;
     S.m = ok ? min(seg.size - S.cb, S.D.chunk) : 0u;
     const key64* K = (seg.flags & F_BUFB) ? B.key1 : B.key0;
     // a root's indices are implicit: its third load repeats the rank load
     const uint32_t* I = (seg.flags & F_IDENT) ? B.bktrank : (seg.flags & F_BUFB) ? B.idx1 : B.idx0;
     const uint32_t qmax = B.n - 1;
 #pragma unroll
-    for (int e = 0; e < kScatPer; ++e) {
+    for (int e = 0; e < ScatLds<IDENT>::kPer; ++e) {
         const uint32_t q = min(seg.begin + S.cb + e * kBigWG + tid, qmax);
         S.br[e] = B.bktrank[q];
         S.kv[e] = K[q];
-        S.iv[e] = I[q];
+        if (!IDENT) S.iv[e] = I[q];     // (a level of roots stages positions)
     }
     // histogram entries; entry e's count is scan[e + 1] - scan[e], and the
     // entry after the segment's last one is segbase + size (fixed up on use)
@@ -482,15 +494,19 @@ __device__ __forceinline__ void scat_fetch(const LargeDesc* __restrict__ desc, c
 // Stage one prefetched chunk in LDS, reordered by bucket (after the
 // previous chunk's image has been written out).  Returns false (uniformly)
 // for an empty or inconsistent chunk.
-__device__ __forceinline__ bool scat_stage(const ScatChunk& S, const Bufs& B, unsigned char* smem)
+template <bool IDENT>
+__device__ __forceinline__ bool scat_stage(const ScatChunk<ScatLds<IDENT>::kPer>& S, const Bufs& B,
+                                           unsigned char* smem)
 {
+    using L = ScatLds<IDENT>;
     const uint32_t tid = scat_tid();
-    key64* s_key = reinterpret_cast<key64*>(smem + kScatKeyOff);
-    uint32_t* s_idx = reinterpret_cast<uint32_t*>(smem + kScatIdxOff);
-    uint16_t* s_bkt = reinterpret_cast<uint16_t*>(smem + kScatBktOff);
-    uint32_t* s_start = reinterpret_cast<uint32_t*>(smem + kScatStartOff);
-    uint32_t* s_base = reinterpret_cast<uint32_t*>(smem + kScatBaseOff);
-    uint32_t* s_tmp = reinterpret_cast<uint32_t*>(smem + kScatTmpOff);
+    key64* s_key = reinterpret_cast<key64*>(smem + L::kKeyOff);
+    uint32_t* s_idx = reinterpret_cast<uint32_t*>(smem + L::kIdxOff);
+    uint16_t* s_pos = reinterpret_cast<uint16_t*>(smem + L::kIdxOff);
+    uint16_t* s_bkt = reinterpret_cast<uint16_t*>(smem + L::kBktOff);
+    uint32_t* s_start = reinterpret_cast<uint32_t*>(smem + L::kStartOff);
+    uint32_t* s_base = reinterpret_cast<uint32_t*>(smem + L::kBaseOff);
+    uint32_t* s_tmp = reinterpret_cast<uint32_t*>(smem + L::kTmpOff);
     __syncthreads();     // the previous chunk's LDS image has been written out
     const uint32_t m = S.m;
     if (m == 0) return false;
@@ -522,43 +538,50 @@ __device__ __forceinline__ bool scat_stage(const ScatChunk& S, const Bufs& B, un
         return false;
     }
 #pragma unroll
-    for (int e = 0; e < kScatPer; ++e) {
+    for (int e = 0; e < L::kPer; ++e) {
         const uint32_t p = e * kBigWG + tid;
         if (p >= m) continue;
         const uint32_t b = S.br[e] >> kRankBits;
         const uint32_t l = b < nbkt ? s_start[b] + (S.br[e] & kRankMask) : 0xFFFFFFFFu;
         if (l >= m) { flag_error(B.cnt, ERR_BOUNDS); continue; }
         s_key[l] = S.kv[e];
-        s_idx[l] = (seg.flags & F_IDENT) ? seg.begin + S.cb + p : S.iv[e];
+        if (IDENT) s_pos[l] = (uint16_t)p;
+        else s_idx[l] = (seg.flags & F_IDENT) ? seg.begin + S.cb + p : S.iv[e];
         s_bkt[l] = (uint16_t)b;
     }
     __syncthreads();
     return true;
 }
 
-// Write a staged chunk of m keys of segment seg out, position by position.
-__device__ __forceinline__ void scat_write(const Item& seg, uint32_t m, const Bufs& B, unsigned char* smem)
+// Write a staged chunk of m keys of segment seg (chunk base cb) out, position
+// by position.
+template <bool IDENT>
+__device__ __forceinline__ void scat_write(const Item& seg, uint32_t m, uint32_t cb, const Bufs& B,
+                                           unsigned char* smem)
 {
+    using L = ScatLds<IDENT>;
     const uint32_t tid = scat_tid();
-    const key64* s_key = reinterpret_cast<const key64*>(smem + kScatKeyOff);
-    const uint32_t* s_idx = reinterpret_cast<const uint32_t*>(smem + kScatIdxOff);
-    const uint16_t* s_bkt = reinterpret_cast<const uint16_t*>(smem + kScatBktOff);
-    const uint32_t* s_start = reinterpret_cast<const uint32_t*>(smem + kScatStartOff);
-    const uint32_t* s_base = reinterpret_cast<const uint32_t*>(smem + kScatBaseOff);
+    const key64* s_key = reinterpret_cast<const key64*>(smem + L::kKeyOff);
+    const uint32_t* s_idx = reinterpret_cast<const uint32_t*>(smem + L::kIdxOff);
+    const uint16_t* s_pos = reinterpret_cast<const uint16_t*>(smem + L::kIdxOff);
+    const uint16_t* s_bkt = reinterpret_cast<const uint16_t*>(smem + L::kBktOff);
+    const uint32_t* s_start = reinterpret_cast<const uint32_t*>(smem + L::kStartOff);
+    const uint32_t* s_base = reinterpret_cast<const uint32_t*>(smem + L::kBaseOff);
     key64* KD = (seg.flags & F_BUFB) ? B.key0 : B.key1;
     uint32_t* ID = (seg.flags & F_BUFB) ? B.idx0 : B.idx1;
 #pragma unroll
-    for (int e = 0; e < kScatPer; ++e) {
+    for (int e = 0; e < L::kPer; ++e) {
         const uint32_t j = e * kBigWG + tid;
         if (j >= m) continue;
         const uint32_t b = s_bkt[j];
         const uint32_t rel = s_base[b] + (j - s_start[b]);
         if (rel >= seg.size) { flag_error(B.cnt, ERR_BOUNDS); continue; }
         KD[seg.begin + rel] = s_key[j];
-        ID[seg.begin + rel] = s_idx[j];
+        ID[seg.begin + rel] = IDENT ? seg.begin + cb + s_pos[j] : s_idx[j];
     }
 }
 
+template <bool IDENT>
 __global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* __restrict__ desc,
                                                      const TileRef* __restrict__ tiles, Bufs B,
                                                      const uint32_t* __restrict__ hscan, uint32_t nchunks)
@@ -573,14 +596,14 @@ __global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* __restrict_
     // next chunk's loads, which overlap the write-out (past the end of its
     // range a workgroup fetches its last chunk again, unused, so that every
     // iteration issues the same loads)
-    ScatChunk R;
-    scat_fetch(desc, tiles, B, hscan, c, R);
+    ScatChunk<ScatLds<IDENT>::kPer> R;
+    scat_fetch<IDENT>(desc, tiles, B, hscan, c, R);
     for (;;) {
         const Item seg = R.D.seg;
-        const uint32_t m = R.m;
-        const bool staged = scat_stage(R, B, smem);
-        scat_fetch(desc, tiles, B, hscan, min(c + per, chi - 1), R);
-        if (staged) scat_write(seg, m, B, smem);
+        const uint32_t m = R.m, cb = R.cb;
+        const bool staged = scat_stage<IDENT>(R, B, smem);
+        scat_fetch<IDENT>(desc, tiles, B, hscan, min(c + per, chi - 1), R);
+        if (staged) scat_write<IDENT>(seg, m, cb, B, smem);
         c += per;
         if (c >= chi) break;
     }
@@ -1796,8 +1819,13 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     TileRef* h_tiles = reinterpret_cast<TileRef*>(c->h_desc + nlarge);
     const TileRef* d_tiles = reinterpret_cast<const TileRef*>(c->d_desc + nlarge);
     uint64_t total = 0;
-    for (uint32_t i = 0; i < nlarge; ++i) total += c->h_large[i].size;
-    const uint64_t climit = std::min<uint64_t>(kChunkMax, std::max<uint64_t>(kChunkMin, div_up(total, kWaves * kMinTiles)));
+    bool all_ident = true;     // a level of roots: the scatter stages positions, chunks may be longer
+    for (uint32_t i = 0; i < nlarge; ++i) {
+        total += c->h_large[i].size;
+        all_ident = all_ident && (c->h_large[i].flags & F_IDENT);
+    }
+    const uint64_t cmax = all_ident ? kChunkMaxIdent : kChunkMax;
+    const uint64_t climit = std::min<uint64_t>(cmax, std::max<uint64_t>(kChunkMin, div_up(total, kWaves * kMinTiles)));
     for (uint32_t i = 0; i < nlarge; ++i) {
         const Item seg = c->h_large[i];
         const uint32_t T = div_up(seg.size, climit * kWaves);
@@ -1857,8 +1885,12 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         Launch L(c, K_SCATTER, elems * 28 - elems_ident * 4 + hl * 8);
         const uint32_t nchunks = (uint32_t)ntiles * kWaves;
         const uint32_t wgs = std::min(c->scatter_wgs, div_up(nchunks, kXcds) * kXcds);
-        hipLaunchKernelGGL(k_scatter, dim3(wgs), dim3(kBigWG), kScatLds, st, c->d_desc, d_tiles, B, c->hist,
-                           nchunks);
+        if (all_ident)
+            hipLaunchKernelGGL(k_scatter<true>, dim3(wgs), dim3(kBigWG), ScatLds<true>::kBytes, st, c->d_desc, d_tiles,
+                               B, c->hist, nchunks);
+        else
+            hipLaunchKernelGGL(k_scatter<false>, dim3(wgs), dim3(kBigWG), ScatLds<false>::kBytes, st, c->d_desc,
+                               d_tiles, B, c->hist, nchunks);
     }
     {
         Launch L(c, K_CHILDREN, hl * 4 / kWaves);
@@ -2131,7 +2163,10 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     if (!out) return set_err(PS5_ERR_ARG, "out is NULL");
     *out = nullptr;
     HIPCHK(hipSetDevice(device));
-    HIPCHK(hipFuncSetAttribute((const void*)k_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, kScatLds));
+    HIPCHK(hipFuncSetAttribute((const void*)k_scatter<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               ScatLds<false>::kBytes));
+    HIPCHK(hipFuncSetAttribute((const void*)k_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               ScatLds<true>::kBytes));
     for (const void* f : { (const void*)k_count<8>, (const void*)k_count<9>, (const void*)k_count<10>,
                            (const void*)k_count<11> })
         HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, count_lds_bytes(kMaxTreebits)));
