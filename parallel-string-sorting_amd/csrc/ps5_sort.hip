@@ -412,9 +412,9 @@ struct ScatLds {
 static_assert(ScatLds<false>::kBytes <= 160 * 1024 && ScatLds<true>::kBytes <= 160 * 1024,
               "a chunk image fits a CU's LDS");
 
-// Move wave chunks (at most kChunkMax keys each, kChunkMaxIdent in a level of
-// roots) to their buckets (distribute,
-// .hpp:1464-1490, but stable).  A chunk is first reordered by bucket in
+// Move wave chunks (at most kChunkMax keys each, kChunkMaxIdent in a level
+// of roots) to their buckets (distribute, .hpp:1464-1490, but stable).  A
+// chunk is first reordered by bucket in
 // LDS, then written out position by position, so consecutive lanes store
 // consecutive addresses inside each bucket: full cache lines instead of the
 // scattered 8-byte stores of a direct scatter.
