@@ -370,6 +370,18 @@ def test_ascii2gb_full(ps5, ctx, oracle):
     assert r == 0, msg
 
 
+@pytest.mark.slow
+def test_large_root_duplicates(ps5, ctx, oracle):
+    """1 GiB of 0..12-character strings over 'ab' (153 M strings, almost all
+    duplicated): a root large enough for the longest root chunks, with heavy
+    equal buckets; verified exactly by the oracle's verifier"""
+    say("large root duplicates")
+    arena, offs = ps5.gen_uniform(0xB17, 1 << 30, 0, 12, b"ab")
+    perm, lcp = ctx.sort(arena, offs)
+    r, msg = oracle.verify(arena, offs, perm, lcp)
+    assert r == 0, msg
+
+
 def _check_batch(oracle, arena, offs64, set_first, perm, lcp):
     for s in range(len(set_first) - 1):
         b, e = int(set_first[s]), int(set_first[s + 1])
