@@ -27,6 +27,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+METRIC = "sorted chars/s (GB/s) & % HBM roofline; batched vs plain sorts/s, p99 latency"
 
 
 def load_ps5():
@@ -43,28 +44,66 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(ps5, workload, budget_s):
-    """The oracle's pS5 restatement (plain contestant, no LCP) on a prefix of
-    the same workload, timed on the host cores."""
+def host_info():
+    """The host the CPU baseline runs on: CPU model, logical CPUs, and the
+    cores this process may use (affinity mask, OMP_NUM_THREADS)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    usable = max(1, min(aff, int(omp))) if omp and omp.isdigit() else aff
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": aff,
+            "omp_num_threads": omp, "threads_used": usable}
+
+
+def cpu_baseline(ps5, workload, arena, offs, budget_s):
+    """The oracle's pS5 restatement timed on the host cores this process may
+    use (every CPU of its affinity mask, capped by OMP_NUM_THREADS: the GPU
+    box grants 16 cores per GPU), on the same workload as the GPU step.
+    Primary figure: with the LCP array (parallel_sample_sort_lcp_base), the
+    output the GPU step produces; the plain contestant (no LCP) beside it.
+    Each timed sort's own output is verified exactly afterwards."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ps5_oracle
-    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
-    # grow the prefix until one run takes a measurable share of the budget
-    frac = 1 / 64
-    while True:
-        arena, offs = ps5.workload(workload, scale=frac)
-        t = ps5_oracle.time_ps5(arena, offs, threads=threads, with_lcp=False)
-        if t * 4 > budget_s or frac >= 1.0:
-            break
-        frac = min(1.0, frac * 4)
-    runs = []
-    while sum(runs) < budget_s and len(runs) < 5:
-        runs.append(ps5_oracle.time_ps5(arena, offs, threads=threads, with_lcp=False))
+    hi = host_info()
+    threads = hi["threads_used"]
+    n = len(offs)
+    # a prefix of the workload when one run of the whole would take more than
+    # a quarter of the budget (strings are laid out in input order)
+    k = n
+    t, perm, lcp = ps5_oracle.time_ps5(arena, offs, threads=threads, with_lcp=True, output=True)
+    while t * 4 > budget_s and k > 1024:
+        k = max(1024, int(k * (budget_s / 4) / t))
+        t, perm, lcp = ps5_oracle.time_ps5(arena, offs[:k], threads=threads, with_lcp=True, output=True)
+    sub = offs[:k]
+    chars = int(offs[k]) if k < n else int(arena.size) - 16
+    runs = [t]
+    verified = ps5_oracle.verify(arena, sub, perm, lcp)[0] == 0
+    while sum(runs) < budget_s / 2 and len(runs) < 5:
+        t, perm, lcp = ps5_oracle.time_ps5(arena, sub, threads=threads, with_lcp=True, output=True)
+        runs.append(t)
+        verified = verified and ps5_oracle.verify(arena, sub, perm, lcp)[0] == 0
+    nolcp = []
+    while sum(nolcp) < budget_s / 4 and len(nolcp) < 3:
+        t, perm, _ = ps5_oracle.time_ps5(arena, sub, threads=threads, with_lcp=False, output=True)
+        nolcp.append(t)
+        verified = verified and ps5_oracle.verify(arena, sub, perm)[0] == 0
     med = float(np.median(runs))
-    chars = int(arena.size)
+    whole = "the whole workload" if k == n else f"a prefix of {k} of its {n} strings"
     return {"value": round(chars / med / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"{workload} prefix of {chars} bytes ({len(offs)} strings), median of {len(runs)} runs "
-                      f"of the oracle's pS5 restatement (treebits 10, no LCP) with {threads} OpenMP threads"}
+            "sample": f"{workload}: {whole} ({chars} bytes), median of {len(runs)} runs of the oracle's pS5 "
+                      f"restatement with the LCP array (treebits 10, oversample 2), {threads} OpenMP threads",
+            "ms": round(med * 1e3, 3), "value_without_lcp": round(chars / float(np.median(nolcp)) / 1e9, 4),
+            "output_verified": bool(verified), "host": hi}
 
 
 def bench_batch(ps5, args):
@@ -84,7 +123,8 @@ def bench_batch(ps5, args):
     d_arena.upload(arena)
     d_offs.upload(offs)
     tuning = ctx.get_tuning()
-    if args.autotune:
+    autotune = args.autotune is not False        # on unless --no-autotune
+    if autotune:
         t0 = time.time()
         tuning, tms = ctx.autotune_batch_device(d_arena, int(arena.size), d_offs, sf, d_perm, d_lcp)
         print(f"[bench] autotune: leaf_target/max_treebits/max_oversample = {tuning}, {tms:.3f} ms "
@@ -150,10 +190,10 @@ def bench_batch(ps5, args):
            "speedup": round(plain_s / batched_s, 2), "batched_chars_gbs": round(arena.size / batched_s / 1e9, 2),
            "identical_to_plain": same, "oracle_checked_sets": (args.sets + 15) // 16, "oracle_failures": bad,
            "tuning": {"leaf_target": tuning[0], "max_treebits": tuning[1], "max_oversample": tuning[2],
-                      "autotuned": bool(args.autotune)},
+                      "autotuned": autotune},
            "config": {"workload": f"{args.sets} sets, strings log-uniform [{args.count_lo}, {args.count_hi}], "
                                   "length U[4,48], alphabet ACGT/digits/bytes 33..126, seed 0x5EED0004"}}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def calibration_copy(ps5):
@@ -309,7 +349,7 @@ def bench_serve(ps5, args):
                                   "one of 1000 prefixes (Zipf 1.1) + 10..60 random chars, Poisson arrivals at "
                                   f"{args.load} of the plain path's capacity, seed 0x5EED0005",
                       "latency": "submit (host arrays) to results in host memory, incl. PCIe both ways"}}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def _kernel_base(name):
@@ -429,12 +469,16 @@ def main():
     ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--max-batch", type=int, default=16 << 20, help="strings per served batch (buffers pre-allocated)")
     ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=None,
-                    help="tune the sort's shape on the workload before timing (default: on in batch mode)")
+                    help="tune the sort's shape on the workload before timing (default: on for the batch, "
+                         "off for single sorts)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="ascii2gb", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--extras", action=argparse.BooleanOptionalAction, default=True,
+                    help="also measure the 10 MiB and high-LCP sorts, the batch and the serving comparison "
+                         "(default on; --no-extras for the headline alone)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--profile", action="store_true", help="print the per-kernel breakdown to stderr")
     ap.add_argument("--calibrate", action="store_true",
@@ -447,25 +491,78 @@ def main():
 
     if args.rocprof_summary:
         return rocprof_summary(args.rocprof_summary)
-    if args.autotune is None:
-        args.autotune = args.mode == "batch"
     ps5 = load_ps5()
     if args.mode == "batch":
-        return bench_batch(ps5, args)
+        print(json.dumps(bench_batch(ps5, args)), flush=True)
+        return 0
     if args.mode == "serve":
-        return bench_serve(ps5, args)
+        print(json.dumps(bench_serve(ps5, args)), flush=True)
+        return 0
     if args.mode == "merge":
         return bench_merge(ps5, args)
+    t_begin = time.time()
+    head = measure_plain(ps5, args, args.workload, args.steps, args.warmup, kernels=True)
+    out = {
+        "metric": METRIC,
+        "value": head["gbs"],
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": head["config"],
+        "roofline": head["roofline"],
+        "cpu_baseline": head.get("cpu_baseline"),
+        "output_verified": head["verified"],
+    }
+    if not head["verified"]:
+        print(json.dumps(out), flush=True)
+        return 1
+    ok = True
+    if args.extras:
+        # the other single-GPU configurations, each with its CPU baseline
+        configs = {}
+        for w in [x for x in ("random10mb", "highlcp") if x != args.workload]:
+            r = measure_plain(ps5, args, w, max(args.steps, 10), args.warmup, kernels=False)
+            ok = ok and r["verified"]
+            configs[w] = {"value": r["gbs"], "unit": "GB/s", "ms_per_step": r["ms_per_step"],
+                          "device_ms": r["device_ms"], "output_verified": r["verified"],
+                          "config": r["config"], "work": r["work"], "cpu_baseline": r.get("cpu_baseline")}
+            print(f"[bench] {w}: {configs[w]}", file=sys.stderr, flush=True)
+        out["configs"] = configs
+        # batched vs plain sorts/s, then served vs plain p50/p99
+        for name, fn in (("batch", bench_batch), ("serve", bench_serve)):
+            t0 = time.time()
+            try:
+                out[name] = fn(ps5, args)
+            except Exception as e:       # report, keep the headline line
+                out[name] = {"error": f"{type(e).__name__}: {e}"}
+                ok = False
+            print(f"[bench] {name} ({time.time() - t0:.1f}s): {out[name]}", file=sys.stderr, flush=True)
+    out["bench_wall_s"] = round(time.time() - t_begin, 1)
+    print(json.dumps(out), flush=True)
+    return 0 if ok else 1
+
+
+def measure_plain(ps5, args, workload, steps, warmup, kernels):
+    """One device-resident sort per step of `workload`: times `steps` sorts
+    after `warmup`, verifies the timed steps' own output exactly, then
+    (kernels=True) one extra profiled sort for the per-kernel roofline, and
+    the CPU baseline on the same input."""
     t0 = time.time()
-    arena, offs = ps5.workload(args.workload)
+    arena, offs = ps5.workload(workload)
     n = len(offs)
     chars = int(arena.size) - 16          # arena bytes including NULs (16 bytes of padding excluded)
-    print(f"[bench] generated {args.workload}: {n} strings, {chars} bytes in {time.time() - t0:.1f}s",
+    print(f"[bench] generated {workload}: {n} strings, {chars} bytes in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
-
+    lib = ps5.load()
     ctx = ps5.Context(0)
-    ctx_reserve = ps5.load().ps5_ctx_reserve(ctx.h, n)
-    assert ctx_reserve == 0
+    assert lib.ps5_ctx_reserve(ctx.h, n) == 0
     d_arena = ps5.DeviceBuffer(arena.size)
     d_offs = ps5.DeviceBuffer(n * 4)
     d_perm = ps5.DeviceBuffer(n * 4)
@@ -473,8 +570,7 @@ def main():
     d_arena.upload(arena)
     d_offs.upload(offs)
     alen = int(arena.size)
-
-    if args.calibrate:
+    if args.calibrate and kernels:
         calibration_copy(ps5)
     tuning = ctx.get_tuning()
     if args.autotune:
@@ -482,26 +578,60 @@ def main():
         tuning, tms = ctx.autotune_device(d_arena, alen, d_offs, n, d_perm, d_lcp)
         print(f"[bench] autotune: leaf_target/max_treebits/max_oversample = {tuning}, {tms:.3f} ms "
               f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         ctx.sort_device(d_arena, alen, d_offs, n, d_perm, d_lcp)
     step_ms = []
-    lib = ps5.load()
+    lib.ps5_device_synchronize()
+    # poison the outputs: what is verified below was written by the timed steps
+    lib.ps5_dev_copy(d_perm.ptr, d_offs.ptr, (n * 4) // 16 * 16)
+    lib.ps5_dev_copy(d_lcp.ptr, d_offs.ptr, (n * 4) // 16 * 16)
     lib.ps5_device_synchronize()
     t_start = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         step_ms.append(ctx.sort_device(d_arena, alen, d_offs, n, d_perm, d_lcp, timed=True))
     lib.ps5_device_synchronize()
     wall = time.perf_counter() - t_start
-    ms_per_step = wall * 1e3 / args.steps
-    value = chars / (wall / args.steps) / 1e9
-    print(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step wall, device {np.median(step_ms):.3f} ms median",
-          file=sys.stderr, flush=True)
+    ms_per_step = wall * 1e3 / steps
+    work = ctx.work_stats()
+    print(f"[bench] {workload} {steps} steps: {ms_per_step:.3f} ms/step wall, device {np.median(step_ms):.3f} ms "
+          f"median, work {work}", file=sys.stderr, flush=True)
+    # exact check of the last timed step's output (the oracle's verifier:
+    # permutation, order, tie order, every LCP)
+    perm = d_perm.download(np.uint32, n)
+    lcp = d_lcp.download(np.uint32, n)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ps5_oracle
+    vr, vmsg = ps5_oracle.verify(arena, offs, perm, lcp)
+    if vr != 0:
+        print(f"[bench] OUTPUT WRONG ({workload}): {vmsg}", file=sys.stderr, flush=True)
+    else:
+        print(f"[bench] {workload}: timed output verified (permutation, order, tie order, LCP)",
+              file=sys.stderr, flush=True)
+    res = {"n": n, "chars": chars, "ms_per_step": round(ms_per_step, 4),
+           "gbs": round(chars / (wall / steps) / 1e9, 3), "device_ms": round(float(np.median(step_ms)), 4),
+           "verified": vr == 0, "work": work,
+           "config": {"workload": WORKLOADS[workload], "strings": n, "arena_bytes": chars,
+                      "output": "permutation + LCP array", "device_resident": True,
+                      "tuning": {"leaf_target": tuning[0], "max_treebits": tuning[1], "max_oversample": tuning[2],
+                                 "autotuned": bool(args.autotune)}}}
+    if kernels:
+        res["roofline"] = kernel_roofline_of(ctx, args, d_arena, alen, d_offs, n, d_perm, d_lcp, workload,
+                                             float(np.median(step_ms)))
+    ctx.close()
+    del d_arena, d_offs, d_perm, d_lcp
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(ps5, workload, arena, offs, args.cpu_budget)
+        print(f"[bench] cpu baseline {workload}: {res['cpu_baseline']}", file=sys.stderr, flush=True)
+    return res
 
-    # per-kernel breakdown with HIP events on the context's stream
+
+def kernel_roofline_of(ctx, args, d_arena, alen, d_offs, n, d_perm, d_lcp, workload, sort_ms):
+    """Per-kernel breakdown of one extra sort, each launch bracketed by HIP
+    events on the context's stream: the dominant kernel's algorithmic bytes
+    per launch over its average launch time, against the HBM peak."""
     ctx.set_profile(True)
     ctx.sort_device(d_arena, alen, d_offs, n, d_perm, d_lcp)
     ks = ctx.kernel_stats()
-    ws = ctx.work_stats()
     ctx.set_profile(False)
     ranked = sorted((k for k in ks if ks[k]["launches"]), key=lambda k: -ks[k]["ms"])
     dom = ranked[0]
@@ -515,10 +645,9 @@ def main():
         a_ms = v["ms"] / v["launches"]
         a = v["bytes"] / v["launches"] / (a_ms * 1e-3) / 1e9
         return {"kernel": k, "achieved": round(a, 2), "frac": round(a / HBM_PEAK_GBS, 4),
-                "traffic": profiled_traffic(k, args.workload), "avg_launch_ms": round(a_ms, 4),
+                "traffic": profiled_traffic(k, workload), "avg_launch_ms": round(a_ms, 4),
                 "alg_bytes_per_launch": int(v["bytes"] / v["launches"])}
     sort_bytes = sum(v["bytes"] for v in ks.values())       # every pass of one sort
-    sort_ms = float(np.median(step_ms))
     if args.profile:
         tot = sum(v["ms"] for v in ks.values())
         for k, v in sorted(ks.items(), key=lambda kv: -kv[1]["ms"]):
@@ -526,54 +655,15 @@ def main():
                 gbs = v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0
                 print(f"[profile] {k:14s} {v['ms']:9.3f} ms {100 * v['ms'] / tot:5.1f}%  launches {v['launches']:6d} "
                       f" alg {v['bytes'] / 1e9:8.3f} GB  {gbs:8.1f} GB/s", file=sys.stderr)
-        print(f"[profile] work {ws}", file=sys.stderr)
-
-    # correctness of the measured output (cheap exact verifier of the oracle)
-    perm = d_perm.download(np.uint32, n)
-    lcp = d_lcp.download(np.uint32, n)
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import ps5_oracle
-    vr, vmsg = ps5_oracle.verify(arena, offs, perm, lcp)
-    if vr != 0:
-        print(f"[bench] OUTPUT WRONG: {vmsg}", file=sys.stderr)
-        sys.exit(1)
-    print("[bench] output verified (permutation, order, tie order, LCP)", file=sys.stderr, flush=True)
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(ps5, args.workload, args.cpu_budget)
-        print(f"[bench] cpu baseline {cpu}", file=sys.stderr, flush=True)
-
-    out = {
-        "metric": "sorted chars/s (GB/s)",
-        "value": round(value, 3),
-        "unit": "GB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic",
-        "config": {"workload": WORKLOADS[args.workload], "strings": n, "arena_bytes": chars,
-                   "output": "permutation + LCP array", "device_resident": True,
-                   "tuning": {"leaf_target": tuning[0], "max_treebits": tuning[1], "max_oversample": tuning[2],
-                              "autotuned": bool(args.autotune)}},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic(dom, args.workload),
-                     "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": int(bytes_per_launch),
-                     "runner_up": kernel_roofline(ranked[1]) if len(ranked) > 1 else None,
-                     "whole_sort": {"alg_bytes": int(sort_bytes), "device_ms": round(sort_ms, 4),
-                                    "achieved": round(sort_bytes / (sort_ms * 1e-3) / 1e9, 2),
-                                    "frac": round(sort_bytes / (sort_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
-    ctx.close()
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": profiled_traffic(dom, workload),
+            "avg_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": int(bytes_per_launch),
+            "runner_up": kernel_roofline(ranked[1]) if len(ranked) > 1 else None,
+            "whole_sort": {"alg_bytes": int(sort_bytes), "device_ms": round(sort_ms, 4),
+                           "achieved": round(sort_bytes / (sort_ms * 1e-3) / 1e9, 2),
+                           "frac": round(sort_bytes / (sort_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 if __name__ == "__main__":
-    main()
+    sys.exit(main())

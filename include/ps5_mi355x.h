@@ -56,9 +56,15 @@ void mi355x_ps5_sort(unsigned char** strings, size_t n);
 void mi355x_ps5_sort_lcp(unsigned char** strings, uintptr_t* lcp, size_t n);
 
 /* Replaces parallel_sample_sortBTCUI_out (bingmann-parallel_sample_sort.cpp:
- * 44-62, _out_base .hpp:1704-1714): sorted pointers go to output[], strings[]
- * is left as it was.  abort()s on failure. */
-void mi355x_ps5_sort_out(unsigned char** strings, unsigned char** output, size_t n);
+ * 44-62), Contest func_type (contest.hpp:88): sorts into a separate output
+ * array, then copies it back into strings[] (as that contestant does, for
+ * psstest's checks).  abort()s on failure. */
+void mi355x_ps5_sort_out(unsigned char** strings, size_t n);
+
+/* Replaces parallel_sample_sort_out_base (bingmann-parallel_sample_sort.hpp:
+ * 1704-1714): the sorted pointers go to output[0..n), strings[] is left as it
+ * was.  abort()s on failure. */
+void mi355x_ps5_sort_out_base(unsigned char** strings, unsigned char** output, size_t n);
 
 /* Behind the Contest func_lcp_cache_type signature (contest.hpp:90-91): sorts,
  * fills lcp[1..n-1] and the distinguishing characters cache[i] = s_i[lcp[i]]
@@ -223,15 +229,28 @@ int ps5_server_create(ps5_server** out, int device, uint64_t hbm_budget, uint32_
 void ps5_server_destroy(ps5_server* srv);
 /* Submit host strings (arena + uint32 offsets, caller keeps them alive until
  * ps5_server_wait returns); results go to perm[n] and lcp[n] (lcp may be
- * NULL, lcp[0] untouched).  Thread-safe.  Returns a request id (0 on error). */
+ * NULL, lcp[0] untouched).  Thread-safe.  Returns a request id, or 0 when the
+ * request is rejected (ps5_last_error says why): a malformed request -- an
+ * offset at or past arena_bytes, or an arena whose last byte is not a NUL --
+ * never enters a batch.  Failure isolation: a batch that fails is run again
+ * one request at a time, so only a request that fails on its own reports an
+ * error; when no slot could start, every request completes with an error. */
 uint64_t ps5_server_submit(ps5_server* srv, const uint8_t* arena, uint64_t arena_bytes,
                            const uint32_t* offs, uint64_t n, uint32_t* perm, uint32_t* lcp);
-/* Block until request `id` is done; its latency from submit to completion. */
+/* Block until request `id` is done; its latency from submit to completion.
+ * Returns the request's own status (its message in ps5_last_error). */
 int ps5_server_wait(ps5_server* srv, uint64_t id, double* latency_ms);
 /* Latency percentiles over completed requests, batches run, and throughput
  * (arena bytes of completed requests / time from first submit to last finish). */
 int ps5_server_stats(ps5_server* srv, double* p50_ms, double* p99_ms, uint64_t* completed,
                      uint64_t* batches, double* throughput_gbs);
+/* Slots running now, and requests completed with an error so far. */
+int ps5_server_health(ps5_server* srv, uint32_t* live_slots, uint64_t* failed);
+/* Test hook: the request that will receive id `id` (ids count up from 1 in
+ * submission order) fails inside its batch as a device error would, which
+ * exercises the failure isolation from the host.  No counterpart in the
+ * reference. */
+int ps5_server_inject_fault(ps5_server* srv, uint64_t id);
 
 /* ---- synthetic inputs (seeded, host memory) ---------------------------- */
 

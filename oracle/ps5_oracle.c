@@ -784,7 +784,28 @@ static int cmp_u64pair(const void* a, const void* b)
     return x < y ? -1 : x > y;
 }
 
-/* pS5 over arena+offs, returning the canonical permutation and LCP array. */
+/* Input index of every sorted string pointer s[i] (offsets are distinct),
+ * then runs of identical strings in ascending input index. */
+static void ptrs_to_perm(const uint8_t* arena, const uint64_t* offs, uint64_t n, str_t* s, uint32_t* perm)
+{
+    uint64_t* tab = (uint64_t*)malloc(2 * n * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) { tab[2 * i] = offs[i]; tab[2 * i + 1] = i; }
+    qsort(tab, n, 2 * sizeof(uint64_t), cmp_u64pair);
+    #pragma omp parallel for schedule(static)
+    for (uint64_t i = 0; i < n; ++i) {
+        uint64_t o = (uint64_t)(s[i] - arena);
+        uint64_t lo = 0, hi = n;
+        while (lo + 1 < hi) { uint64_t m = (lo + hi) / 2; if (tab[2 * m] <= o) lo = m; else hi = m; }
+        perm[i] = (uint32_t)tab[2 * lo + 1];
+    }
+    free(tab);
+    ps5o_canonicalize(arena, offs, n, perm);
+}
+
+/* pS5 over arena+offs, returning the canonical permutation and, if lcp is
+ * not NULL, the LCP array.  With lcp == NULL the restatement runs the plain
+ * contestant's branches (parallel_sample_sortBTCUI: no LCP array, generic
+ * insertion sort, .hpp:1685-1700, inssort.hpp:61-97). */
 void ps5o_ps5_sort(const uint8_t* arena, const uint64_t* offs, uint64_t n,
                    uint32_t* perm, uint32_t* lcp, int nthreads, uint64_t seed)
 {
@@ -793,26 +814,19 @@ void ps5o_ps5_sort(const uint8_t* arena, const uint64_t* offs, uint64_t n,
     for (uint64_t i = 0; i < n; ++i) s[i] = arena + offs[i];
     uint64_t* l = lcp ? (uint64_t*)malloc(n * sizeof(uint64_t)) : NULL;
     ps5o_ps5_sort_ptrs(s, n, l, nthreads, seed);
-    /* map string pointers back to input indices (offsets are distinct) */
-    uint64_t* tab = (uint64_t*)malloc(2 * n * sizeof(uint64_t));
-    for (uint64_t i = 0; i < n; ++i) { tab[2 * i] = offs[i]; tab[2 * i + 1] = i; }
-    qsort(tab, n, 2 * sizeof(uint64_t), cmp_u64pair);
-    for (uint64_t i = 0; i < n; ++i) {
-        uint64_t o = (uint64_t)(s[i] - arena);
-        uint64_t lo = 0, hi = n;
-        while (lo + 1 < hi) { uint64_t m = (lo + hi) / 2; if (tab[2 * m] <= o) lo = m; else hi = m; }
-        perm[i] = (uint32_t)tab[2 * lo + 1];
-    }
-    free(tab);
+    ptrs_to_perm(arena, offs, n, s, perm);
     if (lcp) for (uint64_t i = 1; i < n; ++i) lcp[i] = (uint32_t)l[i];
     free(l);
     free(s);
-    ps5o_canonicalize(arena, offs, n, perm);
 }
 
-/* timed CPU baseline: the plain contestant (no LCP), on arena+offs */
+/* Timed CPU baseline: only the sort call is timed (src/psstest.cpp:593-603),
+ * with_lcp selecting parallel_sample_sort_lcp_base over the plain contestant.
+ * Afterwards, outside the timed region, the canonical permutation (and the
+ * LCP array when with_lcp) go to perm/lcp if they are not NULL, so that the
+ * timed sort's own output can be verified. */
 double ps5o_time_ps5(const uint8_t* arena, const uint64_t* offs, uint64_t n,
-                     int nthreads, int with_lcp)
+                     int nthreads, int with_lcp, uint32_t* perm, uint32_t* lcp)
 {
     str_t* s = (str_t*)malloc((n ? n : 1) * sizeof(str_t));
     for (uint64_t i = 0; i < n; ++i) s[i] = arena + offs[i];
@@ -820,6 +834,8 @@ double ps5o_time_ps5(const uint8_t* arena, const uint64_t* offs, uint64_t n,
     double t0 = omp_get_wtime();
     ps5o_ps5_sort_ptrs(s, n, l, nthreads, 0x5EED);
     double t1 = omp_get_wtime();
+    if (perm && n) ptrs_to_perm(arena, offs, n, s, perm);
+    if (lcp && l) for (uint64_t i = 1; i < n; ++i) lcp[i] = (uint32_t)l[i];
     free(l);
     free(s);
     return t1 - t0;

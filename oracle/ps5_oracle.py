@@ -52,7 +52,7 @@ def load():
     lib.ps5o_ps5_sort.restype = None
     lib.ps5o_ps5_sort.argtypes = [_P, _P, _U64, _P, _P, ctypes.c_int, _U64]
     lib.ps5o_time_ps5.restype = ctypes.c_double
-    lib.ps5o_time_ps5.argtypes = [_P, _P, _U64, ctypes.c_int, ctypes.c_int]
+    lib.ps5o_time_ps5.argtypes = [_P, _P, _U64, ctypes.c_int, ctypes.c_int, _P, _P]
     lib.ps5o_max_threads.restype = ctypes.c_int
     lib.ps5o_max_threads.argtypes = []
     _lib = lib
@@ -77,14 +77,15 @@ def brute_sort(arena, offs):
     return perm[:n], lcp[:n]
 
 
-def ps5_sort(arena, offs, threads=0, seed=7):
-    """The pS5 restatement, ties canonicalised: (perm, lcp)."""
+def ps5_sort(arena, offs, threads=0, seed=7, with_lcp=True):
+    """The pS5 restatement, ties canonicalised: (perm, lcp).  with_lcp=False
+    runs the plain contestant (no LCP array; lcp is then None)."""
     n = len(offs)
     o = offs64(offs)
     perm = np.zeros(max(1, n), np.uint32)
-    lcp = np.zeros(max(1, n), np.uint32)
-    load().ps5o_ps5_sort(_p(arena), _p(o), n, _p(perm), _p(lcp), threads, seed)
-    return perm[:n], lcp[:n]
+    lcp = np.zeros(max(1, n), np.uint32) if with_lcp else None
+    load().ps5o_ps5_sort(_p(arena), _p(o), n, _p(perm), _p(lcp) if with_lcp else None, threads, seed)
+    return perm[:n], (lcp[:n] if with_lcp else None)
 
 
 def verify(arena, offs, perm, lcp=None):
@@ -112,9 +113,29 @@ def dist_chars(arena, offs, perm, lcp):
     return cache[:n]
 
 
-def time_ps5(arena, offs, threads=0, with_lcp=False):
+def time_ps5(arena, offs, threads=0, with_lcp=False, output=False):
+    """Seconds of one timed pS5 sort (the sort call only).  With output=True
+    returns (seconds, perm, lcp) with the timed sort's own canonical output
+    (lcp None without with_lcp)."""
+    n = len(offs)
     o = offs64(offs)
-    return load().ps5o_time_ps5(_p(arena), _p(o), len(offs), threads, 1 if with_lcp else 0)
+    perm = np.zeros(max(1, n), np.uint32) if output else None
+    lcp = np.zeros(max(1, n), np.uint32) if output and with_lcp else None
+    t = load().ps5o_time_ps5(_p(arena), _p(o), n, threads, 1 if with_lcp else 0,
+                             _p(perm) if perm is not None else None, _p(lcp) if lcp is not None else None)
+    if not output:
+        return t
+    return t, perm[:n], (lcp[:n] if lcp is not None else None)
+
+
+def dprefix(arena, offs, perm):
+    """(D, lcpsum) of a sorted order: calc_distinguishing_prefix,
+    src/tools/checker.hpp:155-180."""
+    o = offs64(offs)
+    perm = np.ascontiguousarray(perm, np.uint32)
+    ls = _U64(0)
+    d = load().ps5o_dprefix(_p(arena), _p(o), len(offs), _p(perm), ctypes.byref(ls))
+    return int(d), int(ls.value)
 
 
 def get_key(buf, depth):

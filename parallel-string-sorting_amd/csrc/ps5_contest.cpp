@@ -262,9 +262,17 @@ extern "C" void mi355x_ps5_sort_lcp(unsigned char** strings, uintptr_t* lcp, siz
     die_on(ps5_sort_strings(strings, n, lcp, nullptr), "mi355x_ps5_sort_lcp");
 }
 
-extern "C" void mi355x_ps5_sort_out(unsigned char** strings, unsigned char** output, size_t n)
+extern "C" void mi355x_ps5_sort_out(unsigned char** strings, size_t n)
 {
-    die_on(ps5_sort_strings(strings, n, nullptr, output), "mi355x_ps5_sort_out");
+    // a separate output array, copied back (parallel_sample_sortBTCUI_out)
+    std::vector<unsigned char*> output(n);
+    die_on(ps5_sort_strings(strings, n, nullptr, output.data()), "mi355x_ps5_sort_out");
+    std::copy(output.begin(), output.end(), strings);
+}
+
+extern "C" void mi355x_ps5_sort_out_base(unsigned char** strings, unsigned char** output, size_t n)
+{
+    die_on(ps5_sort_strings(strings, n, nullptr, output), "mi355x_ps5_sort_out_base");
 }
 
 extern "C" void mi355x_ps5_sort_lcp_cache(unsigned char** strings, uintptr_t* lcp, unsigned char* cache, size_t n)

@@ -7,6 +7,9 @@
 
 namespace ps5 {
 
+// Record the calling thread's last error (ps5_last_error) and return code.
+int set_error(int code, const char* fmt, ...);
+
 // Absolute uint64 offsets of a batch staged set by set: for string i of set s
 // (set_first[s] <= i < set_first[s+1]), offs64[i] = set_base[s] + offs32[i].
 // d_first / d_base are device arrays of nsets+1 / nsets entries.  Enqueued on

@@ -9,6 +9,13 @@
 // admitted as soon as a slot and memory free up.  Each request's result
 // depends only on its own strings.  Requests are copied between their host
 // arrays and the slot's device buffers directly; device buffers only grow.
+//
+// Failure isolation (the role of psstest's fork-per-run status
+// classification, src/psstest.cpp:310-370): a malformed request is rejected
+// at submit; a batch that fails is run again one request at a time, so only
+// the request that fails on its own reports an error; a slot that cannot
+// start leaves the others running, and when no slot is left every waiting and
+// later request completes with an error instead of blocking its waiter.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,8 +24,10 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/ps5_mi355x.h"
@@ -40,6 +49,8 @@ struct Request {
     Clock::time_point t_submit, t_start, t_done;
     int status = PS5_OK;
     bool done = false;
+    bool inject_fault = false;    // test hook: fail this request's sort as a device error would
+    std::string why;              // error message of a failed request
 };
 
 }  // namespace
@@ -57,9 +68,14 @@ struct ps5_server {
     uint64_t next_id = 1;
     uint64_t in_use = 0;              // device bytes held by running batches
     bool stop = false;
+    uint32_t live_slots = 0;          // slots that started and have not stopped
+    uint32_t starting = 0;            // slots still setting up
+    std::string slot_error;           // why the last slot failed to start
+    bool dead = false;                // no slot could start: every request fails at once
+    std::unordered_set<uint64_t> fault_ids;   // test hook: ids whose sort fails as a device error would
     std::vector<std::thread> workers;
     std::vector<double> latencies_ms;
-    uint64_t batches = 0, completed = 0, bytes_done = 0;
+    uint64_t batches = 0, completed = 0, failed = 0, bytes_done = 0;
     Clock::time_point t_first, t_last;
     bool any = false;
 };
@@ -118,15 +134,128 @@ struct SlotBuffers {
     }
 };
 
-// One slot: admit a batch, stage every request straight from its host arrays
-// into the slot's device buffers (no host-side packing), sort the batch, copy
-// each request's results straight back.
+// Complete a request (caller holds s->mu).
+void complete_locked(ps5_server* s, Request* r, int rc, const std::string& why, Clock::time_point t_start,
+                     Clock::time_point t_done)
+{
+    r->status = rc;
+    r->why = why;
+    r->t_start = t_start;
+    r->t_done = t_done;
+    r->done = true;
+    s->latencies_ms.push_back(std::chrono::duration<double, std::milli>(t_done - r->t_submit).count());
+    s->completed++;
+    if (rc == PS5_OK) s->bytes_done += r->arena_bytes;
+    else s->failed++;
+    s->t_last = t_done;
+}
+
+// No slot is left to run requests: every waiting request completes with an
+// error, and so will every later one (caller holds s->mu).
+void fail_all_locked(ps5_server* s)
+{
+    s->dead = true;
+    const auto now = Clock::now();
+    const std::string why = "no serving slot could start: " + s->slot_error;
+    while (!s->queue.empty()) {
+        complete_locked(s, s->queue.front(), PS5_ERR_HIP, why, now, now);
+        s->queue.pop_front();
+    }
+}
+
+// Stage the requests of `batch` straight from their host arrays into the
+// slot's device buffers (no host-side packing), sort them as one batch, copy
+// each request's results straight back.  Returns the status of the whole.
+int sort_batch(ps5_ctx* ctx, SlotBuffers& D, const std::vector<Request*>& batch, std::vector<uint64_t>& set_first,
+               std::vector<uint64_t>& set_base, std::string* why)
+{
+    hipStream_t st = (hipStream_t)ps5_ctx_stream(ctx);
+    const uint32_t nsets = (uint32_t)batch.size();
+    set_first.assign(nsets + 1, 0);
+    set_base.assign(nsets, 0);
+    uint64_t abytes = 0;
+    for (uint32_t i = 0; i < nsets; ++i) {
+        set_base[i] = abytes;
+        // every set starts 8-byte aligned and is followed by at least one
+        // zero byte, so no key of one request reaches into the next
+        abytes += (batch[i]->arena_bytes + 8) & ~7ull;
+        set_first[i + 1] = set_first[i] + batch[i]->n;
+    }
+    const uint64_t nstr = set_first[nsets];
+    for (const Request* r : batch) {
+        if (r->inject_fault) {
+            *why = "injected fault (test hook)";
+            return PS5_ERR_INTERNAL;
+        }
+    }
+    int rc = D.reserve(abytes + 16, std::max<uint64_t>(nstr, 1), nsets);
+    if (rc) {
+        *why = "device buffers for the batch could not be allocated";
+        return rc;
+    }
+    hipError_t he = hipSuccess;
+    auto ok = [&](hipError_t e) { if (e != hipSuccess && he == hipSuccess) he = e; };
+    for (uint32_t i = 0; i < nsets; ++i) {
+        const Request* r = batch[i];
+        const uint64_t pad = ((r->arena_bytes + 8) & ~7ull) - r->arena_bytes;
+        if (r->arena_bytes)
+            ok(hipMemcpyAsync(D.arena + set_base[i], r->arena, r->arena_bytes, hipMemcpyHostToDevice, st));
+        ok(hipMemsetAsync(D.arena + set_base[i] + r->arena_bytes, 0, pad, st));
+        if (r->n) ok(hipMemcpyAsync(D.offs32 + set_first[i], r->offs, r->n * 4, hipMemcpyHostToDevice, st));
+    }
+    ok(hipMemsetAsync(D.arena + abytes, 0, 16, st));
+    ok(hipMemcpyAsync(D.first, set_first.data(), (nsets + 1) * 8, hipMemcpyHostToDevice, st));
+    ok(hipMemcpyAsync(D.base, set_base.data(), nsets * 8, hipMemcpyHostToDevice, st));
+    if (he != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        *why = std::string("staging: ") + hipGetErrorString(he);
+        return PS5_ERR_HIP;
+    }
+    if (nstr > 0) {
+        if ((rc = ps5::widen_offsets(ctx, D.offs32, D.first, D.base, nsets, nstr, D.offs64)) ||
+            (rc = ps5_sort_batch_device(ctx, D.arena, abytes + 16, D.offs64, set_first.data(), nsets, D.perm,
+                                        D.lcp))) {
+            *why = ps5_last_error();
+            (void)hipStreamSynchronize(st);
+            return rc;
+        }
+        for (uint32_t i = 0; i < nsets; ++i) {
+            const Request* r = batch[i];
+            if (r->n) ok(hipMemcpyAsync(r->perm, D.perm + set_first[i], r->n * 4, hipMemcpyDeviceToHost, st));
+            if (r->lcp && r->n > 1)
+                ok(hipMemcpyAsync(r->lcp + 1, D.lcp + set_first[i] + 1, (r->n - 1) * 4, hipMemcpyDeviceToHost, st));
+        }
+    }
+    ok(hipStreamSynchronize(st));
+    if (he != hipSuccess) {
+        *why = std::string("results: ") + hipGetErrorString(he);
+        return PS5_ERR_HIP;
+    }
+    return PS5_OK;
+}
+
+// One slot: admit a batch (every waiting request that fits the budget, in
+// arrival order), sort it; when the batch fails, run its requests again one
+// by one so that only a request that fails alone reports an error.
 void run_slot(ps5_server* s)
 {
-    if (hipSetDevice(s->device) != hipSuccess) return;
     ps5_ctx* ctx = nullptr;
-    if (ps5_ctx_create(&ctx, s->device) != PS5_OK) return;
-    hipStream_t st = (hipStream_t)ps5_ctx_stream(ctx);
+    const int rc0 = ps5_ctx_create(&ctx, s->device);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->starting--;
+        if (rc0 != PS5_OK) {
+            s->slot_error = ps5_last_error();
+            if (s->live_slots == 0 && s->starting == 0) fail_all_locked(s);
+        } else {
+            s->live_slots++;
+        }
+    }
+    if (rc0 != PS5_OK) {
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+        return;
+    }
     SlotBuffers D;
     // warm start: buffers for one batch of the configured size, so that the
     // first requests do not pay for device allocations
@@ -160,58 +289,21 @@ void run_slot(ps5_server* s)
             s->in_use += need;
         }
         const auto t_start = Clock::now();
-        const uint32_t nsets = (uint32_t)batch.size();
-        set_first.assign(nsets + 1, 0);
-        set_base.assign(nsets, 0);
-        uint64_t abytes = 0;
-        for (uint32_t i = 0; i < nsets; ++i) {
-            set_base[i] = abytes;
-            abytes += (batch[i]->arena_bytes + 7) & ~7ull;     // every set starts 8-byte aligned
-            set_first[i + 1] = set_first[i] + batch[i]->n;
-        }
-        int rc = D.reserve(abytes + 16, std::max<uint64_t>(nstr, 1), nsets);
-        auto ok = [&](hipError_t e) { if (e != hipSuccess && rc == PS5_OK) rc = PS5_ERR_HIP; };
-        if (rc == PS5_OK) {
-            for (uint32_t i = 0; i < nsets; ++i) {
-                const Request* r = batch[i];
-                const uint64_t pad = ((r->arena_bytes + 7) & ~7ull) - r->arena_bytes;
-                if (r->arena_bytes)
-                    ok(hipMemcpyAsync(D.arena + set_base[i], r->arena, r->arena_bytes, hipMemcpyHostToDevice, st));
-                if (pad) ok(hipMemsetAsync(D.arena + set_base[i] + r->arena_bytes, 0, pad, st));
-                if (r->n)
-                    ok(hipMemcpyAsync(D.offs32 + set_first[i], r->offs, r->n * 4, hipMemcpyHostToDevice, st));
-            }
-            ok(hipMemsetAsync(D.arena + abytes, 0, 16, st));
-            ok(hipMemcpyAsync(D.first, set_first.data(), (nsets + 1) * 8, hipMemcpyHostToDevice, st));
-            ok(hipMemcpyAsync(D.base, set_base.data(), nsets * 8, hipMemcpyHostToDevice, st));
-        }
-        if (rc == PS5_OK && nstr > 0)
-            rc = ps5::widen_offsets(ctx, D.offs32, D.first, D.base, nsets, nstr, D.offs64);
-        if (rc == PS5_OK && nstr > 0)
-            rc = ps5_sort_batch_device(ctx, D.arena, abytes + 16, D.offs64, set_first.data(), nsets, D.perm, D.lcp);
-        if (rc == PS5_OK && nstr > 0) {
-            for (uint32_t i = 0; i < nsets; ++i) {
-                const Request* r = batch[i];
-                if (r->n) ok(hipMemcpyAsync(r->perm, D.perm + set_first[i], r->n * 4, hipMemcpyDeviceToHost, st));
-                if (r->lcp && r->n > 1)
-                    ok(hipMemcpyAsync(r->lcp + 1, D.lcp + set_first[i] + 1, (r->n - 1) * 4, hipMemcpyDeviceToHost,
-                                      st));
+        std::string why;
+        const int rc = sort_batch(ctx, D, batch, set_first, set_base, &why);
+        std::vector<int> rcs(batch.size(), rc);
+        std::vector<std::string> whys(batch.size(), why);
+        if (rc != PS5_OK && batch.size() > 1) {
+            // isolate the failure: every request of the batch on its own
+            for (size_t i = 0; i < batch.size(); ++i) {
+                whys[i].clear();
+                rcs[i] = sort_batch(ctx, D, { batch[i] }, set_first, set_base, &whys[i]);
             }
         }
-        ok(hipStreamSynchronize(st));
         const auto t_done = Clock::now();
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            for (Request* r : batch) {
-                r->status = rc;
-                r->t_start = t_start;
-                r->t_done = t_done;
-                r->done = true;
-                s->latencies_ms.push_back(std::chrono::duration<double, std::milli>(t_done - r->t_submit).count());
-                s->completed++;
-                s->bytes_done += r->arena_bytes;
-                s->t_last = t_done;
-            }
+            for (size_t i = 0; i < batch.size(); ++i) complete_locked(s, batch[i], rcs[i], whys[i], t_start, t_done);
             s->in_use -= need;
             s->batches++;
         }
@@ -220,6 +312,10 @@ void run_slot(ps5_server* s)
     }
     D.release();
     ps5_ctx_destroy(ctx);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->live_slots--;
+    }
 }
 
 }  // namespace
@@ -228,7 +324,8 @@ extern "C" {
 
 int ps5_server_create(ps5_server** out, int device, uint64_t hbm_budget, uint32_t slots, uint64_t max_batch_strings)
 {
-    if (!out || slots == 0 || slots > 16 || hbm_budget == 0) return PS5_ERR_ARG;
+    if (!out || slots == 0 || slots > 16 || hbm_budget == 0)
+        return ps5::set_error(PS5_ERR_ARG, "ps5_server_create: need out, 1..16 slots and a budget");
     ps5_server* s = new ps5_server;
     s->device = device;
     s->budget = hbm_budget;
@@ -237,6 +334,7 @@ int ps5_server_create(ps5_server** out, int device, uint64_t hbm_budget, uint32_
         s->warm_strings = max_batch_strings;
         s->warm_bytes = std::min<uint64_t>(hbm_budget / slots, max_batch_strings * 64);
     }
+    s->starting = slots;
     for (uint32_t i = 0; i < slots; ++i) s->workers.emplace_back(run_slot, s);
     *out = s;
     return PS5_OK;
@@ -258,7 +356,27 @@ void ps5_server_destroy(ps5_server* s)
 uint64_t ps5_server_submit(ps5_server* s, const uint8_t* arena, uint64_t arena_bytes, const uint32_t* offs,
                            uint64_t n, uint32_t* perm, uint32_t* lcp)
 {
-    if (!s || (n && (!arena || !offs || !perm))) return 0;
+    if (!s) { ps5::set_error(PS5_ERR_ARG, "ps5_server_submit: server is NULL"); return 0; }
+    if (n && (!arena || !offs || !perm)) {
+        ps5::set_error(PS5_ERR_ARG, "ps5_server_submit: NULL arena, offsets or result array");
+        return 0;
+    }
+    if (n >= 0xFFFFFF00ull) { ps5::set_error(PS5_ERR_ARG, "ps5_server_submit: n too large"); return 0; }
+    // a malformed request is rejected here, before it can share a batch:
+    // every string must start inside the arena and the arena must end in a
+    // NUL, so that every string ends inside it
+    if (n) {
+        if (arena_bytes == 0 || arena[arena_bytes - 1] != 0) {
+            ps5::set_error(PS5_ERR_ARG, "ps5_server_submit: the arena's last byte must be a NUL");
+            return 0;
+        }
+        uint32_t mx = 0;
+        for (uint64_t i = 0; i < n; ++i) mx = std::max(mx, offs[i]);
+        if ((uint64_t)mx >= arena_bytes) {
+            ps5::set_error(PS5_ERR_ARG, "ps5_server_submit: a string offset lies outside the arena");
+            return 0;
+        }
+    }
     Request* r = new Request;
     r->arena = arena;
     r->arena_bytes = arena_bytes;
@@ -271,33 +389,40 @@ uint64_t ps5_server_submit(ps5_server* s, const uint8_t* arena, uint64_t arena_b
     {
         std::lock_guard<std::mutex> lk(s->mu);
         r->id = s->next_id++;
+        r->inject_fault = s->fault_ids.erase(r->id) != 0;
         s->reqs[r->id] = r;
-        s->queue.push_back(r);
         if (!s->any) { s->t_first = r->t_submit; s->any = true; }
+        if (s->dead) complete_locked(s, r, PS5_ERR_HIP, "no serving slot could start: " + s->slot_error,
+                                     r->t_submit, Clock::now());
+        else s->queue.push_back(r);
     }
     s->cv_work.notify_all();
+    s->cv_done.notify_all();
     return r->id;
 }
 
 int ps5_server_wait(ps5_server* s, uint64_t id, double* latency_ms)
 {
-    if (!s) return PS5_ERR_ARG;
+    if (!s) return ps5::set_error(PS5_ERR_ARG, "ps5_server_wait: server is NULL");
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(id);
-    if (it == s->reqs.end()) return PS5_ERR_ARG;
+    if (it == s->reqs.end()) return ps5::set_error(PS5_ERR_ARG, "ps5_server_wait: unknown request id");
     Request* r = it->second;
     s->cv_done.wait(lk, [&] { return r->done; });
     const int rc = r->status;
+    const std::string why = r->why;
     if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
     s->reqs.erase(it);
     delete r;
-    return rc;
+    lk.unlock();
+    if (rc != PS5_OK) return ps5::set_error(rc, "request %llu failed: %s", (unsigned long long)id, why.c_str());
+    return PS5_OK;
 }
 
 int ps5_server_stats(ps5_server* s, double* p50_ms, double* p99_ms, uint64_t* completed, uint64_t* batches,
                      double* throughput_gbs)
 {
-    if (!s) return PS5_ERR_ARG;
+    if (!s) return ps5::set_error(PS5_ERR_ARG, "ps5_server_stats: server is NULL");
     std::lock_guard<std::mutex> lk(s->mu);
     std::vector<double> v = s->latencies_ms;
     std::sort(v.begin(), v.end());
@@ -308,6 +433,24 @@ int ps5_server_stats(ps5_server* s, double* p50_ms, double* p99_ms, uint64_t* co
     if (batches) *batches = s->batches;
     const double secs = s->any ? std::chrono::duration<double>(s->t_last - s->t_first).count() : 0.0;
     if (throughput_gbs) *throughput_gbs = secs > 0 ? (double)s->bytes_done / secs / 1e9 : 0.0;
+    return PS5_OK;
+}
+
+int ps5_server_health(ps5_server* s, uint32_t* live_slots, uint64_t* failed)
+{
+    if (!s) return ps5::set_error(PS5_ERR_ARG, "ps5_server_health: server is NULL");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (live_slots) *live_slots = s->live_slots;
+    if (failed) *failed = s->failed;
+    return PS5_OK;
+}
+
+int ps5_server_inject_fault(ps5_server* s, uint64_t id)
+{
+    if (!s || id == 0) return ps5::set_error(PS5_ERR_ARG, "ps5_server_inject_fault: bad server or id");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (id < s->next_id) return ps5::set_error(PS5_ERR_ARG, "ps5_server_inject_fault: request already submitted");
+    s->fault_ids.insert(id);
     return PS5_OK;
 }
 

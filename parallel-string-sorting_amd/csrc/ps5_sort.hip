@@ -2533,6 +2533,17 @@ int ps5_device_synchronize(void)
 
 namespace ps5 {
 
+int set_error(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
 int widen_offsets(ps5_ctx* c, const uint32_t* d_offs32, const uint64_t* d_first, const uint64_t* d_base,
                   uint32_t nsets, uint64_t n, uint64_t* d_offs64)
 {

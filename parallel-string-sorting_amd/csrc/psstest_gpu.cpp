@@ -41,12 +41,6 @@ Contest* getContestSingleton()
 
 // ---- contestants -----------------------------------------------------------
 
-static void mi355x_parallel_sample_sortBTCU2_out(unsigned char** strings, size_t n)
-{
-    std::vector<unsigned char*> out(n);
-    mi355x_ps5_sort_out(strings, out.data(), n);
-    memcpy(strings, out.data(), n * sizeof(unsigned char*));   // copy back for verification
-}
 
 // LCP merge sort in the shape of the reference's eberle/parallel_lcp_mergesort
 // contestants (src/parallel/eberle-parallel-lcp-mergesort.hpp:99-126): eight
@@ -88,7 +82,7 @@ PSS_CONTESTANT_PARALLEL(mi355x_ps5_sort_lcp, "mi355x/parallel_sample_sortBTCU2_l
                         "pS5 on MI355X with LCP array output (gfx950)")
 PSS_CONTESTANT_PARALLEL(mi355x_ps5_sort_lcp_cache, "mi355x/parallel_sample_sortBTCU2_lcp_cache",
                         "pS5 on MI355X with LCP array and distinguishing characters (gfx950)")
-PSS_CONTESTANT_PARALLEL(mi355x_parallel_sample_sortBTCU2_out, "mi355x/parallel_sample_sortBTCU2_out",
+PSS_CONTESTANT_PARALLEL(mi355x_ps5_sort_out, "mi355x/parallel_sample_sortBTCU2_out",
                         "pS5 on MI355X, separate output array (gfx950)")
 PSS_CONTESTANT_PARALLEL(mi355x_lcp_mergesort_8way, "mi355x/lcp_mergesort_8way",
                         "8 sorted index ranges merged by the device LCP merge (gfx950)")

@@ -18,7 +18,8 @@ PS5_OK = 0
 # every symbol include/ps5_mi355x.h declares
 EXPORTS = [
     "ps5_last_error",
-    "mi355x_ps5_sort", "mi355x_ps5_sort_lcp", "mi355x_ps5_sort_out", "mi355x_ps5_sort_lcp_cache",
+    "mi355x_ps5_sort", "mi355x_ps5_sort_lcp", "mi355x_ps5_sort_out", "mi355x_ps5_sort_out_base",
+    "mi355x_ps5_sort_lcp_cache",
     "ps5_sort_strings", "ps5_sort_strings_cache", "ps5_dist_chars_device",
     "ps5_ctx_create", "ps5_ctx_destroy", "ps5_ctx_reserve", "ps5_ctx_stream",
     "ps5_sort_device", "ps5_ctx_set_profile", "ps5_ctx_kernel_count", "ps5_ctx_kernel_stat",
@@ -29,6 +30,7 @@ EXPORTS = [
     "ps5_gen_uniform", "ps5_gen_windows", "ps5_gen_reference_random", "ps5_gen_prefixed",
     "ps5_sort_batch_device", "ps5_gen_batch_plan", "ps5_gen_batch_fill",
     "ps5_server_create", "ps5_server_destroy", "ps5_server_submit", "ps5_server_wait", "ps5_server_stats",
+    "ps5_server_health", "ps5_server_inject_fault",
     "ps5_ctx_set_tuning", "ps5_ctx_get_tuning", "ps5_autotune_device", "ps5_autotune_batch_device",
     "ps5_merge_device", "ps5_lcp_merge_strings",
 ]
@@ -62,7 +64,8 @@ def load():
     _sig(lib, "ps5_last_error", ctypes.c_char_p, [])
     _sig(lib, "mi355x_ps5_sort", None, [_P, ctypes.c_size_t])
     _sig(lib, "mi355x_ps5_sort_lcp", None, [_P, _P, ctypes.c_size_t])
-    _sig(lib, "mi355x_ps5_sort_out", None, [_P, _P, ctypes.c_size_t])
+    _sig(lib, "mi355x_ps5_sort_out", None, [_P, ctypes.c_size_t])
+    _sig(lib, "mi355x_ps5_sort_out_base", None, [_P, _P, ctypes.c_size_t])
     _sig(lib, "ps5_sort_strings", _I, [_P, ctypes.c_size_t, _P, _P])
     _sig(lib, "mi355x_ps5_sort_lcp_cache", None, [_P, _P, _P, ctypes.c_size_t])
     _sig(lib, "ps5_sort_strings_cache", _I, [_P, ctypes.c_size_t, _P, _P, _P])
@@ -98,6 +101,8 @@ def load():
     _sig(lib, "ps5_server_destroy", None, [_P])
     _sig(lib, "ps5_server_submit", _U64, [_P, _P, _U64, _P, _U64, _P, _P])
     _sig(lib, "ps5_server_wait", _I, [_P, _U64, ctypes.POINTER(ctypes.c_double)])
+    _sig(lib, "ps5_server_health", _I, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_U64)])
+    _sig(lib, "ps5_server_inject_fault", _I, [_P, _U64])
     _sig(lib, "ps5_server_stats", _I, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(_U64), ctypes.POINTER(_U64), ctypes.POINTER(ctypes.c_double)])
     _sig(lib, "ps5_ctx_set_tuning", _I, [_P, _U32, _U32, _U32])
@@ -428,7 +433,7 @@ class Server:
         rid = load().ps5_server_submit(self.h, _ptr(arena), int(arena.size), _ptr(offs), n, _ptr(perm),
                                        _ptr(lcp) if with_lcp else None)
         if rid == 0:
-            raise Ps5Error("ps5_server_submit failed")
+            raise Ps5Error(f"ps5_server_submit: {load().ps5_last_error().decode()}")
         self._keep[rid] = (arena, offs, perm, lcp, n)
         return rid
 
@@ -437,6 +442,14 @@ class Server:
         _check(load().ps5_server_wait(self.h, rid, ctypes.byref(ms)), "ps5_server_wait")
         arena, offs, perm, lcp, n = self._keep.pop(rid)
         return perm[:n], (lcp[:n] if lcp is not None else None), ms.value
+
+    def health(self):
+        live, failed = _U32(), _U64()
+        _check(load().ps5_server_health(self.h, ctypes.byref(live), ctypes.byref(failed)), "ps5_server_health")
+        return {"live_slots": live.value, "failed": failed.value}
+
+    def inject_fault(self, rid):
+        _check(load().ps5_server_inject_fault(self.h, rid), "ps5_server_inject_fault")
 
     def stats(self):
         p50, p99, tp = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()

@@ -186,6 +186,43 @@ def test_host_entry_contiguous_and_packed(ps5, oracle):
     assert np.array_equal(lcp[1:].astype(np.uint32), el[1:])
 
 
+def test_host_entry_out_shapes(ps5, oracle):
+    """parallel_sample_sortBTCUI_out (func_type: separate output, copied back)
+    and parallel_sample_sort_out_base (strings[] untouched, sorted pointers
+    in output[]), both against the oracle's order"""
+    import ctypes
+    strs = random_strings(40000, 0, 11, b"mno", 31)
+    blob = b"".join(x + b"\0" for x in strs) + b"\0" * 16
+    buf = ctypes.create_string_buffer(blob, len(blob))
+    base = ctypes.addressof(buf)
+    starts, pos = [], 0
+    for x in strs:
+        starts.append(base + pos)
+        pos += len(x) + 1
+    n = len(strs)
+    arena, offs = pack(strs)
+    bp, _ = oracle.brute_sort(arena, offs)
+    want = [starts[i] for i in bp]
+    lib = ps5.load()
+    arr = (ctypes.c_void_p * n)(*starts)
+    lib.mi355x_ps5_sort_out(arr, n)
+    assert [arr[i] for i in range(n)] == want
+    arr = (ctypes.c_void_p * n)(*starts)
+    out = (ctypes.c_void_p * n)()
+    lib.mi355x_ps5_sort_out_base(arr, out, n)
+    assert [arr[i] for i in range(n)] == starts, "strings[] must be left as it was"
+    assert [out[i] for i in range(n)] == want
+    # the same for an input the entry has to pack (pointers in descending
+    # order; identical strings then keep that reversed input order)
+    rev = starts[::-1]
+    rp, _ = oracle.brute_sort(*pack(strs[::-1]))
+    arr = (ctypes.c_void_p * n)(*rev)
+    out = (ctypes.c_void_p * n)()
+    lib.mi355x_ps5_sort_out_base(arr, out, n)
+    assert [arr[i] for i in range(n)] == rev
+    assert [out[i] for i in range(n)] == [rev[i] for i in rp]
+
+
 @pytest.mark.parametrize("layout", ["every_other", "page_gaps", "far_gap", "descending"])
 def test_host_entry_pointer_layouts(ps5, oracle, layout):
     """pointers into one buffer: with gaps of up to a page the span is copied
@@ -382,6 +419,76 @@ def test_large_root_duplicates(ps5, ctx, oracle):
     assert r == 0, msg
 
 
+@pytest.mark.slow
+def test_highlcp_full(ps5, ctx, oracle):
+    """the high-LCP workload at full size: 20 M windows of 100 characters of a
+    2^29-base ACGT text with 5% repeated 2 kb segments; verified exactly"""
+    say("highlcp full")
+    arena, offs = ps5.workload("highlcp")
+    assert len(offs) == 20_000_000
+    perm, lcp = ctx.sort(arena, offs)
+    r, msg = oracle.verify(arena, offs, perm, lcp)
+    assert r == 0, msg
+    assert lcp.max() == 100            # identical windows exist
+
+
+@pytest.mark.slow
+def test_batch_full(ps5, ctx, oracle):
+    """the batch workload at full size: 4096 sets, string counts log-uniform
+    in [1000, 2e6], lengths U[4,48], alphabets ACGT / digits / bytes 33..126
+    (1.1e9 strings, about 30 GB) in one batched pass; every set verified
+    exactly, and a sample of sets identical to sorting that set alone"""
+    say("batch full: generating")
+    arena, offs, sf = ps5.gen_batch(0x5EED0004, 4096, 1000, 2_000_000, 4, 48)
+    say("batch full:", int(sf[-1]), "strings,", arena.size, "bytes")
+    perm, lcp = ctx.sort_batch(arena, offs, sf)
+    say("batch full: sorted; verifying every set")
+    for s in range(4096):
+        b, e = int(sf[s]), int(sf[s + 1])
+        r, msg = oracle.verify(arena, offs[b:e], perm[b:e], lcp[b:e])
+        assert r == 0, f"set {s}: {msg}"
+    # the same sets sorted one by one through the plain device path
+    for s in range(0, 4096, 256):
+        b, e = int(sf[s]), int(sf[s + 1])
+        base = int(offs[b])
+        local = (offs[b:e] - base).astype(np.uint32)
+        end = int(offs[e - 1]) + 64
+        sub = np.concatenate([arena[base:end], np.zeros(16, np.uint8)])
+        pp, pl = ctx.sort(sub, local)
+        assert np.array_equal(pp, perm[b:e]), f"set {s}: batched and alone differ"
+        assert np.array_equal(pl[1:], lcp[b + 1:e]), f"set {s}: LCPs differ"
+
+
+@pytest.mark.slow
+def test_serve_full_sizes(ps5, ctx, oracle):
+    """serving at the workload's request sizes: log-uniform in [10 KB,
+    256 MB] (the largest forced to 256 MB) of prefix-heavy strings (1000
+    host prefixes, Zipf 1.1), Poisson arrivals; every result identical to the
+    request sorted alone and verified exactly"""
+    rng = np.random.default_rng(0x5EED0005)
+    sizes = np.exp(rng.uniform(np.log(10_000), np.log(256 << 20), 40)).astype(np.int64)
+    sizes[int(np.argmax(sizes))] = 256 << 20
+    reqs = [ps5.gen_prefixed(0x5EED0005 * 1000 + i, int(sz)) for i, sz in enumerate(sizes)]
+    say("serve full:", len(reqs), "requests,", sum(int(a.size) for a, _ in reqs), "bytes")
+    alone = [ctx.sort(a, o) for a, o in reqs]
+    srv = ps5.Server(0, hbm_budget=8 << 30, slots=4, max_batch_strings=8 << 20)
+    try:
+        import time
+        gaps = rng.exponential(0.004, len(reqs))
+        ids = []
+        for (a, o), g in zip(reqs, gaps):
+            time.sleep(float(g))
+            ids.append(srv.submit(a, o))
+        mixed = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    say("serve full stats", st)
+    for (a, o), (pa, la), (pm, lm, _) in zip(reqs, alone, mixed):
+        assert np.array_equal(pa, pm) and np.array_equal(la[1:], lm[1:])
+        assert oracle.verify(a, o, pm, lm)[0] == 0
+
+
 def _check_batch(oracle, arena, offs64, set_first, perm, lcp):
     for s in range(len(set_first) - 1):
         b, e = int(set_first[s]), int(set_first[s + 1])
@@ -445,6 +552,45 @@ def test_serve_mix_equals_alone(ps5, ctx, oracle, budget_mb, slots, max_batch):
     for (a, o), (pa, la), (pm, lm, ms) in zip(reqs, alone, mixed):
         assert np.array_equal(pa, pm)
         assert np.array_equal(la[1:], lm[1:])
+        assert oracle.verify(a, o, pm, lm)[0] == 0
+
+
+def test_serve_failure_isolation(ps5, ctx, oracle):
+    """a request whose sort fails inside a batch (a fault planted from the
+    host) fails alone: its batch is run again request by request, every other
+    request's result equals its alone-sort, and every slot keeps serving.  An
+    over-budget request runs alone and is exact too."""
+    reqs = []
+    for i in range(12):
+        a, o = ps5.gen_uniform(500 + i, 200_000 + 50_000 * i, 0, 24, b"ACGT" if i % 2 else ps5.ALPHA_ASCII94)
+        reqs.append((a, o))
+    big = ps5.gen_prefixed(77, 24 << 20)          # larger than the whole budget below
+    reqs.insert(6, big)
+    alone = [ctx.sort(a, o) for a, o in reqs]
+    bad_id = 4                                    # ids count from 1 in submission order
+    srv = ps5.Server(0, hbm_budget=16 << 20, slots=2)
+    try:
+        srv.inject_fault(bad_id)
+        ids = [srv.submit(a, o) for a, o in reqs]
+        out = []
+        for rid in ids:
+            try:
+                out.append(srv.wait(rid))
+            except ps5.Ps5Error as e:
+                say("failed", rid, e)
+                assert rid == bad_id and "injected fault" in str(e)
+                out.append(None)
+        health = srv.health()
+    finally:
+        srv.close()
+    say("health", health)
+    assert health == {"live_slots": 2, "failed": 1}
+    for k, ((a, o), (pa, la), res) in enumerate(zip(reqs, alone, out)):
+        if ids[k] == bad_id:
+            assert res is None
+            continue
+        pm, lm, _ = res
+        assert np.array_equal(pa, pm) and np.array_equal(la[1:], lm[1:])
         assert oracle.verify(a, o, pm, lm)[0] == 0
 
 

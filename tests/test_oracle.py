@@ -130,6 +130,83 @@ def test_ps5_high_lcp_windows(ps5, oracle):
     assert lcp.max() == 100
 
 
+# ---- the plain contestant's branches (no LCP array) --------------------------
+# parallel_sample_sortBTCUI runs pS5 without an LCP array: the generic
+# insertion sort (inssort.hpp:61-97) and the lcp == NULL branches.  That is the
+# sort the CPU baseline times, so it is checked on its own.
+
+@pytest.mark.parametrize("name,strs", CASES, ids=[c[0] for c in CASES])
+def test_ps5_nolcp_matches_brute(oracle, name, strs):
+    arena, offs = pack(strs)
+    perm, lcp = oracle.ps5_sort(arena, offs, threads=4, with_lcp=False)
+    assert lcp is None
+    bp, _ = oracle.brute_sort(arena, offs)
+    assert np.array_equal(perm, bp)
+    assert oracle.verify(arena, offs, perm)[0] == 0
+
+
+@pytest.mark.parametrize("threads", [1, 2, 8])
+def test_ps5_nolcp_parallel_steps(ps5, oracle, threads):
+    """n > 1 Mi: parallel steps and sequential S5 without LCPs."""
+    arena, offs = ps5.gen_uniform(77, 36 * 2**20, 0, 36, b"ACGT")
+    assert len(offs) > 2**20
+    perm, _ = oracle.ps5_sort(arena, offs, threads=threads, with_lcp=False)
+    bp, _ = oracle.brute_sort(arena, offs)
+    assert np.array_equal(perm, bp)
+
+
+@pytest.mark.parametrize("with_lcp", [False, True])
+def test_timed_cpu_sort_output(ps5, oracle, with_lcp):
+    """The timed CPU baseline hands back its own output, which is exact."""
+    arena, offs = ps5.gen_uniform(0x5EED0002, 24 * 2**20, 1, 64, bytes(range(33, 127)))
+    t, perm, lcp = oracle.time_ps5(arena, offs, threads=4, with_lcp=with_lcp, output=True)
+    assert t > 0
+    assert (lcp is not None) == with_lcp
+    assert oracle.verify(arena, offs, perm, lcp)[0] == 0
+    # a planted mistake in that output is caught
+    p = perm.copy()
+    p[10], p[11] = p[11], p[10]
+    assert oracle.verify(arena, offs, p, lcp)[0] != 0
+
+
+# ---- distinguishing prefix (checker.hpp:155-180) -------------------------------
+
+DPREFIX_KATS = [
+    # sorted strings -> (D, lcpsum), worked by hand from the definition: per
+    # neighbour pair, d = LCP + 1; D += d, plus d - d_prev when d grows
+    ([], (0, 0)),
+    ([b"abc"], (0, 0)),
+    ([b"", b"a"], (2, 0)),
+    ([b"a", b"ab", b"b"], (5, 1)),
+    ([b"x", b"x", b"x"], (6, 2)),
+    ([b"abc", b"abd", b"b"], (7, 2)),
+    ([b"ab", b"abc", b"abcd", b"b"], (12, 5)),
+]
+
+
+@pytest.mark.parametrize("strs,want", DPREFIX_KATS)
+def test_dprefix_known_answers(oracle, strs, want):
+    arena, offs = pack(strs)
+    perm = np.arange(len(strs), dtype=np.uint32)
+    assert oracle.dprefix(arena, offs, perm) == want
+
+
+def test_dprefix_lcpsum_and_ties(oracle):
+    """lcpsum is the sum of the exact LCP array; D does not depend on the
+    order of identical strings; an unsorted order changes D."""
+    strs = random_strings(3000, 0, 7, b"abc", 21)
+    arena, offs = pack(strs)
+    perm, lcp = oracle.brute_sort(arena, offs)
+    D, ls = oracle.dprefix(arena, offs, perm)
+    assert ls == int(lcp[1:].astype(np.uint64).sum())
+    j = next(i for i in range(1, len(perm)) if strs[perm[i]] == strs[perm[i - 1]])
+    p = perm.copy()
+    p[j - 1], p[j] = p[j], p[j - 1]
+    assert oracle.dprefix(arena, offs, p) == (D, ls)
+    shuffled = np.random.default_rng(1).permutation(perm).astype(np.uint32)
+    assert oracle.dprefix(arena, offs, shuffled)[1] < ls
+
+
 # ---- the verifier rejects planted mistakes ----------------------------------
 
 def test_verify_rejects_planted_mistakes(oracle):
