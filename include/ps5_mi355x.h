@@ -115,6 +115,25 @@ int ps5_sort_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
                     const uint32_t* d_offs, uint64_t n,
                     uint32_t* d_perm, uint32_t* d_lcp);
 
+/* ps5_sort_device without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Arguments
+ * and results are those of ps5_sort_device; the work buffers come from a
+ * static device array of 96 MiB that every GPU the library runs on holds
+ * (one copy per GPU and process) until the process ends, so n is limited to
+ * ps5_noalloc_max_strings() (2^20); a larger n is rejected with
+ * PS5_ERR_ARG.  The call makes no hipMalloc/hipFree or other allocating call
+ * (the context, created beforehand, holds the small pinned host buffers it
+ * uses).  All calls on one GPU share that array and its contents persist
+ * from call to call; two calls that overlapped on one GPU would race on it,
+ * so the library serialises them per device (a call returns only when its
+ * sort has finished). */
+int ps5_sort_device_noalloc(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
+                            const uint32_t* d_offs, uint64_t n,
+                            uint32_t* d_perm, uint32_t* d_lcp);
+/* Largest n ps5_sort_device_noalloc accepts. */
+uint64_t ps5_noalloc_max_strings(void);
+
 /* Distinguishing characters of a finished sort (inputs as ps5_sort_device,
  * d_perm and d_lcp its results): d_cache[i] = s_i[lcp[i]] for i >= 1, the
  * first character of the i-th smallest string that its predecessor does not

@@ -20,6 +20,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1543,13 +1545,11 @@ const char* kKernelNames[K_NUM] = {
 
 enum { S_BLOCK, S_TINY, S_RUN, S_NUM };
 
-}  // namespace
-
-struct ps5_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t side = nullptr;         // a round's leaves run here beside its last blocks
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+// The work buffers of a sort of up to `cap` strings.  A context owns one set
+// (grown with hipMalloc on demand); the allocation-free entry point swaps in
+// a set carved from the static device scratch for the duration of a call.
+struct WorkBuffers {
+    bool fixed = false;              // carved from the static scratch: never grown or freed
     uint64_t cap = 0;
     key64* key[2] = { nullptr, nullptr };
     uint32_t* idx[2] = { nullptr, nullptr };
@@ -1563,17 +1563,7 @@ struct ps5_ctx {
     Item* l_block[2] = { nullptr, nullptr };
     Item* l_run[2] = { nullptr, nullptr };
     uint32_t cap_large = 0, cap_block = 0, cap_tiny = 0, cap_run = 0;
-    uint32_t* d_cnt = nullptr;
-    uint32_t* h_cnt = nullptr;       // pinned
-    // counters + head of a list, written by k_stage straight into mapped
-    // host memory (no copy engine on the readback path)
-    unsigned char* h_stage = nullptr;    // pinned, mapped, coherent
-    unsigned char* h_stage_dev = nullptr;    // its device address
-    uint32_t* h_upcnt = nullptr;     // pinned: counters uploaded with batch roots
-    Item* h_roots = nullptr;         // pinned: root items of a batch
-    uint64_t roots_cap = 0;
     Item* h_large = nullptr;         // pinned copy of a large list
-    // multi-workgroup step of a level: descriptors and tile table
     // plan of a level: its nlarge descriptors followed by its tile table, in
     // one buffer so that one copy uploads both
     LargeDesc* d_desc = nullptr;
@@ -1586,6 +1576,46 @@ struct ps5_ctx {
     uint64_t hist_cap = 0;
     uint32_t* part = nullptr;
     uint64_t part_cap = 0;
+};
+
+// List capacities of a sort of up to cap strings (each list bounds what the
+// device can append to it; an overflow is flagged, never written).
+constexpr uint32_t cap_large_of(uint64_t cap) { return (uint32_t)(cap / kBlkT + 16); }
+constexpr uint32_t cap_block_of(uint64_t cap) { return (uint32_t)(cap / kTinyT + 16); }
+constexpr uint32_t cap_tiny_of(uint64_t cap) { return (uint32_t)(cap + 16); }
+// an equal-key run of m > kTinyT strings yields ceil(m / kRunChunk) <= m / 64 chunks
+constexpr uint32_t cap_run_of(uint64_t cap) { return (uint32_t)(cap / 64 + 16); }
+// every large segment has at most size/(kWaves * kChunkMin) + 1 tiles
+constexpr uint64_t tiles_cap_of(uint64_t cap) { return cap / ((uint64_t)kWaves * kChunkMin) + cap_large_of(cap) + 1; }
+constexpr uint64_t plan_bytes_of(uint64_t cap)
+{
+    return (uint64_t)cap_large_of(cap) * sizeof(LargeDesc) + tiles_cap_of(cap) * sizeof(TileRef);
+}
+
+}  // namespace
+
+struct ps5_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;         // a round's leaves run here beside its last blocks
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    WorkBuffers w;                   // the buffers run_sort uses
+    // the allocation-free entry point: pinned plan buffers sized for the
+    // scratch's largest sort (allocated with the context), and the scratch's
+    // device address on this context's device
+    Item* na_h_large = nullptr;
+    LargeDesc* na_h_desc = nullptr;
+    LargeDesc* na_h_desc_dev = nullptr;
+    unsigned char* na_scratch = nullptr;
+    uint32_t* d_cnt = nullptr;
+    uint32_t* h_cnt = nullptr;       // pinned
+    // counters + head of a list, written by k_stage straight into mapped
+    // host memory (no copy engine on the readback path)
+    unsigned char* h_stage = nullptr;    // pinned, mapped, coherent
+    unsigned char* h_stage_dev = nullptr;    // its device address
+    uint32_t* h_upcnt = nullptr;     // pinned: counters uploaded with batch roots
+    Item* h_roots = nullptr;         // pinned: root items of a batch
+    uint64_t roots_cap = 0;
     uint64_t* d_merge = nullptr;     // plan of a merge round
     uint64_t merge_cap = 0;
     uint32_t scatter_wgs = 256;         // k_scatter workgroups: one per CU, a multiple of kXcds
@@ -1613,24 +1643,28 @@ namespace {
 
 void free_buffers(ps5_ctx* c)
 {
+    if (c->w.fixed) {     // views into the static scratch: nothing to free
+        c->w = WorkBuffers();
+        return;
+    }
     auto f = [](void* p) { if (p) (void)hipFree(p); };
     for (int i = 0; i < 2; ++i) {
-        f(c->key[i]); f(c->idx[i]); f(c->l_round[i]); f(c->l_lvl[i]);
-        f(c->l_tiny[i]); f(c->l_block[i]); f(c->l_run[i]);
-        c->key[i] = nullptr; c->idx[i] = nullptr; c->l_round[i] = nullptr; c->l_lvl[i] = nullptr;
-        c->l_tiny[i] = nullptr; c->l_block[i] = nullptr; c->l_run[i] = nullptr;
+        f(c->w.key[i]); f(c->w.idx[i]); f(c->w.l_round[i]); f(c->w.l_lvl[i]);
+        f(c->w.l_tiny[i]); f(c->w.l_block[i]); f(c->w.l_run[i]);
+        c->w.key[i] = nullptr; c->w.idx[i] = nullptr; c->w.l_round[i] = nullptr; c->w.l_lvl[i] = nullptr;
+        c->w.l_tiny[i] = nullptr; c->w.l_block[i] = nullptr; c->w.l_run[i] = nullptr;
     }
-    f(c->bkey); f(c->bktrank); f(c->hist); f(c->part); f(c->trees); f(c->d_desc);
-    c->bkey = nullptr; c->bktrank = nullptr; c->hist = nullptr; c->part = nullptr; c->trees = nullptr;
-    c->d_desc = nullptr;
-    if (c->h_large) (void)hipHostFree(c->h_large);
-    if (c->h_desc) (void)hipHostFree(c->h_desc);
-    c->h_large = nullptr; c->h_desc = nullptr; c->h_desc_dev = nullptr;
-    c->cap = 0;
-    c->hist_cap = 0;
-    c->part_cap = 0;
-    c->tiles_cap = 0;
-    c->trees_cap = 0;
+    f(c->w.bkey); f(c->w.bktrank); f(c->w.hist); f(c->w.part); f(c->w.trees); f(c->w.d_desc);
+    c->w.bkey = nullptr; c->w.bktrank = nullptr; c->w.hist = nullptr; c->w.part = nullptr; c->w.trees = nullptr;
+    c->w.d_desc = nullptr;
+    if (c->w.h_large) (void)hipHostFree(c->w.h_large);
+    if (c->w.h_desc) (void)hipHostFree(c->w.h_desc);
+    c->w.h_large = nullptr; c->w.h_desc = nullptr; c->w.h_desc_dev = nullptr;
+    c->w.cap = 0;
+    c->w.hist_cap = 0;
+    c->w.part_cap = 0;
+    c->w.tiles_cap = 0;
+    c->w.trees_cap = 0;
 }
 
 // Splitter trees for `nlarge` segments and `hl` histogram counters of one
@@ -1638,66 +1672,193 @@ void free_buffers(ps5_ctx* c)
 // counts were just read back), so old buffers are idle.
 int grow_level(ps5_ctx* c, uint32_t nlarge, uint64_t hl)
 {
-    if (nlarge > c->trees_cap) {
-        if (c->trees) (void)hipFree(c->trees);
-        c->trees = nullptr;
-        c->trees_cap = 0;
+    if (c->w.fixed && (nlarge > c->w.trees_cap || hl > c->w.hist_cap))
+        return set_err(PS5_ERR_CAPACITY, "a level needs %u trees and %llu histogram entries; the scratch holds "
+                       "%llu and %llu", nlarge, (unsigned long long)hl, (unsigned long long)c->w.trees_cap,
+                       (unsigned long long)c->w.hist_cap);
+    if (nlarge > c->w.trees_cap) {
+        if (c->w.trees) (void)hipFree(c->w.trees);
+        c->w.trees = nullptr;
+        c->w.trees_cap = 0;
         const uint64_t want = std::max<uint64_t>(nlarge + nlarge / 2, 64);
-        HIPCHK(hipMalloc(&c->trees, want * kTreeSlots * sizeof(key64)));
-        c->trees_cap = want;
+        HIPCHK(hipMalloc(&c->w.trees, want * kTreeSlots * sizeof(key64)));
+        c->w.trees_cap = want;
     }
-    if (hl > c->hist_cap) {
-        if (c->hist) (void)hipFree(c->hist);
-        if (c->part) (void)hipFree(c->part);
-        c->hist = nullptr;
-        c->part = nullptr;
-        c->hist_cap = c->part_cap = 0;
+    if (hl > c->w.hist_cap) {
+        if (c->w.hist) (void)hipFree(c->w.hist);
+        if (c->w.part) (void)hipFree(c->w.part);
+        c->w.hist = nullptr;
+        c->w.part = nullptr;
+        c->w.hist_cap = c->w.part_cap = 0;
         const uint64_t want = std::min<uint64_t>(std::max<uint64_t>(hl + hl / 2, 1u << 20), 0xFFFFFFFFull);
-        HIPCHK(hipMalloc(&c->hist, want * sizeof(uint32_t)));
-        c->part_cap = want / kScanBlock + 2;
-        HIPCHK(hipMalloc(&c->part, c->part_cap * sizeof(uint32_t)));
-        c->hist_cap = want;
+        HIPCHK(hipMalloc(&c->w.hist, want * sizeof(uint32_t)));
+        c->w.part_cap = want / kScanBlock + 2;
+        HIPCHK(hipMalloc(&c->w.part, c->w.part_cap * sizeof(uint32_t)));
+        c->w.hist_cap = want;
     }
     return PS5_OK;
 }
 
 int ensure_capacity(ps5_ctx* c, uint64_t n)
 {
-    if (n <= c->cap) return PS5_OK;
+    if (n <= c->w.cap) return PS5_OK;
+    if (c->w.fixed)
+        return set_err(PS5_ERR_CAPACITY, "%llu strings exceed the scratch's %llu", (unsigned long long)n,
+                       (unsigned long long)c->w.cap);
     // a context that has to grow again grows by half at least
-    uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(n, 1024), c->cap + c->cap / 2);
+    uint64_t cap = std::max<uint64_t>(std::max<uint64_t>(n, 1024), c->w.cap + c->w.cap / 2);
     cap = std::min<uint64_t>(cap, std::max<uint64_t>(n, 0xFFFFFF00ull));
     free_buffers(c);
     for (int i = 0; i < 2; ++i) {
-        HIPCHK(hipMalloc(&c->key[i], cap * sizeof(key64)));
-        HIPCHK(hipMalloc(&c->idx[i], cap * sizeof(uint32_t)));
+        HIPCHK(hipMalloc(&c->w.key[i], cap * sizeof(key64)));
+        HIPCHK(hipMalloc(&c->w.idx[i], cap * sizeof(uint32_t)));
     }
-    HIPCHK(hipMalloc(&c->bkey, cap * sizeof(key64)));
-    HIPCHK(hipMalloc(&c->bktrank, cap * sizeof(uint32_t)));
-    c->cap_large = (uint32_t)(cap / kBlkT + 16);
-    c->cap_block = (uint32_t)(cap / kTinyT + 16);
-    c->cap_tiny = (uint32_t)(cap + 16);
-    // an equal-key run of m > kTinyT strings yields ceil(m / kRunChunk) <= m / 64 chunks
-    c->cap_run = (uint32_t)(cap / 64 + 16);
+    HIPCHK(hipMalloc(&c->w.bkey, cap * sizeof(key64)));
+    HIPCHK(hipMalloc(&c->w.bktrank, cap * sizeof(uint32_t)));
+    c->w.cap_large = cap_large_of(cap);
+    c->w.cap_block = cap_block_of(cap);
+    c->w.cap_tiny = cap_tiny_of(cap);
+    c->w.cap_run = cap_run_of(cap);
     for (int i = 0; i < 2; ++i) {
-        HIPCHK(hipMalloc(&c->l_round[i], (uint64_t)c->cap_large * sizeof(Item)));
-        HIPCHK(hipMalloc(&c->l_lvl[i], (uint64_t)c->cap_large * sizeof(Item)));
-        HIPCHK(hipMalloc(&c->l_tiny[i], (uint64_t)c->cap_tiny * sizeof(Item)));
-        HIPCHK(hipMalloc(&c->l_block[i], (uint64_t)kBlkClasses * c->cap_block * sizeof(Item)));
-        HIPCHK(hipMalloc(&c->l_run[i], (uint64_t)c->cap_run * sizeof(Item)));
+        HIPCHK(hipMalloc(&c->w.l_round[i], (uint64_t)c->w.cap_large * sizeof(Item)));
+        HIPCHK(hipMalloc(&c->w.l_lvl[i], (uint64_t)c->w.cap_large * sizeof(Item)));
+        HIPCHK(hipMalloc(&c->w.l_tiny[i], (uint64_t)c->w.cap_tiny * sizeof(Item)));
+        HIPCHK(hipMalloc(&c->w.l_block[i], (uint64_t)kBlkClasses * c->w.cap_block * sizeof(Item)));
+        HIPCHK(hipMalloc(&c->w.l_run[i], (uint64_t)c->w.cap_run * sizeof(Item)));
     }
-    // every large segment has at most size/(kWaves * kChunkMin) + 1 tiles;
     // the splitter trees and histograms of a level grow on demand (grow_level)
-    c->tiles_cap = cap / ((uint64_t)kWaves * kChunkMin) + c->cap_large + 1;
-    const uint64_t plan_bytes = (uint64_t)c->cap_large * sizeof(LargeDesc) + c->tiles_cap * sizeof(TileRef);
+    c->w.tiles_cap = tiles_cap_of(cap);
+    const uint64_t plan_bytes = plan_bytes_of(cap);
     static_assert(sizeof(LargeDesc) % alignof(TileRef) == 0, "tile table must stay aligned after the descriptors");
     static_assert(sizeof(LargeDesc) % 8 == 0 && sizeof(TileRef) % 8 == 0, "the plan is fetched in 8-byte words");
-    HIPCHK(hipMalloc(&c->d_desc, plan_bytes));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_large), (uint64_t)c->cap_large * sizeof(Item)));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->h_desc), plan_bytes, hipHostMallocMapped | hipHostMallocCoherent));
-    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_desc_dev), c->h_desc, 0));
-    c->cap = cap;
+    HIPCHK(hipMalloc(&c->w.d_desc, plan_bytes));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->w.h_large), (uint64_t)c->w.cap_large * sizeof(Item)));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->w.h_desc), plan_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->w.h_desc_dev), c->w.h_desc, 0));
+    c->w.cap = cap;
     return PS5_OK;
+}
+
+// ---- the allocation-free entry point's device scratch -----------------------
+//
+// ps5_sort_device_noalloc takes every work buffer of a sort of up to
+// kNoAllocMaxStrings strings from `scratch`, a static device array: each GPU
+// the library runs on holds its own copy from the time the code object is
+// loaded until the process ends.  Host code never uses the array's name as a
+// device address; the address is resolved once per device with
+// hipGetSymbolAddress (scratch_address) and kept in the context.
+constexpr uint64_t kScratchBytes = 96ull << 20;
+constexpr uint64_t kNoAllocMaxStrings = 1ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Byte offsets of the work buffers of a sort of up to `cap` strings inside
+// the scratch.  Histogram bound per large segment of m strings: at most
+// min(2047, m/16) buckets (2^treebits < 2m/leaf_target, leaf_target >= 64)
+// times at most m/8192 + 1 tiles of kWaves chunks, so cap + cap/4 entries
+// cover a level; a level needs at most cap_large trees.
+struct ScratchLayout {
+    uint64_t key[2], idx[2], bkey, bktrank, l_round[2], l_lvl[2], l_tiny[2], l_block[2], l_run[2];
+    uint64_t desc, trees, hist, part, total;
+    uint64_t trees_cap, hist_cap, part_cap;
+};
+constexpr uint64_t align256(uint64_t x) { return (x + 255) / 256 * 256; }
+constexpr ScratchLayout scratch_layout(uint64_t cap)
+{
+    ScratchLayout L{};
+    uint64_t o = 0;
+    auto take = [&o](uint64_t bytes) { const uint64_t at = o; o = align256(o + bytes); return at; };
+    const uint64_t cl = cap_large_of(cap), cb = cap_block_of(cap), ct = cap_tiny_of(cap), cr = cap_run_of(cap);
+    for (int i = 0; i < 2; ++i) {
+        L.key[i] = take(cap * sizeof(key64));
+        L.idx[i] = take(cap * sizeof(uint32_t));
+    }
+    L.bkey = take(cap * sizeof(key64));
+    L.bktrank = take(cap * sizeof(uint32_t));
+    for (int i = 0; i < 2; ++i) {
+        L.l_round[i] = take(cl * sizeof(Item));
+        L.l_lvl[i] = take(cl * sizeof(Item));
+        L.l_tiny[i] = take(ct * sizeof(Item));
+        L.l_block[i] = take(kBlkClasses * cb * sizeof(Item));
+        L.l_run[i] = take(cr * sizeof(Item));
+    }
+    L.desc = take(plan_bytes_of(cap));
+    L.trees_cap = cl;
+    L.trees = take(L.trees_cap * kTreeSlots * sizeof(key64));
+    L.hist_cap = cap + cap / 4 + 64;
+    L.hist = take(L.hist_cap * sizeof(uint32_t));
+    L.part_cap = L.hist_cap / kScanBlock + 2;
+    L.part = take(L.part_cap * sizeof(uint32_t));
+    L.total = o;
+    return L;
+}
+static_assert(scratch_layout(kNoAllocMaxStrings).total <= kScratchBytes,
+              "the scratch holds the work buffers of its largest sort");
+
+// Device address of `scratch` on `device` (the caller has made it current):
+// resolved on the first call per device and remembered.
+int scratch_address(int device, unsigned char** out)
+{
+    static std::mutex mu;
+    static std::map<int, unsigned char*> addr;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = addr.find(device);
+    if (it == addr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(PS5_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                           hipGetErrorString(e));
+        it = addr.emplace(device, static_cast<unsigned char*>(p)).first;
+    }
+    *out = it->second;
+    return PS5_OK;
+}
+
+// Calls of the allocation-free entry point on one device share its scratch:
+// they are serialised per device inside this process.
+std::mutex& scratch_mutex(int device)
+{
+    static std::mutex mus[64];
+    return mus[(unsigned)device % 64u];
+}
+
+// Work buffers of a sort of up to kNoAllocMaxStrings strings carved from the
+// scratch at `base` (its device address), with the context's pinned plan
+// buffers.  Nothing is allocated.
+WorkBuffers scratch_buffers(unsigned char* base, Item* h_large, LargeDesc* h_desc, LargeDesc* h_desc_dev)
+{
+    constexpr ScratchLayout L = scratch_layout(kNoAllocMaxStrings);
+    const uint64_t cap = kNoAllocMaxStrings;
+    WorkBuffers w;
+    w.fixed = true;
+    w.cap = cap;
+    for (int i = 0; i < 2; ++i) {
+        w.key[i] = reinterpret_cast<key64*>(base + L.key[i]);
+        w.idx[i] = reinterpret_cast<uint32_t*>(base + L.idx[i]);
+        w.l_round[i] = reinterpret_cast<Item*>(base + L.l_round[i]);
+        w.l_lvl[i] = reinterpret_cast<Item*>(base + L.l_lvl[i]);
+        w.l_tiny[i] = reinterpret_cast<Item*>(base + L.l_tiny[i]);
+        w.l_block[i] = reinterpret_cast<Item*>(base + L.l_block[i]);
+        w.l_run[i] = reinterpret_cast<Item*>(base + L.l_run[i]);
+    }
+    w.bkey = reinterpret_cast<key64*>(base + L.bkey);
+    w.bktrank = reinterpret_cast<uint32_t*>(base + L.bktrank);
+    w.cap_large = cap_large_of(cap);
+    w.cap_block = cap_block_of(cap);
+    w.cap_tiny = cap_tiny_of(cap);
+    w.cap_run = cap_run_of(cap);
+    w.h_large = h_large;
+    w.d_desc = reinterpret_cast<LargeDesc*>(base + L.desc);
+    w.h_desc = h_desc;
+    w.h_desc_dev = h_desc_dev;
+    w.tiles_cap = tiles_cap_of(cap);
+    w.trees = reinterpret_cast<key64*>(base + L.trees);
+    w.trees_cap = L.trees_cap;
+    w.hist = reinterpret_cast<uint32_t*>(base + L.hist);
+    w.hist_cap = L.hist_cap;
+    w.part = reinterpret_cast<uint32_t*>(base + L.part);
+    w.part_cap = L.part_cap;
+    return w;
 }
 
 struct Launch {   // counts a launch; brackets it with events when profiling
@@ -1748,18 +1909,18 @@ Sink make_sink(ps5_ctx* c, Item* large, int c_large, int rp)
 {
     Sink s;
     s.large = large;
-    s.block = c->l_block[rp];
-    s.tiny = c->l_tiny[rp];
-    s.run = c->l_run[rp];
+    s.block = c->w.l_block[rp];
+    s.tiny = c->w.l_tiny[rp];
+    s.run = c->w.l_run[rp];
     s.cnt = c->d_cnt;
     s.c_large = c_large;
     s.c_block = C_BLOCK + rp;
     s.c_tiny = C_TINY + rp;
     s.c_run = C_RUN + rp;
-    s.cap_large = c->cap_large;
-    s.cap_block = c->cap_block;
-    s.cap_tiny = c->cap_tiny;
-    s.cap_run = c->cap_run;
+    s.cap_large = c->w.cap_large;
+    s.cap_block = c->w.cap_block;
+    s.cap_tiny = c->w.cap_tiny;
+    s.cap_run = c->w.cap_run;
     return s;
 }
 
@@ -1777,10 +1938,10 @@ int readback(ps5_ctx* c, const Item* list, int ci)
     if (c->h_cnt[C_ERR] & ERR_BOUNDS) return set_err(PS5_ERR_INTERNAL, "device bounds check failed");
     if (list) {
         const uint32_t nl = c->h_cnt[ci];
-        if (nl > c->cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
-        memcpy(c->h_large, c->h_stage + kStageHead, std::min(nl, kPeek) * sizeof(Item));
+        if (nl > c->w.cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
+        memcpy(c->w.h_large, c->h_stage + kStageHead, std::min(nl, kPeek) * sizeof(Item));
         if (nl > kPeek) {
-            HIPCHK(hipMemcpyAsync(c->h_large + kPeek, list + kPeek, (nl - kPeek) * sizeof(Item),
+            HIPCHK(hipMemcpyAsync(c->w.h_large + kPeek, list + kPeek, (nl - kPeek) * sizeof(Item),
                                   hipMemcpyDeviceToHost, c->stream));
             HIPCHK(hipStreamSynchronize(c->stream));
             c->st_syncs++;
@@ -1815,21 +1976,21 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     hipStream_t st = c->stream;
     uint64_t ntiles = 0, hl = 0, elems = 0, elems_ident = 0;
     int kmax = 1;
-    if (nlarge > c->cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
-    TileRef* h_tiles = reinterpret_cast<TileRef*>(c->h_desc + nlarge);
-    const TileRef* d_tiles = reinterpret_cast<const TileRef*>(c->d_desc + nlarge);
+    if (nlarge > c->w.cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
+    TileRef* h_tiles = reinterpret_cast<TileRef*>(c->w.h_desc + nlarge);
+    const TileRef* d_tiles = reinterpret_cast<const TileRef*>(c->w.d_desc + nlarge);
     uint64_t total = 0;
     bool all_ident = true;     // a level of roots: the scatter stages positions, chunks may be longer
     for (uint32_t i = 0; i < nlarge; ++i) {
-        total += c->h_large[i].size;
-        all_ident = all_ident && (c->h_large[i].flags & F_IDENT);
+        total += c->w.h_large[i].size;
+        all_ident = all_ident && (c->w.h_large[i].flags & F_IDENT);
     }
     const uint64_t cmax = all_ident ? kChunkMaxIdent : kChunkMax;
     const uint64_t climit = std::min<uint64_t>(cmax, std::max<uint64_t>(kChunkMin, div_up(total, kWaves * kMinTiles)));
     for (uint32_t i = 0; i < nlarge; ++i) {
-        const Item seg = c->h_large[i];
+        const Item seg = c->w.h_large[i];
         const uint32_t T = div_up(seg.size, climit * kWaves);
-        LargeDesc& D = c->h_desc[i];
+        LargeDesc& D = c->w.h_desc[i];
         D.seg = seg;
         D.T = T;
         D.chunk = div_up(seg.size, (uint64_t)T * kWaves);   // <= climit: equal chunks, the last one short
@@ -1839,7 +2000,7 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         const uint32_t nspl = (1u << D.k) - 1;
         kmax = std::max(kmax, (int)D.k);
         D.oversample = std::max<uint32_t>(2, std::min<uint32_t>(c->max_oversample, (kMaxSamples / nspl) & ~1u));
-        if (ntiles + T > c->tiles_cap) return set_err(PS5_ERR_CAPACITY, "tile table capacity");
+        if (ntiles + T > c->w.tiles_cap) return set_err(PS5_ERR_CAPACITY, "tile table capacity");
         for (uint32_t t = 0; t < T; ++t) h_tiles[ntiles + t] = TileRef{ i, t };
         ntiles += T;
         hl += (uint64_t)((2u << D.k) - 1) * T * kWaves;
@@ -1851,13 +2012,13 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     {   // the plan: pulled from mapped host memory by a kernel (no copy engine)
         const uint64_t words = (nlarge * sizeof(LargeDesc) + ntiles * sizeof(TileRef)) / 8;
         hipLaunchKernelGGL(k_fetch, dim3((uint32_t)std::min<uint64_t>(div_up(words, kWG), 1024)), dim3(kWG), 0, st,
-                           reinterpret_cast<const uint64_t*>(c->h_desc_dev), reinterpret_cast<uint64_t*>(c->d_desc),
+                           reinterpret_cast<const uint64_t*>(c->w.h_desc_dev), reinterpret_cast<uint64_t*>(c->w.d_desc),
                            words);
     }
     const uint32_t hl32 = (uint32_t)hl;
     {
         Launch L(c, K_TREE, (uint64_t)nlarge * kMaxSamples * 8);
-        hipLaunchKernelGGL(k_tree, dim3(nlarge), dim3(kBigWG), 0, st, c->d_desc, B, seed, c->trees, (int)from_arena);
+        hipLaunchKernelGGL(k_tree, dim3(nlarge), dim3(kBigWG), 0, st, c->w.d_desc, B, seed, c->w.trees, (int)from_arena);
     }
     {
         // key 8 R + bktrank 4 W; extracting at the root: offs 4 R + 8 characters R + key 8 W + bktrank 4 W
@@ -1865,20 +2026,20 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         const dim3 g((uint32_t)ntiles), b(kWG);
         const uint32_t lds = count_lds_bytes(kmax);
         if (kmax <= 7)
-            hipLaunchKernelGGL(k_count<8>, g, b, lds, st, c->d_desc, d_tiles, B, c->trees, c->hist, (int)from_arena, kmax);
+            hipLaunchKernelGGL(k_count<8>, g, b, lds, st, c->w.d_desc, d_tiles, B, c->w.trees, c->w.hist, (int)from_arena, kmax);
         else if (kmax == 8)
-            hipLaunchKernelGGL(k_count<9>, g, b, lds, st, c->d_desc, d_tiles, B, c->trees, c->hist, (int)from_arena, kmax);
+            hipLaunchKernelGGL(k_count<9>, g, b, lds, st, c->w.d_desc, d_tiles, B, c->w.trees, c->w.hist, (int)from_arena, kmax);
         else if (kmax == 9)
-            hipLaunchKernelGGL(k_count<10>, g, b, lds, st, c->d_desc, d_tiles, B, c->trees, c->hist, (int)from_arena, kmax);
+            hipLaunchKernelGGL(k_count<10>, g, b, lds, st, c->w.d_desc, d_tiles, B, c->w.trees, c->w.hist, (int)from_arena, kmax);
         else
-            hipLaunchKernelGGL(k_count<11>, g, b, lds, st, c->d_desc, d_tiles, B, c->trees, c->hist, (int)from_arena, kmax);
+            hipLaunchKernelGGL(k_count<11>, g, b, lds, st, c->w.d_desc, d_tiles, B, c->w.trees, c->w.hist, (int)from_arena, kmax);
     }
     {
         const uint32_t nb = div_up(hl32, kScanBlock);
         Launch L(c, K_SCAN, hl * 12);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kWG), 0, st, c->hist, hl32, c->part);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kWG), 0, st, c->part, nb);
-        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, c->hist, hl32, c->part);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kWG), 0, st, c->w.hist, hl32, c->w.part);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kWG), 0, st, c->w.part, nb);
+        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, c->w.hist, hl32, c->w.part);
     }
     {
         // bktrank 4 + key 8 + idx 4 R (no idx at a root), key 8 + idx 4 W
@@ -1886,15 +2047,15 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         const uint32_t nchunks = (uint32_t)ntiles * kWaves;
         const uint32_t wgs = std::min(c->scatter_wgs, div_up(nchunks, kXcds) * kXcds);
         if (all_ident)
-            hipLaunchKernelGGL(k_scatter<true>, dim3(wgs), dim3(kBigWG), ScatLds<true>::kBytes, st, c->d_desc, d_tiles,
-                               B, c->hist, nchunks);
+            hipLaunchKernelGGL(k_scatter<true>, dim3(wgs), dim3(kBigWG), ScatLds<true>::kBytes, st, c->w.d_desc, d_tiles,
+                               B, c->w.hist, nchunks);
         else
-            hipLaunchKernelGGL(k_scatter<false>, dim3(wgs), dim3(kBigWG), ScatLds<false>::kBytes, st, c->d_desc,
-                               d_tiles, B, c->hist, nchunks);
+            hipLaunchKernelGGL(k_scatter<false>, dim3(wgs), dim3(kBigWG), ScatLds<false>::kBytes, st, c->w.d_desc,
+                               d_tiles, B, c->w.hist, nchunks);
     }
     {
         Launch L(c, K_CHILDREN, hl * 4 / kWaves);
-        hipLaunchKernelGGL(k_children, dim3(nlarge), dim3(kWG), 0, st, c->d_desc, c->hist, sink);
+        hipLaunchKernelGGL(k_children, dim3(nlarge), dim3(kWG), 0, st, c->w.d_desc, c->w.hist, sink);
     }
     c->st_segments += nlarge;
     return PS5_OK;
@@ -1907,13 +2068,13 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
     hipStream_t st = c->stream;
     if (!set_first) {
         Item* list; int ci;
-        if (n <= kTinyT) { list = c->l_tiny[0]; ci = C_TINY; }
+        if (n <= kTinyT) { list = c->w.l_tiny[0]; ci = C_TINY; }
         else if (n <= kBlkT) {
             const int cls = block_class(n);
-            list = c->l_block[0] + (uint64_t)cls * c->cap_block;
+            list = c->w.l_block[0] + (uint64_t)cls * c->w.cap_block;
             ci = C_BLOCK + 2 * cls;
         }
-        else { list = c->l_round[0]; ci = C_LROUND; }
+        else { list = c->w.l_round[0]; ci = C_LROUND; }
         Launch L(c, K_MISC, 0);
         hipLaunchKernelGGL(k_setup, dim3(1), dim3(64), 0, st, c->d_cnt, list, ci, n);
         return PS5_OK;
@@ -1945,9 +2106,9 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
             cnt[kind]++;
         }
     }
-    if (cnt[NK - 1] > c->cap_large || cnt[0] > c->cap_tiny) return set_err(PS5_ERR_CAPACITY, "too many sets");
+    if (cnt[NK - 1] > c->w.cap_large || cnt[0] > c->w.cap_tiny) return set_err(PS5_ERR_CAPACITY, "too many sets");
     for (int k = 1; k <= kBlkClasses; ++k)
-        if (cnt[k] > c->cap_block) return set_err(PS5_ERR_CAPACITY, "too many sets");
+        if (cnt[k] > c->w.cap_block) return set_err(PS5_ERR_CAPACITY, "too many sets");
     HIPCHK(hipStreamSynchronize(st));     // h_upcnt may still feed an earlier copy
     for (int i = 0; i < C_NCNT; ++i) c->h_upcnt[i] = 0;
     c->h_upcnt[C_TINY] = cnt[0];
@@ -1955,9 +2116,9 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
     c->h_upcnt[C_LROUND] = cnt[NK - 1];
     const Item* h = c->h_roots;
     for (int kind = 0; kind < NK; ++kind) {
-        Item* dst = kind == 0 ? c->l_tiny[0]
-                  : kind == NK - 1 ? c->l_round[0]
-                  : c->l_block[0] + (uint64_t)(kind - 1) * c->cap_block;
+        Item* dst = kind == 0 ? c->w.l_tiny[0]
+                  : kind == NK - 1 ? c->w.l_round[0]
+                  : c->w.l_block[0] + (uint64_t)(kind - 1) * c->w.cap_block;
         if (cnt[kind]) HIPCHK(hipMemcpyAsync(dst, h, cnt[kind] * sizeof(Item), hipMemcpyHostToDevice, st));
         h += cnt[kind];
     }
@@ -1984,12 +2145,12 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     B.alen = alen;
     B.offs = offs;
     B.offs64 = offs64;
-    B.key0 = c->key[0]; B.key1 = c->key[1];
-    B.idx0 = c->idx[0]; B.idx1 = c->idx[1];
+    B.key0 = c->w.key[0]; B.key1 = c->w.key[1];
+    B.idx0 = c->w.idx[0]; B.idx1 = c->w.idx[1];
     B.perm = perm;
     B.lcp = lcp;
-    B.bkey = c->bkey;
-    B.bktrank = c->bktrank;
+    B.bkey = c->w.bkey;
+    B.bktrank = c->w.bktrank;
 
     // One root larger than a block: its keys are extracted by the first
     // classification pass itself, and the host knows the root without asking.
@@ -2003,8 +2164,8 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     if (fused_root) {
         for (int i = 0; i < C_NCNT; ++i) c->h_cnt[i] = 0;
         c->h_cnt[C_LROUND] = 1;
-        c->h_large[0] = Item{ 0u, n, 0u, F_IDENT };
-    } else if ((rc = readback(c, c->l_round[0], C_LROUND))) {
+        c->w.h_large[0] = Item{ 0u, n, 0u, F_IDENT };
+    } else if ((rc = readback(c, c->w.l_round[0], C_LROUND))) {
         return rc;
     }
 
@@ -2018,25 +2179,25 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
             for (int cls = 0; cls < kBlkClasses; ++cls) mask |= 1u << (C_BLOCK + 2 * cls + np);
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, mask);
         }
-        const Sink nsink = make_sink(c, c->l_round[np], C_LROUND + np, np);
+        const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
         // tiny leaves and equal-key chunks of this round, on stream s
         auto launch_leaves = [&](hipStream_t s) {
             const uint32_t nt = c->h_cnt[C_TINY + rp], nr = c->h_cnt[C_RUN + rp];
             if (nt) {
                 {
                     Launch L(c, K_LEAF_TINY, 0);
-                    hipLaunchKernelGGL(k_leaf_tiny, dim3(div_up(nt, kWaves)), dim3(kWG), 0, s, c->l_tiny[rp],
+                    hipLaunchKernelGGL(k_leaf_tiny, dim3(div_up(nt, kWaves)), dim3(kWG), 0, s, c->w.l_tiny[rp],
                                        c->d_cnt + C_TINY + rp, B, depth);
                 }
-                sum_sizes(c, c->l_tiny[rp], 0, c->d_cnt + C_TINY + rp, nt, S_TINY);
+                sum_sizes(c, c->w.l_tiny[rp], 0, c->d_cnt + C_TINY + rp, nt, S_TINY);
             }
             if (nr) {
                 {
                     Launch L(c, K_LEAF_RUN, 0);
-                    hipLaunchKernelGGL(k_leaf_run, dim3(nr), dim3(kWG), 0, s, c->l_run[rp], c->d_cnt + C_RUN + rp,
+                    hipLaunchKernelGGL(k_leaf_run, dim3(nr), dim3(kWG), 0, s, c->w.l_run[rp], c->d_cnt + C_RUN + rp,
                                        B, nsink, depth);
                 }
-                sum_sizes(c, c->l_run[rp], 0, c->d_cnt + C_RUN + rp, nr, S_RUN);
+                sum_sizes(c, c->w.l_run[rp], 0, c->d_cnt + C_RUN + rp, nr, S_RUN);
             }
         };
         bool leaves_forked = false;
@@ -2064,7 +2225,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                 HIPCHK(hipEventRecord(c->ev_join, c->side));
                 leaves_forked = true;
             }
-            Item* nx_large = c->l_lvl[level & 1];
+            Item* nx_large = c->w.l_lvl[level & 1];
             const int nx_ci = C_LLVL + (level & 1);
             if (round > 0 || level > 0) {     // (every counter starts at zero)
                 Launch L(c, K_MISC, 0);
@@ -2076,7 +2237,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                 return rc;
             for (int cls = 0; cls < kBlkClasses; ++cls) {
                 if (!nblk[cls]) continue;
-                const Item* list = c->l_block[rp] + (uint64_t)cls * c->cap_block;
+                const Item* list = c->w.l_block[rp] + (uint64_t)cls * c->w.cap_block;
                 const uint32_t* cnt = c->d_cnt + C_BLOCK + 2 * cls + rp;
                 {
                     Launch L(c, K_BLOCK, 0);
@@ -2119,18 +2280,18 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
             Launch L(c, K_LEAFBOUND, (uint64_t)(nt + nb + nr) * 36);
             LeafLists LL;
             uint32_t end = 0;
-            LL.list[0] = c->l_tiny[rp];
+            LL.list[0] = c->w.l_tiny[rp];
             LL.end[0] = end += nt;
             for (int cls = 0; cls < kBlkClasses; ++cls) {
-                LL.list[1 + cls] = c->l_block[rp] + (uint64_t)cls * c->cap_block;
+                LL.list[1 + cls] = c->w.l_block[rp] + (uint64_t)cls * c->w.cap_block;
                 LL.end[1 + cls] = end += c->h_cnt[C_BLOCK + 2 * cls + rp];
             }
-            LL.list[kBlkClasses + 1] = c->l_run[rp];
+            LL.list[kBlkClasses + 1] = c->w.l_run[rp];
             LL.end[kBlkClasses + 1] = end += nr;
             hipLaunchKernelGGL(k_leafbound, dim3(div_up((uint64_t)end, kWG)), dim3(kWG), 0, st, LL, B, depth);
         }
         HIPCHK(hipGetLastError());
-        if ((rc = readback(c, c->l_round[np], C_LROUND + np))) return rc;
+        if ((rc = readback(c, c->w.l_round[np], C_LROUND + np))) return rc;
         uint64_t pending = (uint64_t)c->h_cnt[C_LROUND + np] + c->h_cnt[C_TINY + np] + c->h_cnt[C_RUN + np];
         for (int cls = 0; cls < kBlkClasses; ++cls) pending += c->h_cnt[C_BLOCK + 2 * cls + np];
         if (pending == 0) break;
@@ -2189,9 +2350,21 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&c->h_stage_dev), c->h_stage, 0);
     if (e == hipSuccess) e = hipMalloc(&c->d_sums, S_NUM * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_sums), S_NUM * sizeof(unsigned long long));
+    // the allocation-free entry point's pinned plan buffers, sized for the
+    // scratch's largest sort
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&c->na_h_large), cap_large_of(kNoAllocMaxStrings) * sizeof(Item));
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&c->na_h_desc), plan_bytes_of(kNoAllocMaxStrings),
+                          hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&c->na_h_desc_dev), c->na_h_desc, 0);
     if (e != hipSuccess) {
         ps5_ctx_destroy(c);
         return set_err(PS5_ERR_HIP, "context setup failed: %s", hipGetErrorString(e));
+    }
+    if (int rc = scratch_address(device, &c->na_scratch)) {
+        ps5_ctx_destroy(c);
+        return rc;
     }
     *out = c;
     return PS5_OK;
@@ -2212,6 +2385,8 @@ void ps5_ctx_destroy(ps5_ctx* c)
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_roots) (void)hipHostFree(c->h_roots);
     if (c->h_sums) (void)hipHostFree(c->h_sums);
+    if (c->na_h_large) (void)hipHostFree(c->na_h_large);
+    if (c->na_h_desc) (void)hipHostFree(c->na_h_desc);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto& u : c->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     if (c->side) (void)hipStreamSynchronize(c->side);
@@ -2252,6 +2427,39 @@ int ps5_sort_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, co
     return rc;
 }
 
+int ps5_sort_device_noalloc(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, const uint32_t* d_offs,
+                            uint64_t n, uint32_t* d_perm, uint32_t* d_lcp)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
+    if (n == 0) return PS5_OK;
+    if (!d_arena || !d_offs || !d_perm) return set_err(PS5_ERR_ARG, "NULL device pointer");
+    if (n > kNoAllocMaxStrings)
+        return set_err(PS5_ERR_ARG, "n = %llu exceeds the allocation-free limit of %llu strings",
+                       (unsigned long long)n, (unsigned long long)kNoAllocMaxStrings);
+    if (arena_bytes == 0 || arena_bytes > 0xFFFFFFFFull)
+        return set_err(PS5_ERR_ARG, "arena_bytes must be in [1, 2^32)");
+    if (((uintptr_t)d_arena & 7u) != 0) return set_err(PS5_ERR_ARG, "arena must be 8-byte aligned");
+    if (!c->na_scratch) return set_err(PS5_ERR_HIP, "the device scratch of this context is not resolved");
+    HIPCHK(hipSetDevice(c->device));
+    std::lock_guard<std::mutex> g(scratch_mutex(c->device));
+    // the context's own buffers step aside for views into the scratch;
+    // profiling (which creates events) is off for the call
+    WorkBuffers own = scratch_buffers(c->na_scratch, c->na_h_large, c->na_h_desc, c->na_h_desc_dev);
+    std::swap(c->w, own);
+    const bool prof = c->profile;
+    c->profile = false;
+    int rc = run_sort(c, d_arena, arena_bytes, d_offs, nullptr, (uint32_t)n, d_perm, d_lcp, nullptr, 0);
+    if (rc) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->side);
+    }
+    c->profile = prof;
+    std::swap(c->w, own);
+    return rc;
+}
+
+uint64_t ps5_noalloc_max_strings(void) { return kNoAllocMaxStrings; }
+
 int ps5_merge_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, const uint32_t* d_offs,
                      const uint64_t* run_first, uint32_t nruns, const uint32_t* d_idx, const uint32_t* d_lcp,
                      uint32_t* d_idx_out, uint32_t* d_lcp_out)
@@ -2287,16 +2495,16 @@ int ps5_merge_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_bytes, c
     B.offs = d_offs;
     // ping-pong buffers: indices in idx[0]/idx[1], first 8 characters in
     // key[0]/key[1], LCPs in the two halves of bkey
-    uint32_t* ti[2] = { c->idx[0], c->idx[1] };
-    key64* tk[2] = { c->key[0], c->key[1] };
-    uint32_t* tl[2] = { reinterpret_cast<uint32_t*>(c->bkey), reinterpret_cast<uint32_t*>(c->bkey) + c->cap };
+    uint32_t* ti[2] = { c->w.idx[0], c->w.idx[1] };
+    key64* tk[2] = { c->w.key[0], c->w.key[1] };
+    uint32_t* tl[2] = { reinterpret_cast<uint32_t*>(c->w.bkey), reinterpret_cast<uint32_t*>(c->w.bkey) + c->w.cap };
     std::vector<uint64_t> bounds(run_first, run_first + nruns + 1);
     const uint32_t* si = d_idx;
     const uint32_t* sl = d_lcp;
     // the first round reads every string's first 8 characters from ik, keys
     // extracted in input order by one streaming pass (the previous round's
     // output carries them afterwards); key[1] is free until the second merge round writes it
-    key64* ik = c->key[1];
+    key64* ik = c->w.key[1];
     {
         Bufs BI = B;
         BI.key0 = ik;

@@ -22,7 +22,7 @@ EXPORTS = [
     "mi355x_ps5_sort_lcp_cache",
     "ps5_sort_strings", "ps5_sort_strings_cache", "ps5_dist_chars_device",
     "ps5_ctx_create", "ps5_ctx_destroy", "ps5_ctx_reserve", "ps5_ctx_stream",
-    "ps5_sort_device", "ps5_ctx_set_profile", "ps5_ctx_kernel_count", "ps5_ctx_kernel_stat",
+    "ps5_sort_device", "ps5_sort_device_noalloc", "ps5_noalloc_max_strings", "ps5_ctx_set_profile", "ps5_ctx_kernel_count", "ps5_ctx_kernel_stat",
     "ps5_ctx_work_stats",
     "ps5_dev_malloc", "ps5_dev_free", "ps5_memcpy_h2d", "ps5_memcpy_d2h", "ps5_device_synchronize",
     "ps5_dev_copy",
@@ -75,6 +75,8 @@ def load():
     _sig(lib, "ps5_ctx_reserve", _I, [_P, _U64])
     _sig(lib, "ps5_ctx_stream", _P, [_P])
     _sig(lib, "ps5_sort_device", _I, [_P, _P, _U64, _P, _U64, _P, _P])
+    _sig(lib, "ps5_sort_device_noalloc", _I, [_P, _P, _U64, _P, _U64, _P, _P])
+    _sig(lib, "ps5_noalloc_max_strings", _U64, [])
     _sig(lib, "ps5_sort_device_timed", _I, [_P, _P, _U64, _P, _U64, _P, _P, ctypes.POINTER(ctypes.c_double)])
     _sig(lib, "ps5_ctx_set_profile", _I, [_P, _I])
     _sig(lib, "ps5_ctx_kernel_count", _I, [_P])
