@@ -910,6 +910,8 @@ __device__ __forceinline__ void thread_sort(key64 (&k)[E], uint32_t (&p)[E])
 }
 
 constexpr uint32_t kBlockRuns = 64;     // short runs a block finishes itself
+// keys per thread of the four block classes (256, 512, 1024, 2048 keys)
+constexpr int kBlkE0 = 4, kBlkE1 = 4, kBlkE2 = 4, kBlkE3 = 4;
 
 // Finish a block's short runs of equal keys (each at most 64 strings, at
 // sorted positions beg + (s_runs[r] >> 16)) at depth + 8: the runs are
@@ -970,49 +972,20 @@ __device__ __forceinline__ void block_finish_runs(const Bufs& B, key64* s_k, con
     }
 }
 
-// One workgroup of MAXM / E threads sorts a block of 65..MAXM strings
-// completely on chip and finishes every position at this depth (the
-// reference's insertion-sort base case, lcp_inssort.hpp:36-187, and the
-// equal-bucket rules, .hpp:1531-1557, on sorted keys).  Each thread sorts E
-// consecutive keys with a sorting network, then runs are merged pairwise by
-// merge path: a thread finds where its E outputs start in the two runs by
-// binary search (co-rank) and merges them sequentially; on equal keys the
-// left run goes first, so the order stays stable.  This costs far fewer
-// instructions per key than a bitonic network, and the kernel is bound by
-// instruction throughput.  Finishing a sorted position: LCP to its neighbour,
-// identical strings done, runs of equal keys without a NUL handed to round r+1.
+// Merge sort of a block's m keys, in s_k in position order (padding ~0 up
+// to the next multiple of E sorts last): each of MAXM / E threads sorts E
+// consecutive (key, position) pairs with a sorting network, then runs are
+// merged pairwise by merge path: a thread finds where its E outputs start in
+// the two runs by binary search (co-rank) and merges them sequentially; on
+// equal keys the left run goes first, so the order stays stable.  Leaves the
+// sorted keys in s_k and their positions in s_p.
 template <uint32_t MAXM, int E>
-__global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t first, const uint32_t* cnt_items,
-                                                     Bufs B, Sink nxt, uint32_t depth)
+__device__ __forceinline__ void block_merge_sort(uint32_t m, key64* s_k, uint16_t* s_p)
 {
     constexpr uint32_t T = MAXM / E;
-    static_assert(T % 64 == 0 && MAXM <= 65536, "whole waves; positions fit 16 bits");
-    __shared__ key64 s_k[MAXM + 1];     // + 1: a merge step reads the slot after the one it takes
-    __shared__ uint16_t s_p[MAXM];
-    constexpr uint32_t kMaxRuns = kBlockRuns;    // short equal runs finished here (more go to the next round)
-    __shared__ uint32_t s_runs[kMaxRuns];        // (start << 16) | length
-    __shared__ uint32_t s_nruns;
-    static_assert(MAXM * 8 >= 1024 * (T / 64) + 4 * (2 * kMaxRuns + 2),
-                  "the per-wave leaf arrays (1 KB each) and the run batches reuse s_k");
-    const uint32_t idx = first + blockIdx.x;
-    if (idx >= *cnt_items) return;
-    const Item L = items[idx];
-    if (!item_ok(L, B) || L.size > MAXM) {
-        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
-        return;
-    }
-    const uint32_t m = L.size, beg = L.begin, tid = threadIdx.x;
-    const int src = L.flags & F_BUFB;
-    const key64* K = src ? B.key1 : B.key0;
-    const uint32_t* I = src ? B.idx1 : B.idx0;
-    const bool ident = L.flags & F_IDENT;
+    (void)T;
+    const uint32_t tid = threadIdx.x;
     const uint32_t Ta = (m + E - 1) / E, Pm = Ta * E;   // threads in use, padded size
-    if (tid == 0) s_nruns = 0;
-    // coalesced loads; padding (key ~0, position >= m) sorts last
-    for (uint32_t i = tid; i < Pm; i += T) {
-        s_k[i] = i < m ? K[beg + i] : ~0ull;
-    }
-    __syncthreads();
     key64 k[E];
     uint32_t p[E];
     const uint32_t o = tid * E;
@@ -1077,6 +1050,244 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
         }
         __syncthreads();
     }
+}
+
+// Radix partition of a block.  The keys of a block lie between two
+// neighbouring splitters, so their offsets from the block's smallest key
+// usually fit 48 bits; then v = offset << 16 | position orders the block
+// exactly as (key, position) does, in one 64-bit word.  Values are grouped
+// by their top log2(MAXM) offset bits and ranked inside their groups.  A
+// thread's ranking work grows with the sizes of the groups its slots fall
+// in, so a block is merge-sorted instead when its offsets need more than 48
+// bits, a group holds more than kRadixMaxGroup keys, or the key-weighted
+// mean group size (sum of squared group sizes over the keys) exceeds
+// kRadixMeanGroup.
+constexpr uint32_t kRadixMaxGroup = 128;
+constexpr uint32_t kRadixMeanGroup = 16;
+
+template <int NW>
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* s_tmp)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
+    __syncthreads();
+    if (lane_id() == 0) s_tmp[wave_id()] = v;
+    __syncthreads();
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) r = max(r, s_tmp[i]);
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ key64 shfl_xor64(key64 v, int d)
+{
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, d, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), d, 64);
+    return ((key64)hi << 32) | lo;
+}
+
+// Radix-partitioned rank sort of a block's m keys (position i = key K[beg+i]):
+// values v are grouped in LDS by d = v >> (16 + sh), the top log2(MAXM)
+// bits of the offset (slots from LDS atomics, in no particular order), and
+// each value's sorted slot is its group's start plus the group members below
+// it -- exact, so the result is the stable sort.  Returns false when the
+// block is to be merge-sorted; s_k then holds the keys in position order,
+// padded with ~0 up to the next multiple of E, for block_merge_sort.  On
+// success s_k holds the sorted keys and s_p their positions.  s_cnt: MAXM +
+// 2 counters (group counts, the total, a spare for lanes past m).
+template <uint32_t MAXM, int E>
+__device__ __forceinline__ bool block_radix_sort(const key64* __restrict__ K, uint32_t beg, uint32_t m,
+                                                 key64* s_k, uint16_t* s_p, uint32_t* s_cnt, uint32_t* s_tmp)
+{
+    constexpr uint32_t T = MAXM / E;
+    constexpr int NW = (int)(T / 64);
+    constexpr int D = __builtin_ctz(MAXM);      // digit bits: MAXM groups
+    static_assert((MAXM & (MAXM - 1)) == 0 && MAXM / T == (uint32_t)E, "a power-of-two block, E counters per thread");
+    const uint32_t tid = threadIdx.x;
+    const uint32_t Pm = (m + E - 1) / E * E;
+    // w: this thread's keys (positions e*T + tid), later their packed values
+    // every load is unconditional (clamped index), so all E are in flight at once
+    key64 w[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) w[e] = K[beg + min((uint32_t)e * T + tid, m - 1)];
+    key64 kmin = ~0ull, kmax = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        kmin = min(kmin, w[e]);     // (clamped copies of the last key change neither)
+        kmax = max(kmax, w[e]);
+    }
+    for (uint32_t j = tid; j < MAXM + 2; j += T) s_cnt[j] = 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        kmin = min(kmin, shfl_xor64(kmin, d));
+        kmax = max(kmax, shfl_xor64(kmax, d));
+    }
+    key64* s_mm = reinterpret_cast<key64*>(s_tmp);      // [NW] minima, then [NW] maxima
+    if (lane_id() == 0) {
+        s_mm[wave_id()] = kmin;
+        s_mm[NW + wave_id()] = kmax;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        kmin = min(kmin, s_mm[i]);
+        kmax = max(kmax, s_mm[NW + i]);
+    }
+    __syncthreads();
+    const key64 range = kmax - kmin;
+    if ((range >> 48) != 0) {     // (uniform) offsets too wide to pack: merge sort
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = (uint32_t)e * T + tid;
+            if (i < Pm) s_k[i] = i < m ? w[e] : ~0ull;
+        }
+        __syncthreads();
+        return false;
+    }
+    // the packed values, and the shift that leaves a value's group
+    const int sh = 16 + max(0, 64 - (int)__clzll((long long)(range | 1)) - D);
+    // (positions past m count into the spare counter s_cnt[MAXM + 1])
+    uint32_t slot[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = (uint32_t)e * T + tid;
+        w[e] = ((w[e] - kmin) << 16) | i;
+        slot[e] = atomicAdd(&s_cnt[i < m ? (uint32_t)(w[e] >> sh) : MAXM + 1], 1u);
+    }
+    __syncthreads();
+    // group starts (exclusive scan in place, E counters per thread); the
+    // largest per-thread sum of squared group sizes and their total
+    uint32_t sum = 0, sq = 0;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        const uint32_t c = s_cnt[tid * E + j];
+        sum += c;
+        sq += c * c;
+    }
+    uint32_t tot, tsq;
+    const uint32_t over = block_max_u32<NW>(sq, s_tmp);
+    uint32_t run = block_excl_scan<NW>(sum, &tot, s_tmp);
+    (void)block_excl_scan<NW>(sq, &tsq, s_tmp);     // (all counts are read: starts may overwrite them)
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        const uint32_t c = s_cnt[tid * E + j];
+        s_cnt[tid * E + j] = run;
+        run += c;
+    }
+    if (tid == 0) s_cnt[MAXM] = tot;
+    __syncthreads();
+    if (over > kRadixMaxGroup * kRadixMaxGroup || tsq > kRadixMeanGroup * m) {   // (uniform) merge sort instead
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const uint32_t i = (uint32_t)e * T + tid;
+            if (i < Pm) s_k[i] = i < m ? (w[e] >> 16) + kmin : ~0ull;
+        }
+        __syncthreads();
+        return false;
+    }
+    key64* s_v = s_k;     // packed values, grouped
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t i = (uint32_t)e * T + tid;
+        if (i < m) s_v[s_cnt[(uint32_t)(w[e] >> sh)] + slot[e]] = w[e];
+    }
+    __syncthreads();
+    // sorted places: thread t ranks its E consecutive slots [tE, tE + E) in
+    // batches of RB.  Groups are ordered by their top bits, so every value
+    // before a batch's first group is smaller and every value after its last
+    // group larger: a slot's place is G0 plus the values of [G0, G1) (the
+    // batch's groups) below it, and one pass over [G0, G1) ranks the batch.
+    // Places wait in s_p (free while ranking: positions ride in the values).
+    constexpr int RB = E < 4 ? E : 4;
+    const uint32_t j0 = tid * E;
+#pragma unroll 1
+    for (int b0 = 0; b0 < E; b0 += RB) {
+        const uint32_t jb = j0 + (uint32_t)b0;
+        if (jb >= m) break;
+        const uint32_t jl = min(jb + (uint32_t)RB, m) - 1;
+        const uint32_t G0 = s_cnt[(uint32_t)(s_v[jb] >> sh)];
+        const uint32_t G1 = s_cnt[(uint32_t)(s_v[jl] >> sh) + 1];
+        key64 wb[RB];
+        uint32_t rb[RB];
+#pragma unroll
+        for (int e = 0; e < RB; ++e) {
+            wb[e] = s_v[min(jb + (uint32_t)e, m - 1)];
+            rb[e] = G0;
+        }
+        uint32_t q = G0;
+        for (; q + 1 < G1; q += 2) {     // two values per step: their reads overlap
+            const key64 va = s_v[q], vb = s_v[q + 1];
+#pragma unroll
+            for (int e = 0; e < RB; ++e) rb[e] += (va < wb[e] ? 1u : 0u) + (vb < wb[e] ? 1u : 0u);
+        }
+        if (q < G1) {
+            const key64 va = s_v[q];
+#pragma unroll
+            for (int e = 0; e < RB; ++e) rb[e] += va < wb[e] ? 1u : 0u;
+        }
+#pragma unroll
+        for (int e = 0; e < RB; ++e)
+            if (jb + (uint32_t)e < m) s_p[jb + e] = (uint16_t)rb[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const uint32_t j = min(j0 + (uint32_t)e, m - 1);
+        w[e] = s_v[j];
+        slot[e] = s_p[j];     // the place
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (j0 + (uint32_t)e < m) {
+            s_k[slot[e]] = (w[e] >> 16) + kmin;
+            s_p[slot[e]] = (uint16_t)w[e];
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+// One workgroup of MAXM / E threads sorts a block of 65..MAXM strings
+// completely on chip and finishes every position at this depth (the
+// reference's insertion-sort base case, lcp_inssort.hpp:36-187, and the
+// equal-bucket rules, .hpp:1531-1557, on sorted keys).  The sort is the
+// radix partition with in-group ranks (block_radix_sort) when the block's
+// keys spread well over their range, else the merge sort (block_merge_sort);
+// both give the stable (key, position) order.  Finishing a sorted position:
+// LCP to its neighbour, identical strings done, runs of equal keys without a
+// NUL handed to round r+1.
+template <uint32_t MAXM, int E>
+__global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t first, const uint32_t* cnt_items,
+                                                     Bufs B, Sink nxt, uint32_t depth)
+{
+    constexpr uint32_t T = MAXM / E;
+    static_assert(T % 64 == 0 && MAXM <= 65536, "whole waves; positions fit 16 bits");
+    __shared__ key64 s_k[MAXM + 1];     // + 1: a merge step reads the slot after the one it takes
+    __shared__ uint16_t s_p[MAXM];
+    constexpr uint32_t kMaxRuns = kBlockRuns;    // short equal runs finished here (more go to the next round)
+    __shared__ uint32_t s_runs[kMaxRuns];        // (start << 16) | length
+    __shared__ uint32_t s_nruns;
+    __shared__ uint32_t s_cnt[MAXM + 2];         // radix partition: group counts, then starts
+    __shared__ __align__(8) uint32_t s_tmp[4 * (T / 64) + 4];
+    static_assert(MAXM * 8 >= 1024 * (T / 64) + 4 * (2 * kMaxRuns + 2),
+                  "the per-wave leaf arrays (1 KB each) and the run batches reuse s_k");
+    const uint32_t idx = first + blockIdx.x;
+    if (idx >= *cnt_items) return;
+    const Item L = items[idx];
+    if (!item_ok(L, B) || L.size > MAXM) {
+        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
+        return;
+    }
+    const uint32_t m = L.size, beg = L.begin, tid = threadIdx.x;
+    const int src = L.flags & F_BUFB;
+    const key64* K = src ? B.key1 : B.key0;
+    const uint32_t* I = src ? B.idx1 : B.idx0;
+    const bool ident = L.flags & F_IDENT;
+    if (tid == 0) s_nruns = 0;
+    if (!block_radix_sort<MAXM, E>(K, beg, m, s_k, s_p, s_cnt, s_tmp))
+        block_merge_sort<MAXM, E>(m, s_k, s_p);
     // the input index of every sorted position, read where it lies before
     // finish_position may overwrite those slots (continuing runs store theirs)
     constexpr int kPerThread = MAXM / T;
@@ -2242,16 +2453,16 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                 {
                     Launch L(c, K_BLOCK, 0);
                     if (cls == 0)
-                        hipLaunchKernelGGL((k_block<256, 4>), dim3(nblk[cls]), dim3(256 / 4), 0, st, list,
+                        hipLaunchKernelGGL((k_block<256, kBlkE0>), dim3(nblk[cls]), dim3(256 / kBlkE0), 0, st, list,
                                            blk_done[cls], cnt, B, nsink, depth);
                     else if (cls == 1)
-                        hipLaunchKernelGGL((k_block<512, 8>), dim3(nblk[cls]), dim3(512 / 8), 0, st, list,
+                        hipLaunchKernelGGL((k_block<512, kBlkE1>), dim3(nblk[cls]), dim3(512 / kBlkE1), 0, st, list,
                                            blk_done[cls], cnt, B, nsink, depth);
                     else if (cls == 2)
-                        hipLaunchKernelGGL((k_block<1024, 8>), dim3(nblk[cls]), dim3(1024 / 8), 0, st, list,
+                        hipLaunchKernelGGL((k_block<1024, kBlkE2>), dim3(nblk[cls]), dim3(1024 / kBlkE2), 0, st, list,
                                            blk_done[cls], cnt, B, nsink, depth);
                     else
-                        hipLaunchKernelGGL((k_block<kBlkT, 8>), dim3(nblk[cls]), dim3(kBlkT / 8), 0, st, list,
+                        hipLaunchKernelGGL((k_block<kBlkT, kBlkE3>), dim3(nblk[cls]), dim3(kBlkT / kBlkE3), 0, st, list,
                                            blk_done[cls], cnt, B, nsink, depth);
                 }
                 sum_sizes(c, list, blk_done[cls], cnt, blk_done[cls] + nblk[cls], S_BLOCK);
