@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     const uint32_t ce = (item_ok(seg, B) && t < D.T && D.chunk <= cmax) ? min(seg.size, cb + D.chunk) : 0u;
     // kGroups 64-key groups per iteration: all loads are in flight before the
     // tree descents, then the groups are ranked in order (stable)
-    constexpr int kGroups = 4;
+    constexpr int kGroups = 8;
     // extracting at the root, the string offsets of the next iteration are
     // loaded one iteration ahead (the first link of offset -> characters)
     uint64_t off_next[kGroups];
