@@ -134,6 +134,18 @@ int ps5_sort_device_noalloc(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena
 /* Largest n ps5_sort_device_noalloc accepts. */
 uint64_t ps5_noalloc_max_strings(void);
 
+/* Where a sort's levels are planned.  0 (host): after every level the large
+ * segments are read back and the next level is planned on the host, with
+ * grids sized exactly.  1 (device): a kernel plans each level from the
+ * device's own lists and every kernel loops over the work it finds, so a
+ * round of levels runs without any host synchronisation (one per round, to
+ * learn whether another round is needed).  Both give the same results.  No
+ * counterpart in the reference (its levels are recursion on the CPU).  The
+ * default is 0 unless the environment sets PS5_DEV_LEVELS=1. */
+int ps5_ctx_set_planning(ps5_ctx* ctx, int device);
+/* The current planning mode (0 or 1), -1 for a NULL context. */
+int ps5_ctx_get_planning(ps5_ctx* ctx);
+
 /* Distinguishing characters of a finished sort (inputs as ps5_sort_device,
  * d_perm and d_lcp its results): d_cache[i] = s_i[lcp[i]] for i >= 1, the
  * first character of the i-th smallest string that its predecessor does not

@@ -103,8 +103,17 @@ __device__ __forceinline__ uint32_t key_depth(key64 a)
     return a ? 8u - ((uint32_t)__builtin_ctzll(a) >> 3) : 0u;
 }
 
-__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
-__device__ __forceinline__ uint32_t wave_id() { return threadIdx.x >> 6; }
+// tid_x() through an opaque move: values derived from it are recomputed
+// where they are used, instead of being hoisted out of the persistent loops
+// of the kernels and holding registers across every iteration
+__device__ __forceinline__ uint32_t tid_x()
+{
+    uint32_t t;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((uint32_t)threadIdx.x));
+    return t;
+}
+__device__ __forceinline__ uint32_t lane_id() { return tid_x() & 63u; }
+__device__ __forceinline__ uint32_t wave_id() { return (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid_x() >> 6)); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
 // (key, position) order: positions are distinct, so the order is total and

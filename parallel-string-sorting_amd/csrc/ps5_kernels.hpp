@@ -80,7 +80,8 @@ enum : int {
     C_BLOCK = 6,   // [4][2] blocks: C_BLOCK + 2 * size class + round parity
     C_RUN = 14,    // [2]
     C_ERR = 16,    // error flags
-    C_NCNT = 18
+    C_BDONE = 17,  // [4] blocks of the current round already sorted, per size class (device-driven levels)
+    C_NCNT = 21
 };
 constexpr uint32_t ERR_CAP = 1u, ERR_LCP = 2u, ERR_BOUNDS = 4u;
 
@@ -97,6 +98,23 @@ struct LargeDesc {       // one segment of the multi-workgroup sample-sort step
 struct TileRef {         // one workgroup of the multi-workgroup step
     uint32_t seg;        // index into the LargeDesc array
     uint32_t t;          // tile inside the segment
+};
+
+struct LevelHdr {        // totals of one level's multi-workgroup step
+    uint32_t nlarge;     // segments (LargeDesc entries)
+    uint32_t ntiles;     // tiles (TileRef entries), kWaves chunks each
+    uint32_t hl;         // histogram entries
+    uint32_t kmax;       // largest treebits of the level
+    uint32_t pad[4];
+};
+
+// A level's plan in device memory: the header, then cap_large descriptors,
+// then the tile table (fixed offsets, so a plan written on the device needs
+// no pointer arithmetic on the host).
+struct Plan {
+    const LevelHdr* hdr;
+    const LargeDesc* desc;
+    const TileRef* tiles;
 };
 
 struct Sink {            // where new items are appended
@@ -120,6 +138,7 @@ struct Bufs {
     uint32_t* lcp;       // may be null
     key64* bkey;
     uint32_t* bktrank;
+    const uint32_t* pending;   // non-null and non-zero: large segments remain, the round's leaves wait
 };
 
 }  // namespace ps5

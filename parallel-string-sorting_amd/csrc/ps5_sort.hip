@@ -41,7 +41,7 @@ using namespace ps5;
 // so idx0 is not written.
 __global__ __launch_bounds__(kWG) void k_init(Bufs B, uint32_t n)
 {
-    const uint32_t i = blockIdx.x * kWG + threadIdx.x;
+    const uint32_t i = blockIdx.x * kWG + tid_x();
     if (i >= n) return;
     B.key0[i] = load_key(B.arena, B.alen, str_off(B, i));
 }
@@ -49,9 +49,9 @@ __global__ __launch_bounds__(kWG) void k_init(Bufs B, uint32_t n)
 // Reset counters and place the root item.
 __global__ void k_setup(uint32_t* cnt, Item* list, int c_index, uint32_t n)
 {
-    if (threadIdx.x < C_NCNT) cnt[threadIdx.x] = 0;
+    if (tid_x() < C_NCNT) cnt[tid_x()] = 0;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid_x() == 0) {
         Item it = { 0u, n, 0u, F_IDENT };
         list[0] = it;
         cnt[c_index] = 1;
@@ -66,28 +66,139 @@ constexpr uint32_t kStageHead = 128;
 static_assert(C_NCNT * 4 <= kStageHead, "counters must fit the stage head");
 __global__ __launch_bounds__(kWG) void k_stage(const uint32_t* cnt, const Item* list, int ci, unsigned char* out)
 {
-    if (threadIdx.x < C_NCNT) reinterpret_cast<uint32_t*>(out)[threadIdx.x] = cnt[threadIdx.x];
+    if (tid_x() < C_NCNT) reinterpret_cast<uint32_t*>(out)[tid_x()] = cnt[tid_x()];
     if (!list) return;
     const uint32_t nl = min(cnt[ci], kPeek);
     Item* dst = reinterpret_cast<Item*>(out + kStageHead);
-    for (uint32_t i = threadIdx.x; i < nl; i += kWG) dst[i] = list[i];
+    for (uint32_t i = tid_x(); i < nl; i += kWG) dst[i] = list[i];
 }
 
-// copy 8-byte words (the level plan, from mapped host memory to the device)
-__global__ __launch_bounds__(kWG) void k_fetch(const uint64_t* src, uint64_t* dst, uint64_t words)
+// copy 8-byte words of a level plan from mapped host memory to the device:
+// the header with the descriptors, then the tile table
+__global__ __launch_bounds__(kWG) void k_fetch(const uint64_t* src, uint64_t* dst, uint64_t words,
+                                               const uint64_t* src2, uint64_t* dst2, uint64_t words2)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x; i < words; i += (uint64_t)gridDim.x * kWG)
-        dst[i] = src[i];
+    const uint64_t stride = (uint64_t)gridDim.x * kWG;
+    for (uint64_t i = (uint64_t)blockIdx.x * kWG + tid_x(); i < words; i += stride) dst[i] = src[i];
+    for (uint64_t i = (uint64_t)blockIdx.x * kWG + tid_x(); i < words2; i += stride) dst2[i] = src2[i];
 }
 
 // zero the counters whose bit is set in `mask`
 __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 {
-    const int t = threadIdx.x;
+    const int t = tid_x();
     if (t < C_NCNT && ((mask >> t) & 1u)) cnt[t] = 0;
 }
 
 // ---- multi-workgroup sample-sort step (segments > kBlkT) -------------------
+
+// ---- device-planned levels ---------------------------------------------------
+
+// Treebits of a multi-workgroup step on m strings (device copy of the host's
+// large_treebits): the fewest levels of at most maxbits that bring buckets
+// down to about `target` strings, the fan-out spread evenly over them.
+__device__ __forceinline__ uint32_t dev_treebits(uint32_t m, uint32_t target, uint32_t maxbits)
+{
+    const double need = log2(fmax(2.0, (double)m / target));
+    const int levels = max(1, (int)ceil(need / maxbits));
+    const int k = (int)ceil(need / levels);
+    return (uint32_t)max(1, min((int)maxbits, k));
+}
+
+struct PlanArgs {
+    const Item* list;        // the level's large segments
+    int ci;                  // their counter
+    uint32_t cap_large;      // capacity of that list
+    unsigned char* plan;     // header, descriptors, tile table (tiles at tiles_off)
+    uint64_t tiles_off, tiles_cap, hist_cap, trees_cap;
+    uint32_t leaf_target, max_treebits, max_oversample;
+    uint32_t kmax;           // bound of the count launch (treebits are clamped to it)
+    int done_rp;             // >= 0: mark the blocks of round parity done_rp listed so far as sorted
+};
+
+// Plan one level on the device (one workgroup of kBigWG threads): the same
+// descriptors, tile table and totals that the host's large_step computes
+// from a read-back list.  A level that does not fit the buffers is flagged
+// (ERR_CAP) and planned empty.
+__global__ __launch_bounds__(kBigWG) void k_plan(PlanArgs A, uint32_t* cnt)
+{
+    __shared__ uint32_t s_tmp[kBigWaves];
+    __shared__ unsigned long long s_tot;
+    __shared__ uint32_t s_ident, s_kmax;
+    LevelHdr* hdr = reinterpret_cast<LevelHdr*>(A.plan);
+    LargeDesc* desc = reinterpret_cast<LargeDesc*>(A.plan + sizeof(LevelHdr));
+    TileRef* tiles = reinterpret_cast<TileRef*>(A.plan + A.tiles_off);
+    const uint32_t tid = tid_x();
+    if (A.done_rp >= 0 && tid < kBlkClasses) cnt[C_BDONE + tid] = cnt[C_BLOCK + 2 * tid + A.done_rp];
+    uint32_t nl = cnt[A.ci];
+    if (nl > A.cap_large) {      // (an overflow of the list has been flagged)
+        nl = 0;
+        if (tid == 0) flag_error(cnt, ERR_CAP);
+    }
+    if (tid == 0) {
+        s_tot = 0;
+        s_ident = 1;
+        s_kmax = 1;
+    }
+    __syncthreads();
+    unsigned long long tot = 0;
+    bool ident = true;
+    for (uint32_t i = tid; i < nl; i += kBigWG) {
+        tot += A.list[i].size;
+        ident = ident && (A.list[i].flags & F_IDENT);
+    }
+    atomicAdd(&s_tot, tot);
+    if (!ident) atomicAnd(&s_ident, 0u);
+    __syncthreads();
+    const uint64_t cmax = s_ident ? kChunkMaxIdent : kChunkMax;
+    const uint64_t climit = min(cmax, max((uint64_t)kChunkMin, (s_tot + kWaves * kMinTiles - 1) / (kWaves * kMinTiles)));
+    // descriptors: tile and histogram bases are exclusive scans over the
+    // segments (blocks of kBigWG with carries); pad holds the tile base
+    uint32_t tcarry = 0;
+    uint64_t hcarry = 0;
+    for (uint32_t b0 = 0; b0 < nl; b0 += kBigWG) {
+        const uint32_t i = b0 + tid;
+        uint32_t T = 0, hc = 0;
+        LargeDesc D{};
+        if (i < nl) {
+            D.seg = A.list[i];
+            T = (uint32_t)((D.seg.size + climit * kWaves - 1) / (climit * kWaves));
+            D.T = T;
+            D.chunk = (uint32_t)((D.seg.size + (uint64_t)T * kWaves - 1) / ((uint64_t)T * kWaves));
+            D.k = min(dev_treebits(D.seg.size, A.leaf_target, A.max_treebits), A.kmax);
+            const uint32_t nspl = (1u << D.k) - 1;
+            D.oversample = max(2u, min(A.max_oversample, (kMaxSamples / nspl) & ~1u));
+            hc = ((2u << D.k) - 1) * T * kWaves;
+            atomicMax(&s_kmax, D.k);
+        }
+        uint32_t ttot, htot;
+        const uint32_t tex = block_excl_scan<kBigWaves>(T, &ttot, s_tmp);
+        const uint32_t hex = block_excl_scan<kBigWaves>(hc, &htot, s_tmp);
+        if (i < nl) {
+            D.pad = tcarry + tex;
+            D.hist_base = (uint32_t)(hcarry + hex);
+            desc[i] = D;
+        }
+        tcarry += ttot;
+        hcarry += htot;
+    }
+    const bool fits = tcarry <= A.tiles_cap && hcarry <= A.hist_cap && hcarry <= 0xFFFFFFFFull && nl <= A.trees_cap;
+    if (!fits && tid == 0) flag_error(cnt, ERR_CAP);
+    const uint32_t ntiles = fits ? tcarry : 0u;
+    __syncthreads();     // every descriptor is written
+    // tile table: tile t belongs to the last segment whose tile base is <= t
+    for (uint32_t t = tid; t < ntiles; t += kBigWG) {
+        uint32_t lo = 0, hi = nl;            // desc[lo].pad <= t < desc[hi].pad (hi = nl: past the end)
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (desc[mid].pad <= t) lo = mid;
+            else hi = mid;
+        }
+        tiles[t] = TileRef{ lo, t - desc[lo].pad };
+    }
+    if (tid == 0)
+        *hdr = LevelHdr{ fits ? nl : 0u, ntiles, fits ? (uint32_t)hcarry : 0u, s_kmax, { 0u, 0u, 0u, 0u } };
+}
 
 // Splitter trees of the large segments of a level, one workgroup each.
 // D.oversample * nspl random keys are sorted (4-key runs per thread by a
@@ -98,14 +209,12 @@ __global__ void k_reset(uint32_t* cnt, uint32_t mask)
 // sorted order at trees[seg*2048 + 1024 + i], i < nspl.  With from_arena (the root
 // of a sort whose keys are not extracted yet) the samples are read from the
 // arena directly.
-__global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, uint64_t seed, key64* trees,
-                                                  int from_arena)
+__device__ __forceinline__ void tree_of_segment(const LargeDesc D, const Bufs& B, uint64_t seed, key64* tree,
+                                                int from_arena, key64* s_samp)
 {
-    __shared__ key64 s_samp[kMaxSamples + 1];     // + 1: a merge step reads the slot after the one it takes
-    const LargeDesc D = desc[blockIdx.x];
     const Item seg = D.seg;
     if (!item_ok(seg, B)) {
-        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
+        if (tid_x() == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
     const key64* K = (seg.flags & F_BUFB) ? B.key1 : B.key0;
@@ -114,7 +223,7 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
     uint32_t P = 64;
     while (P < ss) P <<= 1;
     if (P > (uint32_t)kMaxSamples) {
-        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
+        if (tid_x() == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
     // every sample load is issued first (kPer consecutive samples per
@@ -124,7 +233,7 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
     // keys need no tie order, only the splitters are kept)
     constexpr int kPer = kMaxSamples / kBigWG;
     static_assert(kMaxSamples % kBigWG == 0 && kPer == 4, "four samples per thread");
-    const uint32_t o = threadIdx.x * kPer;
+    const uint32_t o = tid_x() * kPer;
     key64 v[kPer];
 #pragma unroll
     for (int e = 0; e < kPer; ++e) {
@@ -187,9 +296,18 @@ __global__ __launch_bounds__(kBigWG) void k_tree(const LargeDesc* desc, Bufs B, 
         __syncthreads();
     }
     const key64* sorted = sa;
-    key64* tree = trees + (uint64_t)blockIdx.x * kTreeSlots;
-    for (int t = threadIdx.x + 1; t <= nspl; t += kBigWG) tree[t] = sorted[a / 2 + a * inorder_of(t, k)];
-    for (int i = threadIdx.x; i < nspl; i += kBigWG) tree[1024 + i] = sorted[a / 2 + a * i];
+    for (int t = tid_x() + 1; t <= nspl; t += kBigWG) tree[t] = sorted[a / 2 + a * inorder_of(t, k)];
+    for (int i = tid_x(); i < nspl; i += kBigWG) tree[1024 + i] = sorted[a / 2 + a * i];
+}
+
+__global__ __launch_bounds__(kBigWG) void k_tree(Plan P, Bufs B, uint64_t seed, key64* trees, int from_arena)
+{
+    __shared__ key64 s_samp[kMaxSamples + 1];     // + 1: a merge step reads the slot after the one it takes
+    const uint32_t nlarge = P.hdr->nlarge;
+    for (uint32_t si = blockIdx.x; si < nlarge; si += gridDim.x) {
+        __syncthreads();     // the previous segment's samples are consumed
+        tree_of_segment(P.desc[si], B, seed, trees + (uint64_t)si * kTreeSlots, from_arena, s_samp);
+    }
 }
 
 // Classify one tile of kWaves wave chunks.  Each wave ranks its own chunk
@@ -213,30 +331,25 @@ __host__ __device__ constexpr uint32_t count_lds_bytes(int kmax)
 // NB: bits of a bucket id in this launch (kmax + 1, at least 8): the ballot
 // matching of ranks costs a few vector instructions per bit.
 template <int NB>
-__global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const TileRef* tiles, Bufs B,
-                                                const key64* trees, uint32_t* hist, int from_arena, int kmax)
+__device__ __forceinline__ void count_tile(uint32_t ti, const Plan& P, const Bufs& B, const key64* trees,
+                                           uint32_t* hist, int from_arena, int kmax, key64* s_tree, key64* s_sorted,
+                                           uint16_t* s_hist_base)
 {
-    static_assert(NB >= 2 && NB <= kMaxTreebits + 1, "bucket ids of up to 2047 buckets");
     constexpr uint32_t kNoBkt = (1u << NB) - 1;   // id of inactive lanes, above every bucket of the launch
-    extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ key64 s_tree[1 << (NB - 1)];        // trees of up to NB - 1 levels
-    __shared__ key64 s_sorted[1 << (NB - 1)];
     const uint32_t nb2 = 2u << kmax;               // bucket slots per wave
-    uint16_t* s_hist_base = reinterpret_cast<uint16_t*>(smem);
-    static_assert(kChunkMax <= 65535 && kChunkMaxIdent <= 65535, "per-wave bucket counts must fit 16 bits");
-    const TileRef tr = tiles[xcd_contiguous(blockIdx.x, gridDim.x)];
-    const LargeDesc D = desc[tr.seg];
+    const TileRef tr = P.tiles[ti];
+    const LargeDesc D = P.desc[tr.seg];
     const Item seg = D.seg;
     const int k = (int)D.k, nbkt = (2 << k) - 1;
     if (k > kmax || k + 1 > NB) {   // the launch must size LDS and ids for every tree of the level
-        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
+        if (tid_x() == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
     const key64* tree = trees + (uint64_t)tr.seg * kTreeSlots;
     const int nspl = (1 << k) - 1;
-    for (int i = threadIdx.x; i <= nspl; i += kWG) s_tree[i] = tree[i];
-    for (int i = threadIdx.x; i < nspl; i += kWG) s_sorted[i] = tree[1024 + i];
-    for (uint32_t i = threadIdx.x; i < kWaves * nb2; i += kWG) s_hist_base[i] = 0;
+    for (int i = tid_x(); i <= nspl; i += kWG) s_tree[i] = tree[i];
+    for (int i = tid_x(); i < nspl; i += kWG) s_sorted[i] = tree[1024 + i];
+    for (uint32_t i = tid_x(); i < kWaves * nb2; i += kWG) s_hist_base[i] = 0;
     __syncthreads();
     const key64* K = (seg.flags & F_BUFB) ? B.key1 : B.key0;
     const uint32_t t = tr.t;
@@ -309,9 +422,36 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
     }
     __syncthreads();
     uint32_t* h = hist + D.hist_base;
-    for (int i = threadIdx.x; i < nbkt * kWaves; i += kWG) {
+    for (int i = tid_x(); i < nbkt * kWaves; i += kWG) {
         const uint32_t b = (uint32_t)i / kWaves, ww = (uint32_t)i % kWaves;
         h[((uint32_t)b * D.T + t) * kWaves + ww] = s_hist_base[ww * nb2 + b];
+    }
+}
+
+// XCD-contiguous tiles: workgroup g runs on XCD g % kXcds (the grid is a
+// multiple of kXcds), and XCD x takes the tiles [N x / kXcds, N (x+1) /
+// kXcds): one per workgroup, or with LOOP (levels planned on the device)
+// each workgroup loops over its XCD's share.
+template <int NB, bool LOOP>
+__global__ __launch_bounds__(kWG) void k_count(Plan P, Bufs B, const key64* trees, uint32_t* hist, int from_arena,
+                                                int kmax)
+{
+    static_assert(NB >= 2 && NB <= kMaxTreebits + 1, "bucket ids of up to 2047 buckets");
+    static_assert(kChunkMax <= 65535 && kChunkMaxIdent <= 65535, "per-wave bucket counts must fit 16 bits");
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ key64 s_tree[1 << (NB - 1)];        // trees of up to NB - 1 levels
+    __shared__ key64 s_sorted[1 << (NB - 1)];
+    uint16_t* s_hist_base = reinterpret_cast<uint16_t*>(smem);
+    const uint32_t N = P.hdr->ntiles, per = gridDim.x / kXcds, xcd = blockIdx.x % kXcds;
+    const uint32_t thi = (uint32_t)((uint64_t)N * (xcd + 1) / kXcds);
+    uint32_t ti = (uint32_t)((uint64_t)N * xcd / kXcds) + blockIdx.x / kXcds;
+    if (!LOOP) {
+        if (ti < thi) count_tile<NB>(ti, P, B, trees, hist, from_arena, kmax, s_tree, s_sorted, s_hist_base);
+        return;
+    }
+    for (; ti < thi; ti += per) {
+        __syncthreads();     // the previous tile's counts are written out
+        count_tile<NB>(ti, P, B, trees, hist, from_arena, kmax, s_tree, s_sorted, s_hist_base);
     }
 }
 
@@ -319,28 +459,33 @@ __global__ __launch_bounds__(kWG) void k_count(const LargeDesc* desc, const Tile
 constexpr int kScanPer = 16;
 constexpr int kScanBlock = kWG * kScanPer;   // 4096 entries per block
 
-__global__ __launch_bounds__(kWG) void k_scan_reduce(const uint32_t* in, uint32_t len, uint32_t* part)
+// (the length is the level's hl; workgroups loop over the blocks of kScanBlock entries)
+__global__ __launch_bounds__(kWG) void k_scan_reduce(const uint32_t* in, const LevelHdr* hdr, uint32_t* part)
 {
     __shared__ uint32_t s_tmp[16];
-    const uint32_t base = blockIdx.x * kScanBlock;
-    uint32_t sum = 0;
+    const uint32_t len = hdr->hl, nb = (len + kScanBlock - 1) / kScanBlock;
+    for (uint32_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+        const uint32_t base = blk * kScanBlock;
+        uint32_t sum = 0;
 #pragma unroll
-    for (int j = 0; j < kScanPer; ++j) {
-        const uint32_t i = base + j * kWG + threadIdx.x;
-        if (i < len) sum += in[i];
+        for (int j = 0; j < kScanPer; ++j) {
+            const uint32_t i = base + j * kWG + tid_x();
+            if (i < len) sum += in[i];
+        }
+        uint32_t tot;
+        block_excl_scan<kWaves>(sum, &tot, s_tmp);
+        if (tid_x() == 0) part[blk] = tot;
     }
-    uint32_t tot;
-    block_excl_scan<kWaves>(sum, &tot, s_tmp);
-    if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
 // exclusive scan of the block sums (one workgroup, any count)
-__global__ __launch_bounds__(kWG) void k_scan_top(uint32_t* part, uint32_t nparts)
+__global__ __launch_bounds__(kWG) void k_scan_top(uint32_t* part, const LevelHdr* hdr)
 {
     __shared__ uint32_t s_tmp[16];
+    const uint32_t nparts = (hdr->hl + kScanBlock - 1) / kScanBlock;
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nparts; base += kWG) {
-        const uint32_t i = base + threadIdx.x;
+        const uint32_t i = base + tid_x();
         const uint32_t v = i < nparts ? part[i] : 0u;
         uint32_t tot;
         const uint32_t ex = block_excl_scan<kWaves>(v, &tot, s_tmp);
@@ -352,46 +497,49 @@ __global__ __launch_bounds__(kWG) void k_scan_top(uint32_t* part, uint32_t npart
 // exclusive scan inside each block, offset by its block prefix (in place)
 // (data is 16-byte aligned: a thread's kScanPer consecutive entries move as
 // 16-byte vectors, except in a block cut short by len)
-__global__ __launch_bounds__(kWG) void k_scan_down(uint32_t* data, uint32_t len, const uint32_t* part)
+__global__ __launch_bounds__(kWG) void k_scan_down(uint32_t* data, const LevelHdr* hdr, const uint32_t* part)
 {
     static_assert(kScanPer % 4 == 0, "whole 16-byte vectors per thread");
     __shared__ uint32_t s_tmp[16];
-    const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
-    const bool full = base + kScanPer <= len;
-    uint32_t v[kScanPer];
-    if (full) {
-        const uint4* p = reinterpret_cast<const uint4*>(data + base);
+    const uint32_t len = hdr->hl, nb = (len + kScanBlock - 1) / kScanBlock;
+    for (uint32_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+        const uint32_t base = blk * kScanBlock + tid_x() * kScanPer;
+        const bool full = base + kScanPer <= len;
+        uint32_t v[kScanPer];
+        if (full) {
+            const uint4* p = reinterpret_cast<const uint4*>(data + base);
 #pragma unroll
-        for (int q = 0; q < kScanPer / 4; ++q) {
-            const uint4 x = p[q];
-            v[4 * q] = x.x;
-            v[4 * q + 1] = x.y;
-            v[4 * q + 2] = x.z;
-            v[4 * q + 3] = x.w;
+            for (int q = 0; q < kScanPer / 4; ++q) {
+                const uint4 x = p[q];
+                v[4 * q] = x.x;
+                v[4 * q + 1] = x.y;
+                v[4 * q + 2] = x.z;
+                v[4 * q + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kScanPer; ++j) v[j] = base + j < len ? data[base + j] : 0u;
         }
-    } else {
+        uint32_t sum = 0;
 #pragma unroll
-        for (int j = 0; j < kScanPer; ++j) v[j] = base + j < len ? data[base + j] : 0u;
-    }
-    uint32_t sum = 0;
+        for (int j = 0; j < kScanPer; ++j) sum += v[j];
+        uint32_t tot;
+        uint32_t run = block_excl_scan<kWaves>(sum, &tot, s_tmp) + part[blk];
+        uint32_t o[kScanPer];
 #pragma unroll
-    for (int j = 0; j < kScanPer; ++j) sum += v[j];
-    uint32_t tot;
-    uint32_t run = block_excl_scan<kWaves>(sum, &tot, s_tmp) + part[blockIdx.x];
-    uint32_t o[kScanPer];
+        for (int j = 0; j < kScanPer; ++j) {
+            o[j] = run;
+            run += v[j];
+        }
+        if (full) {
+            uint4* p = reinterpret_cast<uint4*>(data + base);
 #pragma unroll
-    for (int j = 0; j < kScanPer; ++j) {
-        o[j] = run;
-        run += v[j];
-    }
-    if (full) {
-        uint4* p = reinterpret_cast<uint4*>(data + base);
+            for (int q = 0; q < kScanPer / 4; ++q) p[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+        } else {
 #pragma unroll
-        for (int q = 0; q < kScanPer / 4; ++q) p[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
-    } else {
-#pragma unroll
-        for (int j = 0; j < kScanPer; ++j)
-            if (base + j < len) data[base + j] = o[j];
+            for (int j = 0; j < kScanPer; ++j)
+                if (base + j < len) data[base + j] = o[j];
+        }
     }
 }
 
@@ -429,7 +577,7 @@ static_assert(ScatLds<false>::kBytes <= 160 * 1024 && ScatLds<true>::kBytes <= 1
 // so the chunks in flight on an XCD are neighbours that share histogram
 // lines and bucket output lines in its L2.
 
-// threadIdx.x through an opaque move: addresses derived from it are
+// tid_x() through an opaque move: addresses derived from it are
 // recomputed per chunk instead of being hoisted out of the chunk loop,
 // where they would hold registers the two chunk buffers need
 __device__ __forceinline__ uint32_t scat_tid()
@@ -584,11 +732,12 @@ __device__ __forceinline__ void scat_write(const Item& seg, uint32_t m, uint32_t
 }
 
 template <bool IDENT>
-__global__ __launch_bounds__(kBigWG) void k_scatter(const LargeDesc* __restrict__ desc,
-                                                     const TileRef* __restrict__ tiles, Bufs B,
-                                                     const uint32_t* __restrict__ hscan, uint32_t nchunks)
+__global__ __launch_bounds__(kBigWG) void k_scatter(Plan P, Bufs B, const uint32_t* __restrict__ hscan)
 {
     extern __shared__ __align__(16) unsigned char smem[];
+    const LargeDesc* __restrict__ desc = P.desc;
+    const TileRef* __restrict__ tiles = P.tiles;
+    const uint32_t nchunks = P.hdr->ntiles * kWaves;
     const uint32_t per = gridDim.x / kXcds;      // workgroups per XCD (the grid is a multiple of kXcds)
     const uint32_t xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
     const uint32_t chi = (uint32_t)((uint64_t)nchunks * (xcd + 1) / kXcds);
@@ -625,7 +774,7 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
     uint32_t st[kPer], sz[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
-        const int b = threadIdx.x * kPer + j;
+        const int b = tid_x() * kPer + j;
         st[j] = 0;
         sz[j] = 0;
         if (b < nbkt) {
@@ -646,7 +795,7 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
     uint32_t o_r = block_excl_scan<kWaves>(c_r, &tr, s_tmp);
 #pragma unroll
     for (int c = 0; c < kBlkClasses; ++c) o_b[c] = block_excl_scan<kWaves>(c_b[c], &tb[c], s_tmp);
-    if (threadIdx.x == 0) {
+    if (tid_x() == 0) {
         s_tmp[8] = tt ? atomicAdd(&s.cnt[s.c_tiny], tt) : 0;
         s_tmp[9] = tl ? atomicAdd(&s.cnt[s.c_large], tl) : 0;
         s_tmp[10] = tr ? atomicAdd(&s.cnt[s.c_run], tr) : 0;
@@ -661,7 +810,7 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
     for (int c = 0; c < kBlkClasses; ++c) o_b[c] += s_tmp[11 + c];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
-        const int b = threadIdx.x * kPer + j;
+        const int b = tid_x() * kPer + j;
         const uint32_t m = sz[j];
         if (b >= nbkt || m == 0) continue;
         Item it;
@@ -687,12 +836,16 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
     }
 }
 
-__global__ __launch_bounds__(kWG) void k_children(const LargeDesc* desc, const uint32_t* hscan, Sink s)
+__global__ __launch_bounds__(kWG) void k_children(Plan P, const uint32_t* hscan, Sink s)
 {
     __shared__ uint32_t s_tmp[16];
-    const LargeDesc D = desc[blockIdx.x];
-    const uint32_t* h = hscan + D.hist_base;
-    emit_children(D.seg, (2 << D.k) - 1, h, D.T * kWaves, h[0], s, s_tmp);
+    const uint32_t nlarge = P.hdr->nlarge;
+    for (uint32_t si = blockIdx.x; si < nlarge; si += gridDim.x) {
+        const LargeDesc D = P.desc[si];
+        const uint32_t* h = hscan + D.hist_base;
+        emit_children(D.seg, (2 << D.k) - 1, h, D.T * kWaves, h[0], s, s_tmp);
+        __syncthreads();     // s_tmp is reused by the next segment
+    }
 }
 
 // ---- leaves --------------------------------------------------------------------
@@ -854,34 +1007,35 @@ __device__ void wave_finish_leaf(const Bufs& B, uint32_t pos, uint32_t m, key64 
 }
 
 // One wave per tiny leaf (at most 64 strings), finished completely.
-__global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint32_t* cnt_items, Bufs B,
-                                                   uint32_t depth)
+__global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint32_t* cnt_items, uint32_t cap,
+                                                   Bufs B, uint32_t depth)
 {
     __shared__ key64 s_k[kWaves][64];
     __shared__ uint32_t s_i[kWaves][64];
     __shared__ uint32_t s_r[kWaves][64];     // first sorted position of the run holding a position
     const uint32_t w = wave_id(), lane = lane_id();
-    const uint32_t it = blockIdx.x * kWaves + w;
-    const bool valid = it < *cnt_items;
-    Item L = { 0u, 0u, 0u, 0u };
-    if (valid) L = items[it];
-    L.begin = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.begin);
-    L.size = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.size);
-    L.flags = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.flags);
-    if (valid && (!item_ok(L, B) || L.size > kTinyT)) {
-        if (lane == 0) flag_error(B.cnt, ERR_BOUNDS);
-        L.size = 0;
+    if (B.pending && *B.pending) return;      // the round's levels are not finished
+    const uint32_t nitems = min(*cnt_items, cap);     // (an overflow has been flagged)
+    // each wave takes leaves it, it + stride, ... (no workgroup barriers below)
+    for (uint32_t it = blockIdx.x * kWaves + w; it < nitems; it += gridDim.x * kWaves) {
+        Item L = items[it];
+        L.begin = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.begin);
+        L.size = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.size);
+        L.flags = (uint32_t)__builtin_amdgcn_readfirstlane((int)L.flags);
+        if (!item_ok(L, B) || L.size > kTinyT) {
+            if (lane == 0) flag_error(B.cnt, ERR_BOUNDS);
+            continue;      // (wave-uniform)
+        }
+        const uint32_t m = L.size;
+        const int src = L.flags & F_BUFB;
+        key64 k = ~0ull;
+        uint32_t id = 0;
+        if (lane < m) {
+            k = src ? B.key1[L.begin + lane] : B.key0[L.begin + lane];
+            id = (L.flags & F_IDENT) ? L.begin + lane : src ? B.idx1[L.begin + lane] : B.idx0[L.begin + lane];
+        }
+        wave_finish_leaf(B, L.begin + lane, m, k, id, depth, true, 0u, s_k[w], s_i[w], s_r[w]);
     }
-    const uint32_t m = L.size;
-    if (m == 0) return;      // (wave-uniform)
-    const int src = L.flags & F_BUFB;
-    key64 k = ~0ull;
-    uint32_t id = 0;
-    if (lane < m) {
-        k = src ? B.key1[L.begin + lane] : B.key0[L.begin + lane];
-        id = (L.flags & F_IDENT) ? L.begin + lane : src ? B.idx1[L.begin + lane] : B.idx0[L.begin + lane];
-    }
-    wave_finish_leaf(B, L.begin + lane, m, k, id, depth, true, 0u, s_k[w], s_i[w], s_r[w]);
 }
 
 // Sorting network over E (key, position) pairs in one thread's registers
@@ -921,7 +1075,7 @@ constexpr int kBlkE0 = 4, kBlkE1 = 4, kBlkE2 = 4, kBlkE3 = 4;
 __device__ __forceinline__ void block_finish_runs(const Bufs& B, key64* s_k, const uint32_t* s_runs,
                                                   uint32_t nruns, uint32_t beg, uint32_t depth, uint32_t nwaves)
 {
-    const uint32_t tid = threadIdx.x, w = wave_id(), lane = lane_id();
+    const uint32_t tid = tid_x(), w = wave_id(), lane = lane_id();
     // the sorted block is no longer needed: each wave's leaf arrays live in
     // s_k, and after them the run batches
     uint32_t* s_rslot = reinterpret_cast<uint32_t*>(s_k + nwaves * 128);      // first slot of a run in its batch
@@ -984,7 +1138,7 @@ __device__ __forceinline__ void block_merge_sort(uint32_t m, key64* s_k, uint16_
 {
     constexpr uint32_t T = MAXM / E;
     (void)T;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = tid_x();
     const uint32_t Ta = (m + E - 1) / E, Pm = Ta * E;   // threads in use, padded size
     key64 k[E];
     uint32_t p[E];
@@ -1104,7 +1258,7 @@ __device__ __forceinline__ bool block_radix_sort(const key64* __restrict__ K, ui
     constexpr int NW = (int)(T / 64);
     constexpr int D = __builtin_ctz(MAXM);      // digit bits: MAXM groups
     static_assert((MAXM & (MAXM - 1)) == 0 && MAXM / T == (uint32_t)E, "a power-of-two block, E counters per thread");
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = tid_x();
     const uint32_t Pm = (m + E - 1) / E * E;
     // w: this thread's keys (positions e*T + tid), later their packed values
     // every load is unconditional (clamped index), so all E are in flight at once
@@ -1249,6 +1403,18 @@ __device__ __forceinline__ bool block_radix_sort(const key64* __restrict__ K, ui
     return true;
 }
 
+// The block sort's LDS (one workgroup).
+template <uint32_t MAXM, int E>
+struct BlockLds {
+    static constexpr uint32_t T = MAXM / E;
+    key64 k[MAXM + 1];     // + 1: a merge step reads the slot after the one it takes
+    uint16_t p[MAXM];
+    uint32_t runs[kBlockRuns];        // (start << 16) | length
+    uint32_t nruns;
+    uint32_t cnt[MAXM + 2];           // radix partition: group counts, then starts
+    alignas(8) uint32_t tmp[4 * (T / 64) + 4];
+};
+
 // One workgroup of MAXM / E threads sorts a block of 65..MAXM strings
 // completely on chip and finishes every position at this depth (the
 // reference's insertion-sort base case, lcp_inssort.hpp:36-187, and the
@@ -1259,28 +1425,25 @@ __device__ __forceinline__ bool block_radix_sort(const key64* __restrict__ K, ui
 // LCP to its neighbour, identical strings done, runs of equal keys without a
 // NUL handed to round r+1.
 template <uint32_t MAXM, int E>
-__global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t first, const uint32_t* cnt_items,
-                                                     Bufs B, Sink nxt, uint32_t depth)
+__device__ __forceinline__ void block_one(const Item L, const Bufs& B, const Sink& nxt, uint32_t depth,
+                                          BlockLds<MAXM, E>& S)
 {
     constexpr uint32_t T = MAXM / E;
     static_assert(T % 64 == 0 && MAXM <= 65536, "whole waves; positions fit 16 bits");
-    __shared__ key64 s_k[MAXM + 1];     // + 1: a merge step reads the slot after the one it takes
-    __shared__ uint16_t s_p[MAXM];
     constexpr uint32_t kMaxRuns = kBlockRuns;    // short equal runs finished here (more go to the next round)
-    __shared__ uint32_t s_runs[kMaxRuns];        // (start << 16) | length
-    __shared__ uint32_t s_nruns;
-    __shared__ uint32_t s_cnt[MAXM + 2];         // radix partition: group counts, then starts
-    __shared__ __align__(8) uint32_t s_tmp[4 * (T / 64) + 4];
     static_assert(MAXM * 8 >= 1024 * (T / 64) + 4 * (2 * kMaxRuns + 2),
                   "the per-wave leaf arrays (1 KB each) and the run batches reuse s_k");
-    const uint32_t idx = first + blockIdx.x;
-    if (idx >= *cnt_items) return;
-    const Item L = items[idx];
+    key64* s_k = S.k;
+    uint16_t* s_p = S.p;
+    uint32_t* s_runs = S.runs;
+    uint32_t& s_nruns = S.nruns;
+    uint32_t* s_cnt = S.cnt;
+    uint32_t* s_tmp = S.tmp;
     if (!item_ok(L, B) || L.size > MAXM) {
-        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
+        if (tid_x() == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
-    const uint32_t m = L.size, beg = L.begin, tid = threadIdx.x;
+    const uint32_t m = L.size, beg = L.begin, tid = tid_x();
     const int src = L.flags & F_BUFB;
     const key64* K = src ? B.key1 : B.key0;
     const uint32_t* I = src ? B.idx1 : B.idx0;
@@ -1339,49 +1502,75 @@ __global__ __launch_bounds__(MAXM / E) void k_block(const Item* items, uint32_t 
     }
     __syncthreads();     // the runs' keys and indices written above are visible to the whole group
     const uint32_t nruns = min(s_nruns, kMaxRuns);
-    if (nruns == 0) return;     // (uniform)
-    block_finish_runs(B, s_k, s_runs, nruns, beg, depth, T / 64);
+    if (nruns != 0) block_finish_runs(B, s_k, s_runs, nruns, beg, depth, T / 64);     // (uniform)
+}
+
+// Blocks [first, last) of a list (one per workgroup), or with LOOP (levels
+// planned on the device) from *p_first to the list's count, each workgroup
+// looping over them; a count over the list's capacity has been flagged and
+// is clamped.
+template <uint32_t MAXM, int E, bool LOOP>
+__global__ __launch_bounds__(MAXM / E) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_block(const Item* items, uint32_t first, uint32_t last,
+                                                     const uint32_t* p_first, const uint32_t* cnt_items, uint32_t cap,
+                                                     Bufs B, Sink nxt, uint32_t depth)
+{
+    __shared__ BlockLds<MAXM, E> S;
+    if (!LOOP) {
+        const uint32_t idx = first + blockIdx.x;
+        if (idx >= min(last, cap)) return;
+        block_one<MAXM, E>(items[idx], B, nxt, depth, S);
+        return;
+    }
+    const uint32_t lo = *p_first, hi = min(*cnt_items, cap);
+    for (uint32_t idx = lo + blockIdx.x; idx < hi; idx += gridDim.x) {
+        __syncthreads();     // the previous block's LDS is no longer read
+        block_one<MAXM, E>(items[idx], B, nxt, depth, S);
+    }
 }
 
 // One workgroup per chunk of an equal-key leaf (all keys equal).
-__global__ __launch_bounds__(kWG) void k_leaf_run(const Item* items, const uint32_t* cnt_items, Bufs B,
-                                                  Sink nxt, uint32_t depth)
+__global__ __launch_bounds__(kWG) void k_leaf_run(const Item* items, const uint32_t* cnt_items, uint32_t cap,
+                                                  Bufs B, Sink nxt, uint32_t depth)
 {
-    if (blockIdx.x >= *cnt_items) return;
-    const Item L = items[blockIdx.x];
-    const uint32_t c = L.flags >> kFlagShift;
-    if (!item_ok(L, B) || (uint64_t)c * kRunChunk >= L.size) {
-        if (threadIdx.x == 0) flag_error(B.cnt, ERR_BOUNDS);
-        return;
-    }
-    const int src = L.flags & F_BUFB;
-    const key64* K = src ? B.key1 : B.key0;
-    const uint32_t* I = src ? B.idx1 : B.idx0;
-    const uint32_t rend = L.begin + L.size;
-    const uint32_t cb = L.begin + c * kRunChunk;
-    const uint32_t ce = min(rend, cb + kRunChunk);
-    const key64 key = K[cb];    // this chunk's own copy of the run key
-    __syncthreads();
-    const bool term = (key & 0xFFull) == 0;
-    const bool single = L.size == 1;
-    for (uint32_t p = cb + threadIdx.x; p < ce; p += kWG) {
-        const uint32_t id = (L.flags & F_IDENT) ? p : I[p];
-        if (single || term) {
-            B.perm[p] = id;
-            if (p > L.begin && B.lcp) B.lcp[p] = depth + key_depth(key);
-        } else {
-            if (src || (L.flags & F_IDENT)) B.idx0[p] = id;
-            if (id < B.n) B.key0[p] = load_key(B.arena, B.alen, str_off(B, id) + depth + 8);
-            else flag_error(B.cnt, ERR_BOUNDS);
+    if (B.pending && *B.pending) return;      // the round's levels are not finished
+    const uint32_t nitems = min(*cnt_items, cap);     // (an overflow has been flagged)
+    for (uint32_t ii = blockIdx.x; ii < nitems; ii += gridDim.x) {
+        __syncthreads();     // the previous chunk's key read is complete
+        const Item L = items[ii];
+        const uint32_t c = L.flags >> kFlagShift;
+        if (!item_ok(L, B) || (uint64_t)c * kRunChunk >= L.size) {
+            if (tid_x() == 0) flag_error(B.cnt, ERR_BOUNDS);
+            continue;
         }
-    }
-    if (threadIdx.x < 64) {
-        const bool nr = threadIdx.x == 0 && c == 0 && !term && !single;
-        Item ni = { L.begin, L.size, L.begin, 0u };
-        wave_append_segment(nr, ni, nxt);
-        if (threadIdx.x == 0) {
-            if (c == 0) B.bkey[L.begin] = key;
-            if (ce == rend) B.bkey[rend - 1] = key;
+        const int src = L.flags & F_BUFB;
+        const key64* K = src ? B.key1 : B.key0;
+        const uint32_t* I = src ? B.idx1 : B.idx0;
+        const uint32_t rend = L.begin + L.size;
+        const uint32_t cb = L.begin + c * kRunChunk;
+        const uint32_t ce = min(rend, cb + kRunChunk);
+        const key64 key = K[cb];    // this chunk's own copy of the run key
+        __syncthreads();
+        const bool term = (key & 0xFFull) == 0;
+        const bool single = L.size == 1;
+        for (uint32_t p = cb + tid_x(); p < ce; p += kWG) {
+            const uint32_t id = (L.flags & F_IDENT) ? p : I[p];
+            if (single || term) {
+                B.perm[p] = id;
+                if (p > L.begin && B.lcp) B.lcp[p] = depth + key_depth(key);
+            } else {
+                if (src || (L.flags & F_IDENT)) B.idx0[p] = id;
+                if (id < B.n) B.key0[p] = load_key(B.arena, B.alen, str_off(B, id) + depth + 8);
+                else flag_error(B.cnt, ERR_BOUNDS);
+            }
+        }
+        if (tid_x() < 64) {
+            const bool nr = tid_x() == 0 && c == 0 && !term && !single;
+            Item ni = { L.begin, L.size, L.begin, 0u };
+            wave_append_segment(nr, ni, nxt);
+            if (tid_x() == 0) {
+                if (c == 0) B.bkey[L.begin] = key;
+                if (ce == rend) B.bkey[rend - 1] = key;
+            }
         }
     }
 }
@@ -1391,24 +1580,30 @@ __global__ __launch_bounds__(kWG) void k_leaf_run(const Item* items, const uint3
 // (sample_sort_lcp, .hpp:311-396).
 struct LeafLists {        // the leaf lists of a round, concatenated in this order
     const Item* list[kBlkClasses + 2];   // tiny, blocks by class, equal-key chunks
-    uint32_t end[kBlkClasses + 2];       // running totals of the list lengths
+    int cnt[kBlkClasses + 2];            // their counters (indices into the counter array)
+    uint32_t cap[kBlkClasses + 2];       // their capacities (an overflow has been flagged)
 };
 
 __global__ __launch_bounds__(kWG) void k_leafbound(LeafLists LL, Bufs B, uint32_t depth)
 {
     constexpr int NL = kBlkClasses + 2;
-    const uint32_t i = blockIdx.x * kWG + threadIdx.x;
-    if (i >= LL.end[NL - 1]) return;
-    int l = 0;
+    if (B.pending && *B.pending) return;      // the round's levels are not finished
+    uint32_t end[NL];        // running totals of the list lengths
+    uint32_t run = 0;
 #pragma unroll
-    for (int j = 0; j < NL - 1; ++j) l += i >= LL.end[j] ? 1 : 0;
-    const Item L = LL.list[l][i - (l ? LL.end[l - 1] : 0u)];
-    if (l == NL - 1 && (L.flags >> kFlagShift) != 0) return;   // only the first chunk of an equal-key leaf
-    if (L.begin <= L.rbeg || !B.lcp) return;
-    if (L.begin >= B.n) { flag_error(B.cnt, ERR_BOUNDS); return; }
-    const key64 a = B.bkey[L.begin - 1], c = B.bkey[L.begin];
-    if (a == c) { flag_error(B.cnt, ERR_LCP); return; }
-    B.lcp[L.begin] = depth + lcp_key(a, c);
+    for (int j = 0; j < NL; ++j) end[j] = run += min(B.cnt[LL.cnt[j]], LL.cap[j]);
+    for (uint32_t i = blockIdx.x * kWG + tid_x(); i < end[NL - 1]; i += gridDim.x * kWG) {
+        int l = 0;
+#pragma unroll
+        for (int j = 0; j < NL - 1; ++j) l += i >= end[j] ? 1 : 0;
+        const Item L = LL.list[l][i - (l ? end[l - 1] : 0u)];
+        if (l == NL - 1 && (L.flags >> kFlagShift) != 0) continue;   // only the first chunk of an equal-key leaf
+        if (L.begin <= L.rbeg || !B.lcp) continue;
+        if (L.begin >= B.n) { flag_error(B.cnt, ERR_BOUNDS); continue; }
+        const key64 a = B.bkey[L.begin - 1], c = B.bkey[L.begin];
+        if (a == c) { flag_error(B.cnt, ERR_LCP); continue; }
+        B.lcp[L.begin] = depth + lcp_key(a, c);
+    }
 }
 
 // Batch outputs: input index -> index inside its set (perm[pos] - first of
@@ -1416,7 +1611,7 @@ __global__ __launch_bounds__(kWG) void k_leafbound(LeafLists LL, Bufs B, uint32_
 __global__ __launch_bounds__(kWG) void k_localize(uint32_t* perm, uint32_t n, const uint64_t* set_first,
                                                    uint32_t nsets)
 {
-    const uint32_t pos = blockIdx.x * kWG + threadIdx.x;
+    const uint32_t pos = blockIdx.x * kWG + tid_x();
     if (pos >= n) return;
     uint32_t lo = 0, hi = nsets;   // largest s with set_first[s] <= pos
     while (lo + 1 < hi) {
@@ -1437,7 +1632,7 @@ __global__ __launch_bounds__(kWG) void k_dist_chars(const uint8_t* arena, const 
                                                     const uint32_t* perm, const uint32_t* lcp, uint32_t n,
                                                     uint8_t* cache, uint32_t* cnt, uint64_t alen)
 {
-    const uint32_t i = blockIdx.x * kWG + threadIdx.x;
+    const uint32_t i = blockIdx.x * kWG + tid_x();
     if (i == 0 || i >= n) return;
     const uint64_t p = (uint64_t)offs[perm[i]] + lcp[i];
     if (p >= alen) { flag_error(cnt, ERR_BOUNDS); return; }
@@ -1448,14 +1643,14 @@ __global__ __launch_bounds__(kWG) void k_dist_chars(const uint8_t* arena, const 
 __global__ __launch_bounds__(kWG) void k_gather_ptrs(const uint32_t* perm, const uint32_t* offs, uint64_t base,
                                                      uint64_t n, uint64_t* ptrs)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * kWG + tid_x();
     if (i < n) ptrs[i] = base + offs[perm[i] < n ? perm[i] : 0u];
 }
 
 // wide[i] = lcp[i] as uint64 (entry 0 is copied too; callers skip it)
 __global__ __launch_bounds__(kWG) void k_widen_lcp(const uint32_t* lcp, uint64_t n, uint64_t* wide)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * kWG + tid_x();
     if (i < n) wide[i] = lcp[i];
 }
 
@@ -1463,7 +1658,7 @@ __global__ __launch_bounds__(kWG) void k_widen_lcp(const uint32_t* lcp, uint64_t
 __global__ __launch_bounds__(kWG) void k_widen(const uint32_t* offs32, const uint64_t* first, const uint64_t* base,
                                                uint32_t nsets, uint64_t n, uint64_t* offs64)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x;
+    const uint64_t i = (uint64_t)blockIdx.x * kWG + tid_x();
     if (i >= n) return;
     uint32_t lo = 0, hi = nsets;   // largest s with first[s] <= i
     while (lo + 1 < hi) {
@@ -1560,7 +1755,7 @@ __global__ __launch_bounds__(kWG) void k_merge_part(Bufs B, const uint32_t* src_
                                                     const uint64_t* first, const uint64_t* mid,
                                                     const uint64_t* tstart, uint32_t npairs, uint32_t* co)
 {
-    const uint64_t g = (uint64_t)blockIdx.x * kWG + threadIdx.x;
+    const uint64_t g = (uint64_t)blockIdx.x * kWG + tid_x();
     if (g >= tstart[npairs]) return;
     const uint32_t p = merge_pair(tstart, npairs, g);
     const uint64_t a0 = first[p], b0 = mid[p];
@@ -1603,7 +1798,7 @@ __global__ __launch_bounds__(kWG) void k_merge_tile(Bufs B, const uint32_t* src_
     const uint64_t jb0 = diag0 - ia0;
     const uint32_t na = (uint32_t)(ia1 - ia0), nb = len - na;
     const uint32_t sb = na + 1;        // first slot of B
-    for (uint32_t k = threadIdx.x; k < len + 2; k += kWG) {
+    for (uint32_t k = tid_x(); k < len + 2; k += kWG) {
         uint64_t pos;
         bool valid;
         if (k < sb) {
@@ -1620,7 +1815,7 @@ __global__ __launch_bounds__(kWG) void k_merge_tile(Bufs B, const uint32_t* src_
         }
     }
     __syncthreads();
-    const uint32_t d = threadIdx.x * kMergeE;
+    const uint32_t d = tid_x() * kMergeE;
     if (d < len) {
         auto leq = [&](uint32_t x, uint32_t y, uint32_t* l) {
             return merge_leq(B, s_idx[x], s_idx[y], s_key[x], s_key[y], l);
@@ -1682,7 +1877,7 @@ __global__ __launch_bounds__(kWG) void k_merge_tile(Bufs B, const uint32_t* src_
     }
     __syncthreads();
     const uint64_t o0 = a0 + diag0;
-    for (uint32_t k = threadIdx.x; k < len; k += kWG) {
+    for (uint32_t k = tid_x(); k < len; k += kWG) {
         const uint32_t s = s_oslot[k];
         dst_idx[o0 + k] = s_idx[s];
         dst_lcp[o0 + k] = s_olcp[k];
@@ -1693,7 +1888,7 @@ __global__ __launch_bounds__(kWG) void k_merge_tile(Bufs B, const uint32_t* src_
 // plain copy of 16-byte words (calibrates the HBM byte counters)
 __global__ __launch_bounds__(kWG) void k_copy16(const uint4* src, uint4* dst, uint64_t n16)
 {
-    for (uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * kWG)
+    for (uint64_t i = (uint64_t)blockIdx.x * kWG + tid_x(); i < n16; i += (uint64_t)gridDim.x * kWG)
         dst[i] = src[i];
 }
 
@@ -1704,7 +1899,7 @@ __global__ __launch_bounds__(kWG) void k_sum_sizes(const Item* items, uint32_t f
     __shared__ uint32_t s_tmp[16];
     const uint32_t n = *cnt_items;
     uint32_t v = 0;
-    for (uint32_t i = first + blockIdx.x * kWG + threadIdx.x; i < n; i += gridDim.x * kWG) {
+    for (uint32_t i = first + blockIdx.x * kWG + tid_x(); i < n; i += gridDim.x * kWG) {
         const Item it = items[i];
         if (it.flags & F_SORTED) {   // a chunk of an equal-key leaf
             const uint32_t c = it.flags >> kFlagShift, cb = c * kRunChunk;
@@ -1715,7 +1910,7 @@ __global__ __launch_bounds__(kWG) void k_sum_sizes(const Item* items, uint32_t f
     }
     uint32_t tot;
     block_excl_scan<kWaves>(v, &tot, s_tmp);
-    if (threadIdx.x == 0 && tot) atomicAdd(out, (unsigned long long)tot);
+    if (tid_x() == 0 && tot) atomicAdd(out, (unsigned long long)tot);
 }
 
 // ===========================================================================
@@ -1777,9 +1972,9 @@ struct WorkBuffers {
     Item* h_large = nullptr;         // pinned copy of a large list
     // plan of a level: its nlarge descriptors followed by its tile table, in
     // one buffer so that one copy uploads both
-    LargeDesc* d_desc = nullptr;
-    LargeDesc* h_desc = nullptr;     // pinned, mapped: the plan is pulled by k_fetch
-    LargeDesc* h_desc_dev = nullptr; // its device address
+    unsigned char* d_plan = nullptr;
+    unsigned char* h_plan = nullptr;     // pinned, mapped: the host's plans are pulled by k_fetch
+    unsigned char* h_plan_dev = nullptr; // its device address
     uint64_t tiles_cap = 0;
     key64* trees = nullptr;          // kTreeSlots splitter slots per large segment
     uint64_t trees_cap = 0;
@@ -1798,9 +1993,21 @@ constexpr uint32_t cap_tiny_of(uint64_t cap) { return (uint32_t)(cap + 16); }
 constexpr uint32_t cap_run_of(uint64_t cap) { return (uint32_t)(cap / 64 + 16); }
 // every large segment has at most size/(kWaves * kChunkMin) + 1 tiles
 constexpr uint64_t tiles_cap_of(uint64_t cap) { return cap / ((uint64_t)kWaves * kChunkMin) + cap_large_of(cap) + 1; }
-constexpr uint64_t plan_bytes_of(uint64_t cap)
+// a plan: the level header, cap_large descriptors, the tile table
+constexpr uint64_t plan_tiles_off(uint64_t cap)
 {
-    return (uint64_t)cap_large_of(cap) * sizeof(LargeDesc) + tiles_cap_of(cap) * sizeof(TileRef);
+    return sizeof(LevelHdr) + (uint64_t)cap_large_of(cap) * sizeof(LargeDesc);
+}
+constexpr uint64_t plan_bytes_of(uint64_t cap) { return plan_tiles_off(cap) + tiles_cap_of(cap) * sizeof(TileRef); }
+static_assert(sizeof(LevelHdr) % 8 == 0 && sizeof(LargeDesc) % 8 == 0 && sizeof(TileRef) % 8 == 0,
+              "a plan is copied in 8-byte words");
+Plan plan_view(const unsigned char* base, uint64_t cap)
+{
+    Plan P;
+    P.hdr = reinterpret_cast<const LevelHdr*>(base);
+    P.desc = reinterpret_cast<const LargeDesc*>(base + sizeof(LevelHdr));
+    P.tiles = reinterpret_cast<const TileRef*>(base + plan_tiles_off(cap));
+    return P;
 }
 
 }  // namespace
@@ -1815,8 +2022,8 @@ struct ps5_ctx {
     // scratch's largest sort (allocated with the context), and the scratch's
     // device address on this context's device
     Item* na_h_large = nullptr;
-    LargeDesc* na_h_desc = nullptr;
-    LargeDesc* na_h_desc_dev = nullptr;
+    unsigned char* na_h_plan = nullptr;
+    unsigned char* na_h_plan_dev = nullptr;
     unsigned char* na_scratch = nullptr;
     uint32_t* d_cnt = nullptr;
     uint32_t* h_cnt = nullptr;       // pinned
@@ -1839,6 +2046,7 @@ struct ps5_ctx {
     uint32_t max_treebits = kMaxTreebits;   // splitter-tree depth limit of one level
     uint32_t max_oversample = 32;    // sample keys per splitter (limited by kMaxSamples)
     uint64_t seed = 0x5EED5EEDull;
+    bool dev_levels = false;         // plan levels on the device: no host synchronisation inside a round
     // profiling
     bool profile = false;
     bool trace = false;              // PS5_TRACE set: per-round work to stderr (diagnostics)
@@ -1865,12 +2073,12 @@ void free_buffers(ps5_ctx* c)
         c->w.key[i] = nullptr; c->w.idx[i] = nullptr; c->w.l_round[i] = nullptr; c->w.l_lvl[i] = nullptr;
         c->w.l_tiny[i] = nullptr; c->w.l_block[i] = nullptr; c->w.l_run[i] = nullptr;
     }
-    f(c->w.bkey); f(c->w.bktrank); f(c->w.hist); f(c->w.part); f(c->w.trees); f(c->w.d_desc);
+    f(c->w.bkey); f(c->w.bktrank); f(c->w.hist); f(c->w.part); f(c->w.trees); f(c->w.d_plan);
     c->w.bkey = nullptr; c->w.bktrank = nullptr; c->w.hist = nullptr; c->w.part = nullptr; c->w.trees = nullptr;
-    c->w.d_desc = nullptr;
+    c->w.d_plan = nullptr;
     if (c->w.h_large) (void)hipHostFree(c->w.h_large);
-    if (c->w.h_desc) (void)hipHostFree(c->w.h_desc);
-    c->w.h_large = nullptr; c->w.h_desc = nullptr; c->w.h_desc_dev = nullptr;
+    if (c->w.h_plan) (void)hipHostFree(c->w.h_plan);
+    c->w.h_large = nullptr; c->w.h_plan = nullptr; c->w.h_plan_dev = nullptr;
     c->w.cap = 0;
     c->w.hist_cap = 0;
     c->w.part_cap = 0;
@@ -1940,12 +2148,10 @@ int ensure_capacity(ps5_ctx* c, uint64_t n)
     // the splitter trees and histograms of a level grow on demand (grow_level)
     c->w.tiles_cap = tiles_cap_of(cap);
     const uint64_t plan_bytes = plan_bytes_of(cap);
-    static_assert(sizeof(LargeDesc) % alignof(TileRef) == 0, "tile table must stay aligned after the descriptors");
-    static_assert(sizeof(LargeDesc) % 8 == 0 && sizeof(TileRef) % 8 == 0, "the plan is fetched in 8-byte words");
-    HIPCHK(hipMalloc(&c->w.d_desc, plan_bytes));
+    HIPCHK(hipMalloc(&c->w.d_plan, plan_bytes));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->w.h_large), (uint64_t)c->w.cap_large * sizeof(Item)));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->w.h_desc), plan_bytes, hipHostMallocMapped | hipHostMallocCoherent));
-    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->w.h_desc_dev), c->w.h_desc, 0));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->w.h_plan), plan_bytes, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->w.h_plan_dev), c->w.h_plan, 0));
     c->w.cap = cap;
     return PS5_OK;
 }
@@ -2036,7 +2242,7 @@ std::mutex& scratch_mutex(int device)
 // Work buffers of a sort of up to kNoAllocMaxStrings strings carved from the
 // scratch at `base` (its device address), with the context's pinned plan
 // buffers.  Nothing is allocated.
-WorkBuffers scratch_buffers(unsigned char* base, Item* h_large, LargeDesc* h_desc, LargeDesc* h_desc_dev)
+WorkBuffers scratch_buffers(unsigned char* base, Item* h_large, unsigned char* h_plan, unsigned char* h_plan_dev)
 {
     constexpr ScratchLayout L = scratch_layout(kNoAllocMaxStrings);
     const uint64_t cap = kNoAllocMaxStrings;
@@ -2059,9 +2265,9 @@ WorkBuffers scratch_buffers(unsigned char* base, Item* h_large, LargeDesc* h_des
     w.cap_tiny = cap_tiny_of(cap);
     w.cap_run = cap_run_of(cap);
     w.h_large = h_large;
-    w.d_desc = reinterpret_cast<LargeDesc*>(base + L.desc);
-    w.h_desc = h_desc;
-    w.h_desc_dev = h_desc_dev;
+    w.d_plan = base + L.desc;
+    w.h_plan = h_plan;
+    w.h_plan_dev = h_plan_dev;
     w.tiles_cap = tiles_cap_of(cap);
     w.trees = reinterpret_cast<key64*>(base + L.trees);
     w.trees_cap = L.trees_cap;
@@ -2181,15 +2387,39 @@ int large_treebits(uint32_t m, uint32_t target, int maxbits)
     return std::max(1, std::min(maxbits, k));
 }
 
-// One multi-workgroup sample-sort step over all large segments of a level.
+// k_count instance for bucket ids of NB = kmax + 1 bits (at least 8)
+void launch_count(ps5_ctx* c, const Bufs& B, Plan P, dim3 g, int kmax, bool from_arena, bool loop)
+{
+    hipStream_t st = c->stream;
+    const dim3 b(kWG);
+    const uint32_t lds = count_lds_bytes(kmax);
+#define PS5_COUNT_LAUNCH(NB)                                                                                          \
+    do {                                                                                                              \
+        if (loop)                                                                                                     \
+            hipLaunchKernelGGL((k_count<NB, true>), g, b, lds, st, P, B, c->w.trees, c->w.hist, (int)from_arena,      \
+                               kmax);                                                                                 \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_count<NB, false>), g, b, lds, st, P, B, c->w.trees, c->w.hist, (int)from_arena,     \
+                               kmax);                                                                                 \
+    } while (0)
+    if (kmax <= 7) PS5_COUNT_LAUNCH(8);
+    else if (kmax == 8) PS5_COUNT_LAUNCH(9);
+    else if (kmax == 9) PS5_COUNT_LAUNCH(10);
+    else PS5_COUNT_LAUNCH(11);
+#undef PS5_COUNT_LAUNCH
+}
+
+// One multi-workgroup sample-sort step over all large segments of a level,
+// planned on the host from the level's large list (read back into h_large).
 int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uint64_t seed, bool from_arena)
 {
     hipStream_t st = c->stream;
     uint64_t ntiles = 0, hl = 0, elems = 0, elems_ident = 0;
     int kmax = 1;
     if (nlarge > c->w.cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
-    TileRef* h_tiles = reinterpret_cast<TileRef*>(c->w.h_desc + nlarge);
-    const TileRef* d_tiles = reinterpret_cast<const TileRef*>(c->w.d_desc + nlarge);
+    LevelHdr* h_hdr = reinterpret_cast<LevelHdr*>(c->w.h_plan);
+    LargeDesc* h_descs = reinterpret_cast<LargeDesc*>(c->w.h_plan + sizeof(LevelHdr));
+    TileRef* h_tiles = reinterpret_cast<TileRef*>(c->w.h_plan + plan_tiles_off(c->w.cap));
     uint64_t total = 0;
     bool all_ident = true;     // a level of roots: the scatter stages positions, chunks may be longer
     for (uint32_t i = 0; i < nlarge; ++i) {
@@ -2201,7 +2431,7 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     for (uint32_t i = 0; i < nlarge; ++i) {
         const Item seg = c->w.h_large[i];
         const uint32_t T = div_up(seg.size, climit * kWaves);
-        LargeDesc& D = c->w.h_desc[i];
+        LargeDesc& D = h_descs[i];
         D.seg = seg;
         D.T = T;
         D.chunk = div_up(seg.size, (uint64_t)T * kWaves);   // <= climit: equal chunks, the last one short
@@ -2219,38 +2449,34 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         if (seg.flags & F_IDENT) elems_ident += seg.size;
         if (hl > 0xFFFFFFFFull) return set_err(PS5_ERR_CAPACITY, "histogram capacity");
     }
+    *h_hdr = LevelHdr{ nlarge, (uint32_t)ntiles, (uint32_t)hl, (uint32_t)kmax, { 0u, 0u, 0u, 0u } };
     if (int rc = grow_level(c, nlarge, hl)) return rc;
     {   // the plan: pulled from mapped host memory by a kernel (no copy engine)
-        const uint64_t words = (nlarge * sizeof(LargeDesc) + ntiles * sizeof(TileRef)) / 8;
-        hipLaunchKernelGGL(k_fetch, dim3((uint32_t)std::min<uint64_t>(div_up(words, kWG), 1024)), dim3(kWG), 0, st,
-                           reinterpret_cast<const uint64_t*>(c->w.h_desc_dev), reinterpret_cast<uint64_t*>(c->w.d_desc),
-                           words);
+        const uint64_t words = (sizeof(LevelHdr) + nlarge * sizeof(LargeDesc)) / 8, words2 = ntiles * sizeof(TileRef) / 8;
+        const uint64_t toff = plan_tiles_off(c->w.cap);
+        hipLaunchKernelGGL(k_fetch, dim3((uint32_t)std::min<uint64_t>(div_up(std::max(words, words2), kWG), 1024)),
+                           dim3(kWG), 0, st, reinterpret_cast<const uint64_t*>(c->w.h_plan_dev),
+                           reinterpret_cast<uint64_t*>(c->w.d_plan), words,
+                           reinterpret_cast<const uint64_t*>(c->w.h_plan_dev + toff),
+                           reinterpret_cast<uint64_t*>(c->w.d_plan + toff), words2);
     }
+    const Plan P = plan_view(c->w.d_plan, c->w.cap);
     const uint32_t hl32 = (uint32_t)hl;
     {
         Launch L(c, K_TREE, (uint64_t)nlarge * kMaxSamples * 8);
-        hipLaunchKernelGGL(k_tree, dim3(nlarge), dim3(kBigWG), 0, st, c->w.d_desc, B, seed, c->w.trees, (int)from_arena);
+        hipLaunchKernelGGL(k_tree, dim3(nlarge), dim3(kBigWG), 0, st, P, B, seed, c->w.trees, (int)from_arena);
     }
     {
         // key 8 R + bktrank 4 W; extracting at the root: offs 4 R + 8 characters R + key 8 W + bktrank 4 W
         Launch L(c, K_COUNT, (from_arena ? elems * 24 : elems * 12) + hl * 4);
-        const dim3 g((uint32_t)ntiles), b(kWG);
-        const uint32_t lds = count_lds_bytes(kmax);
-        if (kmax <= 7)
-            hipLaunchKernelGGL(k_count<8>, g, b, lds, st, c->w.d_desc, d_tiles, B, c->w.trees, c->w.hist, (int)from_arena, kmax);
-        else if (kmax == 8)
-            hipLaunchKernelGGL(k_count<9>, g, b, lds, st, c->w.d_desc, d_tiles, B, c->w.trees, c->w.hist, (int)from_arena, kmax);
-        else if (kmax == 9)
-            hipLaunchKernelGGL(k_count<10>, g, b, lds, st, c->w.d_desc, d_tiles, B, c->w.trees, c->w.hist, (int)from_arena, kmax);
-        else
-            hipLaunchKernelGGL(k_count<11>, g, b, lds, st, c->w.d_desc, d_tiles, B, c->w.trees, c->w.hist, (int)from_arena, kmax);
+        launch_count(c, B, P, dim3(div_up(ntiles, kXcds) * kXcds), kmax, from_arena, false);
     }
     {
         const uint32_t nb = div_up(hl32, kScanBlock);
         Launch L(c, K_SCAN, hl * 12);
-        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kWG), 0, st, c->w.hist, hl32, c->w.part);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kWG), 0, st, c->w.part, nb);
-        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, c->w.hist, hl32, c->w.part);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kWG), 0, st, c->w.hist, P.hdr, c->w.part);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kWG), 0, st, c->w.part, P.hdr);
+        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, c->w.hist, P.hdr, c->w.part);
     }
     {
         // bktrank 4 + key 8 + idx 4 R (no idx at a root), key 8 + idx 4 W
@@ -2258,15 +2484,13 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         const uint32_t nchunks = (uint32_t)ntiles * kWaves;
         const uint32_t wgs = std::min(c->scatter_wgs, div_up(nchunks, kXcds) * kXcds);
         if (all_ident)
-            hipLaunchKernelGGL(k_scatter<true>, dim3(wgs), dim3(kBigWG), ScatLds<true>::kBytes, st, c->w.d_desc, d_tiles,
-                               B, c->w.hist, nchunks);
+            hipLaunchKernelGGL(k_scatter<true>, dim3(wgs), dim3(kBigWG), ScatLds<true>::kBytes, st, P, B, c->w.hist);
         else
-            hipLaunchKernelGGL(k_scatter<false>, dim3(wgs), dim3(kBigWG), ScatLds<false>::kBytes, st, c->w.d_desc,
-                               d_tiles, B, c->w.hist, nchunks);
+            hipLaunchKernelGGL(k_scatter<false>, dim3(wgs), dim3(kBigWG), ScatLds<false>::kBytes, st, P, B, c->w.hist);
     }
     {
         Launch L(c, K_CHILDREN, hl * 4 / kWaves);
-        hipLaunchKernelGGL(k_children, dim3(nlarge), dim3(kWG), 0, st, c->w.d_desc, c->w.hist, sink);
+        hipLaunchKernelGGL(k_children, dim3(nlarge), dim3(kWG), 0, st, P, c->w.hist, sink);
     }
     c->st_segments += nlarge;
     return PS5_OK;
@@ -2339,6 +2563,249 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
 
 constexpr uint32_t kForkLeafGroups = 8192;   // leaf workgroups worth a side stream
 
+// Largest grids of the device-planned levels: their kernels loop over the
+// work the device lists, so these bound the workgroups of a launch.
+constexpr uint32_t kDevTreeWgs = 512, kDevCountWgs = 2048, kDevScanWgs = 1024, kDevChildWgs = 1024,
+                   kDevTinyWgs = 8192, kDevRunWgs = 2048, kDevBoundWgs = 1024;
+static_assert(kBlkE0 == kBlkE1 && kBlkE1 == kBlkE2 && kBlkE2 == kBlkE3, "one thread count per 256 keys");
+
+// Blocks of size class cls and round parity rp: the list entries [first,
+// first + count) (or, with dev_first, from the device counter C_BDONE + cls
+// to the list's count), `grid` workgroups.
+void launch_block_class(ps5_ctx* c, const Bufs& B, int rp, int cls, uint32_t first, uint32_t count, bool dev_first,
+                        uint32_t grid, const Sink& nsink, uint32_t depth)
+{
+    const uint32_t last = first + count;
+    hipStream_t st = c->stream;
+    const Item* list = c->w.l_block[rp] + (uint64_t)cls * c->w.cap_block;
+    const uint32_t* cnt = c->d_cnt + C_BLOCK + 2 * cls + rp;
+    const uint32_t* pf = dev_first ? c->d_cnt + C_BDONE + cls : nullptr;
+    const uint32_t cap = c->w.cap_block;
+    Launch L(c, K_BLOCK, 0);
+#define PS5_BLOCK_LAUNCH(M, E, LOOP)                                                                              \
+    hipLaunchKernelGGL((k_block<M, E, LOOP>), dim3(grid), dim3(M / E), 0, st, list, first, last, pf, cnt, cap, B, \
+                       nsink, depth)
+    if (dev_first) {
+        if (cls == 0) PS5_BLOCK_LAUNCH(256, kBlkE0, true);
+        else if (cls == 1) PS5_BLOCK_LAUNCH(512, kBlkE1, true);
+        else if (cls == 2) PS5_BLOCK_LAUNCH(1024, kBlkE2, true);
+        else PS5_BLOCK_LAUNCH(kBlkT, kBlkE3, true);
+    } else {
+        if (cls == 0) PS5_BLOCK_LAUNCH(256, kBlkE0, false);
+        else if (cls == 1) PS5_BLOCK_LAUNCH(512, kBlkE1, false);
+        else if (cls == 2) PS5_BLOCK_LAUNCH(1024, kBlkE2, false);
+        else PS5_BLOCK_LAUNCH(kBlkT, kBlkE3, false);
+    }
+#undef PS5_BLOCK_LAUNCH
+}
+
+// Tiny leaves and equal-key chunks of round parity rp on stream s (grids of
+// g_tiny and g_run workgroups; 0 skips a list).
+void launch_leaves(ps5_ctx* c, const Bufs& B, int rp, const Sink& nsink, uint32_t depth, uint32_t g_tiny,
+                   uint32_t g_run, uint64_t n_tiny, uint64_t n_run, hipStream_t s)
+{
+    if (g_tiny) {
+        {
+            Launch L(c, K_LEAF_TINY, 0);
+            hipLaunchKernelGGL(k_leaf_tiny, dim3(g_tiny), dim3(kWG), 0, s, c->w.l_tiny[rp], c->d_cnt + C_TINY + rp,
+                               c->w.cap_tiny, B, depth);
+        }
+        sum_sizes(c, c->w.l_tiny[rp], 0, c->d_cnt + C_TINY + rp, (uint32_t)n_tiny, S_TINY);
+    }
+    if (g_run) {
+        {
+            Launch L(c, K_LEAF_RUN, 0);
+            hipLaunchKernelGGL(k_leaf_run, dim3(g_run), dim3(kWG), 0, s, c->w.l_run[rp], c->d_cnt + C_RUN + rp,
+                               c->w.cap_run, B, nsink, depth);
+        }
+        sum_sizes(c, c->w.l_run[rp], 0, c->d_cnt + C_RUN + rp, (uint32_t)n_run, S_RUN);
+    }
+}
+
+// LCPs between neighbouring leaves of round parity rp (grid workgroups).
+void launch_leafbound(ps5_ctx* c, const Bufs& B, int rp, uint32_t depth, uint32_t grid, uint64_t nleaves)
+{
+    Launch L(c, K_LEAFBOUND, nleaves * 36);
+    LeafLists LL;
+    LL.list[0] = c->w.l_tiny[rp];
+    LL.cnt[0] = C_TINY + rp;
+    LL.cap[0] = c->w.cap_tiny;
+    for (int cls = 0; cls < kBlkClasses; ++cls) {
+        LL.list[1 + cls] = c->w.l_block[rp] + (uint64_t)cls * c->w.cap_block;
+        LL.cnt[1 + cls] = C_BLOCK + 2 * cls + rp;
+        LL.cap[1 + cls] = c->w.cap_block;
+    }
+    LL.list[kBlkClasses + 1] = c->w.l_run[rp];
+    LL.cnt[kBlkClasses + 1] = C_RUN + rp;
+    LL.cap[kBlkClasses + 1] = c->w.cap_run;
+    hipLaunchKernelGGL(k_leafbound, dim3(grid), dim3(kWG), 0, c->stream, LL, B, depth);
+}
+
+// One level planned on the device: k_plan turns the large list `list`
+// (counter ci) into the level's plan, and the step's kernels read their work
+// from it, in grids bounded by what the host knows (at most bound_large
+// segments holding S strings), so nothing comes back to the host.
+void dev_level(ps5_ctx* c, const Bufs& B, const Item* list, int ci, uint32_t bound_large, uint64_t S,
+               const Sink& sink, uint64_t seed, bool from_arena, bool ident, int kmax, int done_rp)
+{
+    hipStream_t st = c->stream;
+    const uint64_t n = S;     // strings the level's segments hold at most
+    PlanArgs A;
+    A.list = list;
+    A.ci = ci;
+    A.cap_large = c->w.cap_large;
+    A.plan = c->w.d_plan;
+    A.tiles_off = plan_tiles_off(c->w.cap);
+    A.tiles_cap = c->w.tiles_cap;
+    A.hist_cap = c->w.hist_cap;
+    A.trees_cap = c->w.trees_cap;
+    A.leaf_target = c->leaf_target;
+    A.max_treebits = c->max_treebits;
+    A.max_oversample = c->max_oversample;
+    A.kmax = (uint32_t)kmax;
+    A.done_rp = done_rp;
+    {
+        Launch L(c, K_MISC, 0);
+        hipLaunchKernelGGL(k_plan, dim3(1), dim3(kBigWG), 0, st, A, c->d_cnt);
+    }
+    const Plan P = plan_view(c->w.d_plan, c->w.cap);
+    const uint32_t gl = std::max<uint32_t>(1, bound_large);
+    {
+        Launch L(c, K_TREE, 0);
+        hipLaunchKernelGGL(k_tree, dim3(std::min(gl, kDevTreeWgs)), dim3(kBigWG), 0, st, P, B, seed, c->w.trees,
+                           (int)from_arena);
+    }
+    const uint64_t tiles_bound = n / ((uint64_t)kWaves * kChunkMin) + gl;
+    {
+        Launch L(c, K_COUNT, 0);
+        launch_count(c, B, P, dim3((uint32_t)std::min<uint64_t>(kDevCountWgs, div_up(tiles_bound, kXcds) * kXcds)),
+                     kmax, from_arena, true);
+    }
+    {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(kDevScanWgs, div_up(c->w.hist_cap, kScanBlock));
+        Launch L(c, K_SCAN, 0);
+        hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kWG), 0, st, c->w.hist, P.hdr, c->w.part);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kWG), 0, st, c->w.part, P.hdr);
+        hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kWG), 0, st, c->w.hist, P.hdr, c->w.part);
+    }
+    {
+        Launch L(c, K_SCATTER, 0);
+        const uint32_t wgs = (uint32_t)std::min<uint64_t>(c->scatter_wgs, div_up(tiles_bound * kWaves, kXcds) * kXcds);
+        if (ident)
+            hipLaunchKernelGGL(k_scatter<true>, dim3(wgs), dim3(kBigWG), ScatLds<true>::kBytes, st, P, B, c->w.hist);
+        else
+            hipLaunchKernelGGL(k_scatter<false>, dim3(wgs), dim3(kBigWG), ScatLds<false>::kBytes, st, P, B, c->w.hist);
+    }
+    {
+        Launch L(c, K_CHILDREN, 0);
+        hipLaunchKernelGGL(k_children, dim3(std::min(gl, kDevChildWgs)), dim3(kWG), 0, st, P, c->w.hist, sink);
+    }
+}
+
+// Levels a round needs when a segment may hold m strings, each level's
+// buckets assumed up to 4x their mean size (blocks are sorted by the level
+// that lists them, so no level of blocks alone is needed).
+uint32_t dev_levels_for(const ps5_ctx* c, uint64_t m)
+{
+    uint32_t L = 0;
+    while (m > kBlkT && L < 8) {
+        const int k = large_treebits((uint32_t)std::min<uint64_t>(m, 0xFFFFFFFFull), c->leaf_target,
+                                     (int)c->max_treebits);
+        m = (m * 4 + (1ull << k) - 1) >> k;
+        ++L;
+    }
+    return std::max<uint32_t>(L, 1);
+}
+
+// The host-planned levels of a round from the large list of counter cur_ci
+// (in h_cnt / h_large): each level's large list is read back and planned on
+// the host, and the blocks listed before a level are sorted in it
+// (blk_done: blocks already sorted per size class).
+int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t* blk_done, bool fused_root,
+                bool* leaves_forked)
+{
+    hipStream_t st = c->stream;
+    const uint32_t depth = round * 8;
+    const int rp = round & 1, np = rp ^ 1;
+    const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
+    int rc;
+    for (uint32_t level = 0;; ++level) {
+        if (level > 4096) return set_err(PS5_ERR_INTERNAL, "sample-sort levels do not terminate");
+        const uint32_t nlarge = c->h_cnt[cur_ci];
+        uint32_t nblk[kBlkClasses], nblk_all = 0;
+        for (int cls = 0; cls < kBlkClasses; ++cls) {
+            nblk[cls] = c->h_cnt[C_BLOCK + 2 * cls + rp] - blk_done[cls];
+            nblk_all += nblk[cls];
+        }
+        if (nlarge == 0 && nblk_all == 0) break;
+        c->st_levels++;
+        // the last level (blocks only): the round's other leaves are all
+        // listed, and when they are many they run on the side stream
+        // beside the blocks (not when profiling, whose events time the
+        // main stream; for few leaves the fork costs more than it hides)
+        const uint32_t nt = c->h_cnt[C_TINY + rp], nr = c->h_cnt[C_RUN + rp];
+        if (nlarge == 0 && !c->profile && div_up(nt, kWaves) + nr >= kForkLeafGroups) {
+            HIPCHK(hipEventRecord(c->ev_fork, st));
+            HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            launch_leaves(c, B, rp, nsink, depth, div_up(nt, kWaves), nr, nt, nr, c->side);
+            HIPCHK(hipEventRecord(c->ev_join, c->side));
+            *leaves_forked = true;
+        }
+        Item* nx_large = c->w.l_lvl[level & 1];
+        int nx_ci = C_LLVL + (level & 1);
+        if (nx_ci == cur_ci) {      // (a round continued from device levels may start on either list)
+            nx_large = c->w.l_lvl[(level & 1) ^ 1];
+            nx_ci = C_LLVL + ((level & 1) ^ 1);
+        }
+        if (round > 0 || level > 0 || cur_ci != C_LROUND + rp) {     // (every counter starts at zero)
+            Launch L(c, K_MISC, 0);
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, 1u << nx_ci);
+        }
+        const Sink sink = make_sink(c, nx_large, nx_ci, rp);
+        const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
+        if (nlarge && (rc = large_step(c, B, nlarge, sink, seed, fused_root && round == 0 && level == 0)))
+            return rc;
+        for (int cls = 0; cls < kBlkClasses; ++cls) {
+            if (!nblk[cls]) continue;
+            launch_block_class(c, B, rp, cls, blk_done[cls], nblk[cls], false, nblk[cls], nsink, depth);
+            sum_sizes(c, c->w.l_block[rp] + (uint64_t)cls * c->w.cap_block, blk_done[cls],
+                      c->d_cnt + C_BLOCK + 2 * cls + rp, blk_done[cls] + nblk[cls], S_BLOCK);
+            blk_done[cls] += nblk[cls];
+        }
+        HIPCHK(hipGetLastError());
+        // blocks only append to the next round's lists: after a level
+        // without large segments this round has no more levels
+        if (nlarge == 0) break;
+        if ((rc = readback(c, nx_large, nx_ci))) return rc;
+        cur_ci = nx_ci;
+    }
+    return PS5_OK;
+}
+
+// Leaves and leaf boundaries of a round planned on the host (h_cnt holds
+// the round's final counts), then the read-back of the next round's lists.
+int host_round_end(ps5_ctx* c, const Bufs& B, uint32_t round, bool leaves_forked)
+{
+    hipStream_t st = c->stream;
+    const uint32_t depth = round * 8;
+    const int rp = round & 1, np = rp ^ 1;
+    const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
+    const uint32_t nt = c->h_cnt[C_TINY + rp], nr = c->h_cnt[C_RUN + rp];
+    uint32_t nb = 0;
+    for (int cls = 0; cls < kBlkClasses; ++cls) nb += c->h_cnt[C_BLOCK + 2 * cls + rp];
+    c->st_leaves += nt + nb + nr;
+    if (c->trace) {
+        fprintf(stderr, "[ps5] round %u depth %u: tiny %u, blocks %u/%u/%u/%u, equal chunks %u, levels %llu\n",
+                round, depth, nt, c->h_cnt[C_BLOCK + rp], c->h_cnt[C_BLOCK + 2 + rp], c->h_cnt[C_BLOCK + 4 + rp],
+                c->h_cnt[C_BLOCK + 6 + rp], nr, (unsigned long long)c->st_levels);
+    }
+    if (leaves_forked) HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
+    else launch_leaves(c, B, rp, nsink, depth, div_up(nt, kWaves), nr, nt, nr, st);
+    if (nt + nb + nr && B.lcp) launch_leafbound(c, B, rp, depth, div_up((uint64_t)nt + nb + nr, kWG), nt + nb + nr);
+    HIPCHK(hipGetLastError());
+    return readback(c, c->w.l_round[np], C_LROUND + np);
+}
+
 int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* offs, const uint64_t* offs64,
              uint32_t n, uint32_t* perm, uint32_t* lcp, const uint64_t* set_first, uint32_t nsets)
 {
@@ -2362,6 +2829,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     B.lcp = lcp;
     B.bkey = c->w.bkey;
     B.bktrank = c->w.bktrank;
+    B.pending = nullptr;
 
     // One root larger than a block: its keys are extracted by the first
     // classification pass itself, and the host knows the root without asking.
@@ -2372,12 +2840,38 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     }
     if ((rc = setup_roots(c, n, set_first, nsets))) return rc;
     HIPCHK(hipGetLastError());
+    // device-planned levels: the plan buffers take their largest size now
+    // (the levels' sizes are not known on the host)
+    const bool dev = c->dev_levels;
+    uint32_t root_kmax = 1;                     // largest treebits of a large root
+    uint64_t root_max = n;                      // largest root (no segment of any round is larger)
+    uint64_t root_sum = n;                      // strings in large roots
+    if (!set_first) {
+        root_kmax = (uint32_t)large_treebits(n, c->leaf_target, (int)c->max_treebits);
+    } else {
+        root_max = 0;
+        root_sum = 0;
+        for (uint32_t s = 0; s < nsets; ++s) {
+            const uint64_t m = set_first[s + 1] - set_first[s];
+            root_max = std::max(root_max, m);
+            if (m <= kBlkT) continue;
+            root_sum += m;
+            root_kmax = std::max<uint32_t>(root_kmax, (uint32_t)large_treebits((uint32_t)m, c->leaf_target,
+                                                                               (int)c->max_treebits));
+        }
+    }
+    if (dev && (rc = grow_level(c, c->w.cap_large, c->w.cap + c->w.cap / 4 + 64))) return rc;
     if (fused_root) {
         for (int i = 0; i < C_NCNT; ++i) c->h_cnt[i] = 0;
         c->h_cnt[C_LROUND] = 1;
         c->w.h_large[0] = Item{ 0u, n, 0u, F_IDENT };
-    } else if ((rc = readback(c, c->w.l_round[0], C_LROUND))) {
-        return rc;
+    } else if (!dev) {
+        if ((rc = readback(c, c->w.l_round[0], C_LROUND))) return rc;
+    } else if (set_first) {     // the counts setup_roots uploaded
+        memcpy(c->h_cnt, c->h_upcnt, C_NCNT * sizeof(uint32_t));
+    } else {                    // one root, a tiny leaf or a block
+        for (int i = 0; i < C_NCNT; ++i) c->h_cnt[i] = 0;
+        c->h_cnt[n <= kTinyT ? C_TINY : C_BLOCK + 2 * block_class(n)] = 1;
     }
 
     for (uint32_t round = 0;; ++round) {
@@ -2387,122 +2881,99 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
         if (round > 0) {   // the lists of round r+1 are empty again (consumed in round r-1; set up empty in round 0)
             Launch L(c, K_MISC, 0);
             uint32_t mask = (1u << (C_LROUND + np)) | (1u << (C_TINY + np)) | (1u << (C_RUN + np));
-            for (int cls = 0; cls < kBlkClasses; ++cls) mask |= 1u << (C_BLOCK + 2 * cls + np);
+            for (int cls = 0; cls < kBlkClasses; ++cls) mask |= (1u << (C_BLOCK + 2 * cls + np)) | (1u << (C_BDONE + cls));
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, mask);
         }
         const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
-        // tiny leaves and equal-key chunks of this round, on stream s
-        auto launch_leaves = [&](hipStream_t s) {
-            const uint32_t nt = c->h_cnt[C_TINY + rp], nr = c->h_cnt[C_RUN + rp];
-            if (nt) {
-                {
-                    Launch L(c, K_LEAF_TINY, 0);
-                    hipLaunchKernelGGL(k_leaf_tiny, dim3(div_up(nt, kWaves)), dim3(kWG), 0, s, c->w.l_tiny[rp],
-                                       c->d_cnt + C_TINY + rp, B, depth);
-                }
-                sum_sizes(c, c->w.l_tiny[rp], 0, c->d_cnt + C_TINY + rp, nt, S_TINY);
-            }
-            if (nr) {
-                {
-                    Launch L(c, K_LEAF_RUN, 0);
-                    hipLaunchKernelGGL(k_leaf_run, dim3(nr), dim3(kWG), 0, s, c->w.l_run[rp], c->d_cnt + C_RUN + rp,
-                                       B, nsink, depth);
-                }
-                sum_sizes(c, c->w.l_run[rp], 0, c->d_cnt + C_RUN + rp, nr, S_RUN);
-            }
-        };
-        bool leaves_forked = false;
         uint32_t blk_done[kBlkClasses] = {};
-        int cur_ci = C_LROUND + rp;
-        for (uint32_t level = 0;; ++level) {
-            if (level > 4096) return set_err(PS5_ERR_INTERNAL, "sample-sort levels do not terminate");
-            const uint32_t nlarge = c->h_cnt[cur_ci];
-            uint32_t nblk[kBlkClasses], nblk_all = 0;
-            for (int cls = 0; cls < kBlkClasses; ++cls) {
-                nblk[cls] = c->h_cnt[C_BLOCK + 2 * cls + rp] - blk_done[cls];
-                nblk_all += nblk[cls];
-            }
-            if (nlarge == 0 && nblk_all == 0) break;
-            c->st_levels++;
-            // the last level (blocks only): the round's other leaves are all
-            // listed, and when they are many they run on the side stream
-            // beside the blocks (not when profiling, whose events time the
-            // main stream; for few leaves the fork costs more than it hides)
-            if (nlarge == 0 && !c->profile &&
-                div_up(c->h_cnt[C_TINY + rp], kWaves) + c->h_cnt[C_RUN + rp] >= kForkLeafGroups) {
-                HIPCHK(hipEventRecord(c->ev_fork, st));
-                HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-                launch_leaves(c->side);
-                HIPCHK(hipEventRecord(c->ev_join, c->side));
-                leaves_forked = true;
-            }
-            Item* nx_large = c->w.l_lvl[level & 1];
-            const int nx_ci = C_LLVL + (level & 1);
-            if (round > 0 || level > 0) {     // (every counter starts at zero)
-                Launch L(c, K_MISC, 0);
-                hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, 1u << nx_ci);
-            }
-            const Sink sink = make_sink(c, nx_large, nx_ci, rp);
-            const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
-            if (nlarge && (rc = large_step(c, B, nlarge, sink, seed, fused_root && round == 0 && level == 0)))
-                return rc;
-            for (int cls = 0; cls < kBlkClasses; ++cls) {
-                if (!nblk[cls]) continue;
-                const Item* list = c->w.l_block[rp] + (uint64_t)cls * c->w.cap_block;
-                const uint32_t* cnt = c->d_cnt + C_BLOCK + 2 * cls + rp;
-                {
-                    Launch L(c, K_BLOCK, 0);
-                    if (cls == 0)
-                        hipLaunchKernelGGL((k_block<256, kBlkE0>), dim3(nblk[cls]), dim3(256 / kBlkE0), 0, st, list,
-                                           blk_done[cls], cnt, B, nsink, depth);
-                    else if (cls == 1)
-                        hipLaunchKernelGGL((k_block<512, kBlkE1>), dim3(nblk[cls]), dim3(512 / kBlkE1), 0, st, list,
-                                           blk_done[cls], cnt, B, nsink, depth);
-                    else if (cls == 2)
-                        hipLaunchKernelGGL((k_block<1024, kBlkE2>), dim3(nblk[cls]), dim3(1024 / kBlkE2), 0, st, list,
-                                           blk_done[cls], cnt, B, nsink, depth);
-                    else
-                        hipLaunchKernelGGL((k_block<kBlkT, kBlkE3>), dim3(nblk[cls]), dim3(kBlkT / kBlkE3), 0, st, list,
-                                           blk_done[cls], cnt, B, nsink, depth);
+        bool leaves_forked = false;
+        if (!dev) {
+            if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, fused_root, &leaves_forked))) return rc;
+            if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
+        } else {
+            // every level of the round without a host synchronisation.  The
+            // host knows the round's lists (read back with the previous
+            // round, or set up): S strings in nl0 large segments bound the
+            // grids; a round without large segments is planned on the host
+            const uint32_t nl0 = c->h_cnt[C_LROUND + rp];
+            uint64_t S = 0, mmax = 0;
+            if (round == 0) {
+                S = root_sum;
+                mmax = root_max;
+            } else if (nl0 <= kPeek) {
+                for (uint32_t i = 0; i < nl0; ++i) {
+                    S += c->w.h_large[i].size;
+                    mmax = std::max<uint64_t>(mmax, c->w.h_large[i].size);
                 }
-                sum_sizes(c, list, blk_done[cls], cnt, blk_done[cls] + nblk[cls], S_BLOCK);
-                blk_done[cls] += nblk[cls];
+            } else {
+                S = n;
+                mmax = root_max;
             }
-            HIPCHK(hipGetLastError());
-            // blocks only append to the next round's lists: after a level
-            // without large segments this round has no more levels
-            if (nlarge == 0) break;
-            if ((rc = readback(c, nx_large, nx_ci))) return rc;
-            cur_ci = nx_ci;
-        }
-        // leaves of this round
-        const uint32_t nt = c->h_cnt[C_TINY + rp], nr = c->h_cnt[C_RUN + rp];
-        uint32_t nb = 0;
-        for (int cls = 0; cls < kBlkClasses; ++cls) nb += c->h_cnt[C_BLOCK + 2 * cls + rp];
-        c->st_leaves += nt + nb + nr;
-        if (c->trace) {
-            fprintf(stderr, "[ps5] round %u depth %u: tiny %u, blocks %u/%u/%u/%u, equal chunks %u, levels %llu\n",
-                    round, depth, nt, c->h_cnt[C_BLOCK + rp], c->h_cnt[C_BLOCK + 2 + rp], c->h_cnt[C_BLOCK + 4 + rp],
-                    c->h_cnt[C_BLOCK + 6 + rp], nr, (unsigned long long)c->st_levels);
-        }
-        if (leaves_forked) HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
-        else launch_leaves(st);
-        if (nt + nb + nr && lcp) {
-            Launch L(c, K_LEAFBOUND, (uint64_t)(nt + nb + nr) * 36);
-            LeafLists LL;
-            uint32_t end = 0;
-            LL.list[0] = c->w.l_tiny[rp];
-            LL.end[0] = end += nt;
-            for (int cls = 0; cls < kBlkClasses; ++cls) {
-                LL.list[1 + cls] = c->w.l_block[rp] + (uint64_t)cls * c->w.cap_block;
-                LL.end[1 + cls] = end += c->h_cnt[C_BLOCK + 2 * cls + rp];
+            const uint32_t L = nl0 ? dev_levels_for(c, mmax) : 0;
+            if (L == 0) {
+                if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, false, &leaves_forked))) return rc;
+                if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
+            } else {
+                int in_ci = C_LROUND + rp;
+                const Item* in_list = c->w.l_round[rp];
+                const uint32_t deeper_large = (uint32_t)std::min<uint64_t>(c->w.cap_large, S / (kBlkT + 1) + 1);
+                for (uint32_t level = 0; level < L; ++level) {
+                    c->st_levels++;
+                    const int out_ci = C_LLVL + (level & 1);
+                    if (!(round == 0 && level == 0)) {
+                        Launch Lr(c, K_MISC, 0);
+                        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, 1u << out_ci);
+                    }
+                    const Sink sink = make_sink(c, c->w.l_lvl[level & 1], out_ci, rp);
+                    const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
+                    const bool rootlvl = round == 0 && level == 0;
+                    dev_level(c, B, in_list, in_ci, level == 0 ? nl0 : deeper_large, S, sink, seed,
+                              fused_root && rootlvl, rootlvl, rootlvl ? (int)root_kmax : (int)c->max_treebits,
+                              level > 0 ? rp : -1);
+                    for (int cls = 0; cls < kBlkClasses; ++cls) {
+                        // enough workgroups for every wave slot of the GPU (32 per CU)
+                        const uint32_t minm = cls == 0 ? kTinyT + 1 : (128u << cls) + 1;
+                        const uint32_t waves = (256u << cls) / (uint32_t)kBlkE0 / 64;
+                        const uint64_t full = (uint64_t)c->scatter_wgs * 32 / waves;
+                        const uint64_t known = level == 0 ? c->h_cnt[C_BLOCK + 2 * cls + rp] : 0;
+                        const uint32_t g = (uint32_t)std::min<uint64_t>(full, known + S / minm + 1);
+                        launch_block_class(c, B, rp, cls, 0, 0, true, g, nsink, depth);
+                    }
+                    HIPCHK(hipGetLastError());
+                    in_ci = out_ci;
+                    in_list = c->w.l_lvl[level & 1];
+                }
+                // leaves with persistent grids: they skip their work when
+                // large segments remain (counter in_ci), which the host then
+                // finishes
+                const uint64_t nt0 = c->h_cnt[C_TINY + rp], nr0 = c->h_cnt[C_RUN + rp];
+                const uint32_t gt = (uint32_t)std::min<uint64_t>(kDevTinyWgs, div_up(nt0 + S, kWaves));
+                const uint32_t gr = (uint32_t)std::min<uint64_t>(kDevRunWgs, nr0 + S / kTinyT + 1);
+                B.pending = c->d_cnt + in_ci;
+                launch_leaves(c, B, rp, nsink, depth, gt, gr, nt0 + S, nr0 + S, st);
+                if (lcp) {
+                    uint64_t known = nt0 + nr0;
+                    for (int cls = 0; cls < kBlkClasses; ++cls) known += c->h_cnt[C_BLOCK + 2 * cls + rp];
+                    launch_leafbound(c, B, rp, depth,
+                                     (uint32_t)std::min<uint64_t>(kDevBoundWgs, div_up(known + S, kWG)), known + S);
+                }
+                B.pending = nullptr;
+                HIPCHK(hipGetLastError());
+                if ((rc = readback(c, c->w.l_round[np], C_LROUND + np))) return rc;
+                if (c->h_cnt[in_ci] != 0) {
+                    // more levels than planned: read the remaining large list
+                    // back and finish the round on the host
+                    for (int cls = 0; cls < kBlkClasses; ++cls) blk_done[cls] = c->h_cnt[C_BLOCK + 2 * cls + rp];
+                    if ((rc = readback(c, in_list, in_ci))) return rc;
+                    if ((rc = host_levels(c, B, round, in_ci, blk_done, false, &leaves_forked))) return rc;
+                    if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
+                } else {
+                    uint32_t nb = 0;
+                    for (int cls = 0; cls < kBlkClasses; ++cls) nb += c->h_cnt[C_BLOCK + 2 * cls + rp];
+                    c->st_leaves += c->h_cnt[C_TINY + rp] + nb + c->h_cnt[C_RUN + rp];
+                }
             }
-            LL.list[kBlkClasses + 1] = c->w.l_run[rp];
-            LL.end[kBlkClasses + 1] = end += nr;
-            hipLaunchKernelGGL(k_leafbound, dim3(div_up((uint64_t)end, kWG)), dim3(kWG), 0, st, LL, B, depth);
         }
-        HIPCHK(hipGetLastError());
-        if ((rc = readback(c, c->w.l_round[np], C_LROUND + np))) return rc;
         uint64_t pending = (uint64_t)c->h_cnt[C_LROUND + np] + c->h_cnt[C_TINY + np] + c->h_cnt[C_RUN + np];
         for (int cls = 0; cls < kBlkClasses; ++cls) pending += c->h_cnt[C_BLOCK + 2 * cls + np];
         if (pending == 0) break;
@@ -2539,12 +3010,15 @@ int ps5_ctx_create(ps5_ctx** out, int device)
                                ScatLds<false>::kBytes));
     HIPCHK(hipFuncSetAttribute((const void*)k_scatter<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                ScatLds<true>::kBytes));
-    for (const void* f : { (const void*)k_count<8>, (const void*)k_count<9>, (const void*)k_count<10>,
-                           (const void*)k_count<11> })
+    for (const void* f : { (const void*)k_count<8, false>, (const void*)k_count<9, false>,
+                           (const void*)k_count<10, false>, (const void*)k_count<11, false>,
+                           (const void*)k_count<8, true>, (const void*)k_count<9, true>,
+                           (const void*)k_count<10, true>, (const void*)k_count<11, true> })
         HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, count_lds_bytes(kMaxTreebits)));
     ps5_ctx* c = new ps5_ctx;
     c->device = device;
     c->trace = getenv("PS5_TRACE") != nullptr;
+    if (const char* dl = getenv("PS5_DEV_LEVELS")) c->dev_levels = atoi(dl) != 0;
     int ncu = 0;
     hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
     c->scatter_wgs = std::max<uint32_t>(kXcds, (uint32_t)std::max(ncu, 0) / kXcds * kXcds);
@@ -2566,9 +3040,9 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     if (e == hipSuccess)
         e = hipHostMalloc(reinterpret_cast<void**>(&c->na_h_large), cap_large_of(kNoAllocMaxStrings) * sizeof(Item));
     if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void**>(&c->na_h_desc), plan_bytes_of(kNoAllocMaxStrings),
+        e = hipHostMalloc(reinterpret_cast<void**>(&c->na_h_plan), plan_bytes_of(kNoAllocMaxStrings),
                           hipHostMallocMapped | hipHostMallocCoherent);
-    if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&c->na_h_desc_dev), c->na_h_desc, 0);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&c->na_h_plan_dev), c->na_h_plan, 0);
     if (e != hipSuccess) {
         ps5_ctx_destroy(c);
         return set_err(PS5_ERR_HIP, "context setup failed: %s", hipGetErrorString(e));
@@ -2597,7 +3071,7 @@ void ps5_ctx_destroy(ps5_ctx* c)
     if (c->h_roots) (void)hipHostFree(c->h_roots);
     if (c->h_sums) (void)hipHostFree(c->h_sums);
     if (c->na_h_large) (void)hipHostFree(c->na_h_large);
-    if (c->na_h_desc) (void)hipHostFree(c->na_h_desc);
+    if (c->na_h_plan) (void)hipHostFree(c->na_h_plan);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto& u : c->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     if (c->side) (void)hipStreamSynchronize(c->side);
@@ -2655,7 +3129,7 @@ int ps5_sort_device_noalloc(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_b
     std::lock_guard<std::mutex> g(scratch_mutex(c->device));
     // the context's own buffers step aside for views into the scratch;
     // profiling (which creates events) is off for the call
-    WorkBuffers own = scratch_buffers(c->na_scratch, c->na_h_large, c->na_h_desc, c->na_h_desc_dev);
+    WorkBuffers own = scratch_buffers(c->na_scratch, c->na_h_large, c->na_h_plan, c->na_h_plan_dev);
     std::swap(c->w, own);
     const bool prof = c->profile;
     c->profile = false;
@@ -2869,6 +3343,18 @@ int ps5_ctx_get_tuning(ps5_ctx* c, uint32_t* leaf_target, uint32_t* max_treebits
     if (max_treebits) *max_treebits = c->max_treebits;
     if (max_oversample) *max_oversample = c->max_oversample;
     return PS5_OK;
+}
+
+int ps5_ctx_set_planning(ps5_ctx* c, int device)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
+    c->dev_levels = device != 0;
+    return PS5_OK;
+}
+
+int ps5_ctx_get_planning(ps5_ctx* c)
+{
+    return c ? (c->dev_levels ? 1 : 0) : -1;
 }
 
 int ps5_ctx_set_profile(ps5_ctx* c, int on)

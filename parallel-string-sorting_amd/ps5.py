@@ -22,7 +22,8 @@ EXPORTS = [
     "mi355x_ps5_sort_lcp_cache",
     "ps5_sort_strings", "ps5_sort_strings_cache", "ps5_dist_chars_device",
     "ps5_ctx_create", "ps5_ctx_destroy", "ps5_ctx_reserve", "ps5_ctx_stream",
-    "ps5_sort_device", "ps5_sort_device_noalloc", "ps5_noalloc_max_strings", "ps5_ctx_set_profile", "ps5_ctx_kernel_count", "ps5_ctx_kernel_stat",
+    "ps5_sort_device", "ps5_sort_device_noalloc", "ps5_noalloc_max_strings", "ps5_ctx_set_profile",
+    "ps5_ctx_set_planning", "ps5_ctx_get_planning", "ps5_ctx_kernel_count", "ps5_ctx_kernel_stat",
     "ps5_ctx_work_stats",
     "ps5_dev_malloc", "ps5_dev_free", "ps5_memcpy_h2d", "ps5_memcpy_d2h", "ps5_device_synchronize",
     "ps5_dev_copy",
@@ -79,6 +80,8 @@ def load():
     _sig(lib, "ps5_noalloc_max_strings", _U64, [])
     _sig(lib, "ps5_sort_device_timed", _I, [_P, _P, _U64, _P, _U64, _P, _P, ctypes.POINTER(ctypes.c_double)])
     _sig(lib, "ps5_ctx_set_profile", _I, [_P, _I])
+    _sig(lib, "ps5_ctx_set_planning", _I, [_P, _I])
+    _sig(lib, "ps5_ctx_get_planning", _I, [_P])
     _sig(lib, "ps5_ctx_kernel_count", _I, [_P])
     _sig(lib, "ps5_ctx_kernel_stat", _I, [_P, _I, ctypes.POINTER(ctypes.c_char_p),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_U64),
@@ -394,6 +397,13 @@ class Context:
                                                 *[ctypes.byref(x) for x in v], ctypes.byref(ms)),
                "ps5_autotune_batch_device")
         return tuple(x.value for x in v), ms.value
+
+    def set_planning(self, device):
+        """0: levels planned on the host; 1: on the device (no synchronisation inside a round)."""
+        _check(load().ps5_ctx_set_planning(self.h, int(device)), "ps5_ctx_set_planning")
+
+    def get_planning(self):
+        return int(load().ps5_ctx_get_planning(self.h))
 
     def set_profile(self, on):
         _check(load().ps5_ctx_set_profile(self.h, 1 if on else 0), "ps5_ctx_set_profile")
