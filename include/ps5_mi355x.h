@@ -146,6 +146,21 @@ int ps5_ctx_set_planning(ps5_ctx* ctx, int device);
 /* The current planning mode (0 or 1), -1 for a NULL context. */
 int ps5_ctx_get_planning(ps5_ctx* ctx);
 
+/* Capture the first `rounds` rounds of a sort (8 characters of depth each)
+ * as one HIP graph, planned on the device, and replay it for later sorts with
+ * the same buffers, string count, set boundaries and settings (0, the
+ * default: off; at most 64).  A sort then needs one host synchronisation for
+ * those rounds; strings that still tie after them continue round by round,
+ * and a captured round that needs more sample-sort levels than the graph
+ * holds is sorted again with host-planned levels, so results are always those
+ * of ps5_sort_device.  The first sort of a shape pays the capture.  No
+ * counterpart in the reference; for callers that sort the same shape
+ * repeatedly (batches of fixed-size sets).  Not used by
+ * ps5_sort_device_noalloc, since instantiating a graph allocates. */
+int ps5_ctx_set_graph_rounds(ps5_ctx* ctx, uint32_t rounds);
+/* The captured round count, -1 for a NULL context. */
+int ps5_ctx_get_graph_rounds(ps5_ctx* ctx);
+
 /* Distinguishing characters of a finished sort (inputs as ps5_sort_device,
  * d_perm and d_lcp its results): d_cache[i] = s_i[lcp[i]] for i >= 1, the
  * first character of the i-th smallest string that its predecessor does not

@@ -84,6 +84,7 @@ enum : int {
     C_NCNT = 21
 };
 constexpr uint32_t ERR_CAP = 1u, ERR_LCP = 2u, ERR_BOUNDS = 4u;
+constexpr uint32_t ERR_LEVELS = 8u;          // a captured round had more levels than it was built with
 
 struct LargeDesc {       // one segment of the multi-workgroup sample-sort step
     Item seg;
@@ -139,6 +140,7 @@ struct Bufs {
     key64* bkey;
     uint32_t* bktrank;
     const uint32_t* pending;   // non-null and non-zero: large segments remain, the round's leaves wait
+    uint32_t pending_err;      // ... and that is an error (ERR_LEVELS: a captured graph cannot continue the round)
 };
 
 }  // namespace ps5

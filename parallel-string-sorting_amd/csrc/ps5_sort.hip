@@ -1014,7 +1014,10 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
     __shared__ uint32_t s_i[kWaves][64];
     __shared__ uint32_t s_r[kWaves][64];     // first sorted position of the run holding a position
     const uint32_t w = wave_id(), lane = lane_id();
-    if (B.pending && *B.pending) return;      // the round's levels are not finished
+    if (B.pending && *B.pending) {      // the round's levels are not finished
+        if (B.pending_err && tid_x() == 0) flag_error(B.cnt, ERR_LEVELS);
+        return;
+    }
     const uint32_t nitems = min(*cnt_items, cap);     // (an overflow has been flagged)
     // each wave takes leaves it, it + stride, ... (no workgroup barriers below)
     for (uint32_t it = blockIdx.x * kWaves + w; it < nitems; it += gridDim.x * kWaves) {
@@ -1532,7 +1535,10 @@ __global__ __launch_bounds__(MAXM / E) __attribute__((amdgpu_waves_per_eu(8, 8))
 __global__ __launch_bounds__(kWG) void k_leaf_run(const Item* items, const uint32_t* cnt_items, uint32_t cap,
                                                   Bufs B, Sink nxt, uint32_t depth)
 {
-    if (B.pending && *B.pending) return;      // the round's levels are not finished
+    if (B.pending && *B.pending) {      // the round's levels are not finished
+        if (B.pending_err && tid_x() == 0) flag_error(B.cnt, ERR_LEVELS);
+        return;
+    }
     const uint32_t nitems = min(*cnt_items, cap);     // (an overflow has been flagged)
     for (uint32_t ii = blockIdx.x; ii < nitems; ii += gridDim.x) {
         __syncthreads();     // the previous chunk's key read is complete
@@ -1587,7 +1593,10 @@ struct LeafLists {        // the leaf lists of a round, concatenated in this ord
 __global__ __launch_bounds__(kWG) void k_leafbound(LeafLists LL, Bufs B, uint32_t depth)
 {
     constexpr int NL = kBlkClasses + 2;
-    if (B.pending && *B.pending) return;      // the round's levels are not finished
+    if (B.pending && *B.pending) {      // the round's levels are not finished
+        if (B.pending_err && tid_x() == 0) flag_error(B.cnt, ERR_LEVELS);
+        return;
+    }
     uint32_t end[NL];        // running totals of the list lengths
     uint32_t run = 0;
 #pragma unroll
@@ -2047,6 +2056,15 @@ struct ps5_ctx {
     uint32_t max_oversample = 32;    // sample keys per splitter (limited by kMaxSamples)
     uint64_t seed = 0x5EED5EEDull;
     bool dev_levels = false;         // plan levels on the device: no host synchronisation inside a round
+    // a sort's first rounds as one captured HIP graph (0: off), and the
+    // instantiated graph of the last sort, replayed while its inputs match
+    uint32_t graph_rounds = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint64_t> graph_key;
+    uint64_t w_gen = 0;              // bumped whenever a work buffer is allocated again
+    uint32_t graph_levels = 0;       // non-zero: fewer levels per captured round (PS5_GRAPH_LEVELS; tests
+                                     // use it to reach the host-planned fallback)
     // profiling
     bool profile = false;
     bool trace = false;              // PS5_TRACE set: per-round work to stderr (diagnostics)
@@ -2102,6 +2120,7 @@ int grow_level(ps5_ctx* c, uint32_t nlarge, uint64_t hl)
         const uint64_t want = std::max<uint64_t>(nlarge + nlarge / 2, 64);
         HIPCHK(hipMalloc(&c->w.trees, want * kTreeSlots * sizeof(key64)));
         c->w.trees_cap = want;
+        c->w_gen++;
     }
     if (hl > c->w.hist_cap) {
         if (c->w.hist) (void)hipFree(c->w.hist);
@@ -2114,6 +2133,7 @@ int grow_level(ps5_ctx* c, uint32_t nlarge, uint64_t hl)
         c->w.part_cap = want / kScanBlock + 2;
         HIPCHK(hipMalloc(&c->w.part, c->w.part_cap * sizeof(uint32_t)));
         c->w.hist_cap = want;
+        c->w_gen++;
     }
     return PS5_OK;
 }
@@ -2153,6 +2173,7 @@ int ensure_capacity(ps5_ctx* c, uint64_t n)
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->w.h_plan), plan_bytes, hipHostMallocMapped | hipHostMallocCoherent));
     HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->w.h_plan_dev), c->w.h_plan, 0));
     c->w.cap = cap;
+    c->w_gen++;
     return PS5_OK;
 }
 
@@ -2353,6 +2374,7 @@ int readback(ps5_ctx* c, const Item* list, int ci)
     if (c->h_cnt[C_ERR] & ERR_CAP) return set_err(PS5_ERR_CAPACITY, "work list capacity exceeded");
     if (c->h_cnt[C_ERR] & ERR_LCP) return set_err(PS5_ERR_INTERNAL, "equal keys across a leaf boundary");
     if (c->h_cnt[C_ERR] & ERR_BOUNDS) return set_err(PS5_ERR_INTERNAL, "device bounds check failed");
+    if (c->h_cnt[C_ERR] & ERR_LEVELS) return set_err(PS5_ERR_INTERNAL, "a captured round needed more levels");
     if (list) {
         const uint32_t nl = c->h_cnt[ci];
         if (nl > c->w.cap_large) return set_err(PS5_ERR_CAPACITY, "large list overflow");
@@ -2561,6 +2583,7 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
     return PS5_OK;
 }
 
+constexpr uint32_t kMaxGraphRounds = 64;      // rounds one captured graph may hold
 constexpr uint32_t kForkLeafGroups = 8192;   // leaf workgroups worth a side stream
 
 // Largest grids of the device-planned levels: their kernels loop over the
@@ -2806,6 +2829,181 @@ int host_round_end(ps5_ctx* c, const Bufs& B, uint32_t round, bool leaves_forked
     return readback(c, c->w.l_round[np], C_LROUND + np);
 }
 
+// One sort call's state, shared by its rounds.
+struct SortRun {
+    Bufs B;
+    uint32_t n;
+    uint64_t alen;
+    bool fused_root;          // one root whose keys the first count pass extracts
+    uint32_t root_kmax;       // largest treebits of a large root
+    uint64_t root_max;        // largest root: no segment of any round is larger
+    uint64_t root_sum;        // strings in large roots
+};
+
+// Round `round` of a sort, planned on the host or (dev) on the device; sets
+// *done when no work is left for another round.
+int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
+{
+    int rc;
+    hipStream_t st = c->stream;
+    Bufs& B = R.B;
+    const uint32_t n = R.n;
+    const uint64_t alen = R.alen;
+    const bool fused_root = R.fused_root;
+    const uint32_t root_kmax = R.root_kmax;
+    const uint64_t root_max = R.root_max, root_sum = R.root_sum;
+    uint32_t* lcp = B.lcp;
+    *done = false;
+    const uint32_t depth = round * 8;
+    const int rp = round & 1, np = rp ^ 1;
+    c->st_rounds++;
+    if (round > 0) {   // the lists of round r+1 are empty again (consumed in round r-1; set up empty in round 0)
+        Launch L(c, K_MISC, 0);
+        uint32_t mask = (1u << (C_LROUND + np)) | (1u << (C_TINY + np)) | (1u << (C_RUN + np));
+        for (int cls = 0; cls < kBlkClasses; ++cls) mask |= (1u << (C_BLOCK + 2 * cls + np)) | (1u << (C_BDONE + cls));
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, mask);
+    }
+    const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
+    uint32_t blk_done[kBlkClasses] = {};
+    bool leaves_forked = false;
+    if (!dev) {
+        if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, fused_root, &leaves_forked))) return rc;
+        if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
+    } else {
+        // every level of the round without a host synchronisation.  The
+        // host knows the round's lists (read back with the previous
+        // round, or set up): S strings in nl0 large segments bound the
+        // grids; a round without large segments is planned on the host
+        const uint32_t nl0 = c->h_cnt[C_LROUND + rp];
+        uint64_t S = 0, mmax = 0;
+        if (round == 0) {
+            S = root_sum;
+            mmax = root_max;
+        } else if (nl0 <= kPeek) {
+            for (uint32_t i = 0; i < nl0; ++i) {
+                S += c->w.h_large[i].size;
+                mmax = std::max<uint64_t>(mmax, c->w.h_large[i].size);
+            }
+        } else {
+            S = n;
+            mmax = root_max;
+        }
+        const uint32_t L = nl0 ? dev_levels_for(c, mmax) : 0;
+        if (L == 0) {
+            if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, false, &leaves_forked))) return rc;
+            if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
+        } else {
+            int in_ci = C_LROUND + rp;
+            const Item* in_list = c->w.l_round[rp];
+            const uint32_t deeper_large = (uint32_t)std::min<uint64_t>(c->w.cap_large, S / (kBlkT + 1) + 1);
+            for (uint32_t level = 0; level < L; ++level) {
+                c->st_levels++;
+                const int out_ci = C_LLVL + (level & 1);
+                if (!(round == 0 && level == 0)) {
+                    Launch Lr(c, K_MISC, 0);
+                    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, 1u << out_ci);
+                }
+                const Sink sink = make_sink(c, c->w.l_lvl[level & 1], out_ci, rp);
+                const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
+                const bool rootlvl = round == 0 && level == 0;
+                dev_level(c, B, in_list, in_ci, level == 0 ? nl0 : deeper_large, S, sink, seed,
+                          fused_root && rootlvl, rootlvl, rootlvl ? (int)root_kmax : (int)c->max_treebits,
+                          level > 0 ? rp : -1);
+                for (int cls = 0; cls < kBlkClasses; ++cls) {
+                    // enough workgroups for every wave slot of the GPU (32 per CU)
+                    const uint32_t minm = cls == 0 ? kTinyT + 1 : (128u << cls) + 1;
+                    const uint32_t waves = (256u << cls) / (uint32_t)kBlkE0 / 64;
+                    const uint64_t full = (uint64_t)c->scatter_wgs * 32 / waves;
+                    const uint64_t known = level == 0 ? c->h_cnt[C_BLOCK + 2 * cls + rp] : 0;
+                    const uint32_t g = (uint32_t)std::min<uint64_t>(full, known + S / minm + 1);
+                    launch_block_class(c, B, rp, cls, 0, 0, true, g, nsink, depth);
+                }
+                HIPCHK(hipGetLastError());
+                in_ci = out_ci;
+                in_list = c->w.l_lvl[level & 1];
+            }
+            // leaves with persistent grids: they skip their work when
+            // large segments remain (counter in_ci), which the host then
+            // finishes
+            const uint64_t nt0 = c->h_cnt[C_TINY + rp], nr0 = c->h_cnt[C_RUN + rp];
+            const uint32_t gt = (uint32_t)std::min<uint64_t>(kDevTinyWgs, div_up(nt0 + S, kWaves));
+            const uint32_t gr = (uint32_t)std::min<uint64_t>(kDevRunWgs, nr0 + S / kTinyT + 1);
+            B.pending = c->d_cnt + in_ci;
+            launch_leaves(c, B, rp, nsink, depth, gt, gr, nt0 + S, nr0 + S, st);
+            if (lcp) {
+                uint64_t known = nt0 + nr0;
+                for (int cls = 0; cls < kBlkClasses; ++cls) known += c->h_cnt[C_BLOCK + 2 * cls + rp];
+                launch_leafbound(c, B, rp, depth,
+                                 (uint32_t)std::min<uint64_t>(kDevBoundWgs, div_up(known + S, kWG)), known + S);
+            }
+            B.pending = nullptr;
+            HIPCHK(hipGetLastError());
+            if ((rc = readback(c, c->w.l_round[np], C_LROUND + np))) return rc;
+            if (c->h_cnt[in_ci] != 0) {
+                // more levels than planned: read the remaining large list
+                // back and finish the round on the host
+                for (int cls = 0; cls < kBlkClasses; ++cls) blk_done[cls] = c->h_cnt[C_BLOCK + 2 * cls + rp];
+                if ((rc = readback(c, in_list, in_ci))) return rc;
+                if ((rc = host_levels(c, B, round, in_ci, blk_done, false, &leaves_forked))) return rc;
+                if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
+            } else {
+                uint32_t nb = 0;
+                for (int cls = 0; cls < kBlkClasses; ++cls) nb += c->h_cnt[C_BLOCK + 2 * cls + rp];
+                c->st_leaves += c->h_cnt[C_TINY + rp] + nb + c->h_cnt[C_RUN + rp];
+            }
+        }
+    }
+    uint64_t pending = (uint64_t)c->h_cnt[C_LROUND + np] + c->h_cnt[C_TINY + np] + c->h_cnt[C_RUN + np];
+    for (int cls = 0; cls < kBlkClasses; ++cls) pending += c->h_cnt[C_BLOCK + 2 * cls + np];
+    if (pending == 0) { *done = true; return PS5_OK; }
+    if (depth > alen + 8) return set_err(PS5_ERR_INTERNAL, "depth exceeds the arena: no progress");
+    return PS5_OK;
+}
+
+// Round `round` for a captured graph: planned on the device with grids
+// bounded by what the host knows before the sort (n strings, roots of at
+// most root_max), no host synchronisation; L device levels, and a round that
+// needs more is flagged (ERR_LEVELS) by its leaf kernels.
+void graph_round(ps5_ctx* c, SortRun& R, uint32_t round, uint32_t L, uint32_t nl_root)
+{
+    hipStream_t st = c->stream;
+    Bufs B = R.B;
+    const uint64_t n = R.n;
+    const uint32_t depth = round * 8;
+    const int rp = round & 1, np = rp ^ 1;
+    if (round > 0) {
+        uint32_t mask = (1u << (C_LROUND + np)) | (1u << (C_TINY + np)) | (1u << (C_RUN + np));
+        for (int cls = 0; cls < kBlkClasses; ++cls) mask |= (1u << (C_BLOCK + 2 * cls + np)) | (1u << (C_BDONE + cls));
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, mask);
+    }
+    const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
+    const uint64_t S = round == 0 ? R.root_sum : n;
+    const uint32_t any_large = (uint32_t)std::min<uint64_t>(c->w.cap_large, n / (kBlkT + 1) + 1);
+    int in_ci = C_LROUND + rp;
+    const Item* in_list = c->w.l_round[rp];
+    for (uint32_t level = 0; level < L; ++level) {
+        const int out_ci = C_LLVL + (level & 1);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, 1u << out_ci);
+        const Sink sink = make_sink(c, c->w.l_lvl[level & 1], out_ci, rp);
+        const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
+        const bool rootlvl = round == 0 && level == 0;
+        dev_level(c, B, in_list, in_ci, rootlvl ? nl_root : any_large, S, sink, seed, R.fused_root && rootlvl,
+                  rootlvl, rootlvl ? (int)R.root_kmax : (int)c->max_treebits, level > 0 ? rp : -1);
+        for (int cls = 0; cls < kBlkClasses; ++cls) {
+            const uint32_t waves = (256u << cls) / (uint32_t)kBlkE0 / 64;
+            const uint32_t g = (uint32_t)((uint64_t)c->scatter_wgs * 32 / waves);
+            launch_block_class(c, B, rp, cls, 0, 0, true, g, nsink, depth);
+        }
+        in_ci = out_ci;
+        in_list = c->w.l_lvl[level & 1];
+    }
+    B.pending = c->d_cnt + in_ci;
+    B.pending_err = 1;
+    launch_leaves(c, B, rp, nsink, depth, (uint32_t)std::min<uint64_t>(kDevTinyWgs, div_up(n, kWaves)),
+                  (uint32_t)std::min<uint64_t>(kDevRunWgs, c->w.cap_run), n, n, st);
+    if (B.lcp) launch_leafbound(c, B, rp, depth, (uint32_t)std::min<uint64_t>(kDevBoundWgs, div_up(n, kWG)), n);
+}
+
 int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* offs, const uint64_t* offs64,
              uint32_t n, uint32_t* perm, uint32_t* lcp, const uint64_t* set_first, uint32_t nsets)
 {
@@ -2830,6 +3028,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     B.bkey = c->w.bkey;
     B.bktrank = c->w.bktrank;
     B.pending = nullptr;
+    B.pending_err = 0;
 
     // One root larger than a block: its keys are extracted by the first
     // classification pass itself, and the host knows the root without asking.
@@ -2842,7 +3041,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     HIPCHK(hipGetLastError());
     // device-planned levels: the plan buffers take their largest size now
     // (the levels' sizes are not known on the host)
-    const bool dev = c->dev_levels;
+    bool dev = c->dev_levels;
     uint32_t root_kmax = 1;                     // largest treebits of a large root
     uint64_t root_max = n;                      // largest root (no segment of any round is larger)
     uint64_t root_sum = n;                      // strings in large roots
@@ -2874,110 +3073,78 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
         c->h_cnt[n <= kTinyT ? C_TINY : C_BLOCK + 2 * block_class(n)] = 1;
     }
 
-    for (uint32_t round = 0;; ++round) {
-        const uint32_t depth = round * 8;
-        const int rp = round & 1, np = rp ^ 1;
-        c->st_rounds++;
-        if (round > 0) {   // the lists of round r+1 are empty again (consumed in round r-1; set up empty in round 0)
-            Launch L(c, K_MISC, 0);
-            uint32_t mask = (1u << (C_LROUND + np)) | (1u << (C_TINY + np)) | (1u << (C_RUN + np));
-            for (int cls = 0; cls < kBlkClasses; ++cls) mask |= (1u << (C_BLOCK + 2 * cls + np)) | (1u << (C_BDONE + cls));
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, mask);
+    SortRun R{ B, n, alen, fused_root, root_kmax, root_max, root_sum };
+    uint32_t start = 0;
+    if (c->graph_rounds && !c->profile) {
+        // the first graph_rounds rounds as one captured graph, replayed while
+        // the sort's inputs, buffers and shape settings are the same
+        uint32_t nl_root = 0;
+        for (uint32_t s = 0; set_first && s < nsets; ++s) nl_root += set_first[s + 1] - set_first[s] > kBlkT ? 1 : 0;
+        if (!set_first) nl_root = n > kBlkT ? 1 : 0;
+        std::vector<uint64_t> key = { (uint64_t)(uintptr_t)arena, alen, (uint64_t)(uintptr_t)offs,
+                                      (uint64_t)(uintptr_t)offs64, n, (uint64_t)(uintptr_t)perm,
+                                      (uint64_t)(uintptr_t)lcp, nsets, (uint64_t)(uintptr_t)c->w.key[0], c->w.cap,
+                                      (uint64_t)(uintptr_t)c->w.trees, c->w.trees_cap, c->leaf_target,
+                                      c->max_treebits, c->max_oversample, c->graph_rounds, c->seed,
+                                      c->w_gen, c->w.fixed ? 1u : 0u, c->graph_levels };
+        for (uint32_t s = 0; set_first && s <= nsets; ++s) key.push_back(set_first[s]);
+        if ((rc = grow_level(c, c->w.cap_large, c->w.cap + c->w.cap / 4 + 64))) return rc;
+        if (!c->graph_exec || key != c->graph_key) {
+            if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+            if (c->graph) (void)hipGraphDestroy(c->graph);
+            c->graph_exec = nullptr;
+            c->graph = nullptr;
+            c->graph_key.clear();
+            uint32_t L = dev_levels_for(c, root_max);
+            if (c->graph_levels) L = std::min(L, c->graph_levels);
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            for (uint32_t r = 0; r < c->graph_rounds; ++r) graph_round(c, R, r, L, nl_root);
+            hipLaunchKernelGGL(k_stage, dim3(1), dim3(kWG), 0, st, c->d_cnt, c->w.l_round[c->graph_rounds & 1],
+                               C_LROUND + (int)(c->graph_rounds & 1), c->h_stage_dev);
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            if (ec != hipSuccess) return set_err(PS5_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
+            c->graph = g;
+            HIPCHK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+            c->graph_key = key;
         }
-        const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
-        uint32_t blk_done[kBlkClasses] = {};
-        bool leaves_forked = false;
-        if (!dev) {
-            if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, fused_root, &leaves_forked))) return rc;
-            if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
+        HIPCHK(hipGraphLaunch(c->graph_exec, st));
+        HIPCHK(hipStreamSynchronize(st));
+        c->st_syncs++;
+        c->st_rounds += c->graph_rounds;
+        memcpy(c->h_cnt, c->h_stage, C_NCNT * sizeof(uint32_t));
+        if (c->h_cnt[C_ERR] & ERR_LEVELS) {
+            // a round needed more levels than the graph holds: sort again,
+            // every level planned on the host
+            if (!fused_root) {
+                hipLaunchKernelGGL(k_init, dim3(div_up(n, kWG)), dim3(kWG), 0, st, B, n);
+            }
+            if ((rc = setup_roots(c, n, set_first, nsets))) return rc;
+            if (fused_root) {
+                for (int i = 0; i < C_NCNT; ++i) c->h_cnt[i] = 0;
+                c->h_cnt[C_LROUND] = 1;
+                c->w.h_large[0] = Item{ 0u, n, 0u, F_IDENT };
+            } else if ((rc = readback(c, c->w.l_round[0], C_LROUND))) {
+                return rc;
+            }
+            dev = false;
         } else {
-            // every level of the round without a host synchronisation.  The
-            // host knows the round's lists (read back with the previous
-            // round, or set up): S strings in nl0 large segments bound the
-            // grids; a round without large segments is planned on the host
-            const uint32_t nl0 = c->h_cnt[C_LROUND + rp];
-            uint64_t S = 0, mmax = 0;
-            if (round == 0) {
-                S = root_sum;
-                mmax = root_max;
-            } else if (nl0 <= kPeek) {
-                for (uint32_t i = 0; i < nl0; ++i) {
-                    S += c->w.h_large[i].size;
-                    mmax = std::max<uint64_t>(mmax, c->w.h_large[i].size);
-                }
-            } else {
-                S = n;
-                mmax = root_max;
-            }
-            const uint32_t L = nl0 ? dev_levels_for(c, mmax) : 0;
-            if (L == 0) {
-                if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, false, &leaves_forked))) return rc;
-                if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
-            } else {
-                int in_ci = C_LROUND + rp;
-                const Item* in_list = c->w.l_round[rp];
-                const uint32_t deeper_large = (uint32_t)std::min<uint64_t>(c->w.cap_large, S / (kBlkT + 1) + 1);
-                for (uint32_t level = 0; level < L; ++level) {
-                    c->st_levels++;
-                    const int out_ci = C_LLVL + (level & 1);
-                    if (!(round == 0 && level == 0)) {
-                        Launch Lr(c, K_MISC, 0);
-                        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, 1u << out_ci);
-                    }
-                    const Sink sink = make_sink(c, c->w.l_lvl[level & 1], out_ci, rp);
-                    const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
-                    const bool rootlvl = round == 0 && level == 0;
-                    dev_level(c, B, in_list, in_ci, level == 0 ? nl0 : deeper_large, S, sink, seed,
-                              fused_root && rootlvl, rootlvl, rootlvl ? (int)root_kmax : (int)c->max_treebits,
-                              level > 0 ? rp : -1);
-                    for (int cls = 0; cls < kBlkClasses; ++cls) {
-                        // enough workgroups for every wave slot of the GPU (32 per CU)
-                        const uint32_t minm = cls == 0 ? kTinyT + 1 : (128u << cls) + 1;
-                        const uint32_t waves = (256u << cls) / (uint32_t)kBlkE0 / 64;
-                        const uint64_t full = (uint64_t)c->scatter_wgs * 32 / waves;
-                        const uint64_t known = level == 0 ? c->h_cnt[C_BLOCK + 2 * cls + rp] : 0;
-                        const uint32_t g = (uint32_t)std::min<uint64_t>(full, known + S / minm + 1);
-                        launch_block_class(c, B, rp, cls, 0, 0, true, g, nsink, depth);
-                    }
-                    HIPCHK(hipGetLastError());
-                    in_ci = out_ci;
-                    in_list = c->w.l_lvl[level & 1];
-                }
-                // leaves with persistent grids: they skip their work when
-                // large segments remain (counter in_ci), which the host then
-                // finishes
-                const uint64_t nt0 = c->h_cnt[C_TINY + rp], nr0 = c->h_cnt[C_RUN + rp];
-                const uint32_t gt = (uint32_t)std::min<uint64_t>(kDevTinyWgs, div_up(nt0 + S, kWaves));
-                const uint32_t gr = (uint32_t)std::min<uint64_t>(kDevRunWgs, nr0 + S / kTinyT + 1);
-                B.pending = c->d_cnt + in_ci;
-                launch_leaves(c, B, rp, nsink, depth, gt, gr, nt0 + S, nr0 + S, st);
-                if (lcp) {
-                    uint64_t known = nt0 + nr0;
-                    for (int cls = 0; cls < kBlkClasses; ++cls) known += c->h_cnt[C_BLOCK + 2 * cls + rp];
-                    launch_leafbound(c, B, rp, depth,
-                                     (uint32_t)std::min<uint64_t>(kDevBoundWgs, div_up(known + S, kWG)), known + S);
-                }
-                B.pending = nullptr;
-                HIPCHK(hipGetLastError());
-                if ((rc = readback(c, c->w.l_round[np], C_LROUND + np))) return rc;
-                if (c->h_cnt[in_ci] != 0) {
-                    // more levels than planned: read the remaining large list
-                    // back and finish the round on the host
-                    for (int cls = 0; cls < kBlkClasses; ++cls) blk_done[cls] = c->h_cnt[C_BLOCK + 2 * cls + rp];
-                    if ((rc = readback(c, in_list, in_ci))) return rc;
-                    if ((rc = host_levels(c, B, round, in_ci, blk_done, false, &leaves_forked))) return rc;
-                    if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
-                } else {
-                    uint32_t nb = 0;
-                    for (int cls = 0; cls < kBlkClasses; ++cls) nb += c->h_cnt[C_BLOCK + 2 * cls + rp];
-                    c->st_leaves += c->h_cnt[C_TINY + rp] + nb + c->h_cnt[C_RUN + rp];
-                }
-            }
+            if (c->h_cnt[C_ERR] & ERR_CAP) return set_err(PS5_ERR_CAPACITY, "work list capacity exceeded");
+            if (c->h_cnt[C_ERR] & ERR_LCP) return set_err(PS5_ERR_INTERNAL, "equal keys across a leaf boundary");
+            if (c->h_cnt[C_ERR] & ERR_BOUNDS) return set_err(PS5_ERR_INTERNAL, "device bounds check failed");
+            const int rp = c->graph_rounds & 1;
+            uint64_t pending = (uint64_t)c->h_cnt[C_LROUND + rp] + c->h_cnt[C_TINY + rp] + c->h_cnt[C_RUN + rp];
+            for (int cls = 0; cls < kBlkClasses; ++cls) pending += c->h_cnt[C_BLOCK + 2 * cls + rp];
+            if (pending == 0) return PS5_OK;
+            // more rounds than captured: continue round by round
+            if ((rc = readback(c, c->w.l_round[rp], C_LROUND + rp))) return rc;
+            start = c->graph_rounds;
         }
-        uint64_t pending = (uint64_t)c->h_cnt[C_LROUND + np] + c->h_cnt[C_TINY + np] + c->h_cnt[C_RUN + np];
-        for (int cls = 0; cls < kBlkClasses; ++cls) pending += c->h_cnt[C_BLOCK + 2 * cls + np];
-        if (pending == 0) break;
-        if (depth > alen + 8) return set_err(PS5_ERR_INTERNAL, "depth exceeds the arena: no progress");
+    }
+    for (uint32_t round = start;; ++round) {
+        bool done = false;
+        if ((rc = sort_round(c, R, round, dev, &done))) return rc;
+        if (done) break;
     }
     if (c->profile) {
         HIPCHK(hipMemcpyAsync(c->h_sums, c->d_sums, S_NUM * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -3019,6 +3186,8 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     c->device = device;
     c->trace = getenv("PS5_TRACE") != nullptr;
     if (const char* dl = getenv("PS5_DEV_LEVELS")) c->dev_levels = atoi(dl) != 0;
+    if (const char* gr = getenv("PS5_GRAPH_ROUNDS")) c->graph_rounds = std::min<uint32_t>(atoi(gr), kMaxGraphRounds);
+    if (const char* gl = getenv("PS5_GRAPH_LEVELS")) c->graph_levels = (uint32_t)std::max(atoi(gl), 0);
     int ncu = 0;
     hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
     c->scatter_wgs = std::max<uint32_t>(kXcds, (uint32_t)std::max(ncu, 0) / kXcds * kXcds);
@@ -3075,6 +3244,8 @@ void ps5_ctx_destroy(ps5_ctx* c)
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto& u : c->ev_used) { (void)hipEventDestroy(u.second.first); (void)hipEventDestroy(u.second.second); }
     if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -3128,17 +3299,21 @@ int ps5_sort_device_noalloc(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_b
     HIPCHK(hipSetDevice(c->device));
     std::lock_guard<std::mutex> g(scratch_mutex(c->device));
     // the context's own buffers step aside for views into the scratch;
-    // profiling (which creates events) is off for the call
+    // profiling (which creates events) and captured graphs (whose
+    // instantiation allocates) are off for the call
     WorkBuffers own = scratch_buffers(c->na_scratch, c->na_h_large, c->na_h_plan, c->na_h_plan_dev);
     std::swap(c->w, own);
     const bool prof = c->profile;
+    const uint32_t graph_rounds = c->graph_rounds;
     c->profile = false;
+    c->graph_rounds = 0;
     int rc = run_sort(c, d_arena, arena_bytes, d_offs, nullptr, (uint32_t)n, d_perm, d_lcp, nullptr, 0);
     if (rc) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->side);
     }
     c->profile = prof;
+    c->graph_rounds = graph_rounds;
     std::swap(c->w, own);
     return rc;
 }
@@ -3355,6 +3530,28 @@ int ps5_ctx_set_planning(ps5_ctx* c, int device)
 int ps5_ctx_get_planning(ps5_ctx* c)
 {
     return c ? (c->dev_levels ? 1 : 0) : -1;
+}
+
+int ps5_ctx_set_graph_rounds(ps5_ctx* c, uint32_t rounds)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
+    if (rounds > kMaxGraphRounds) return set_err(PS5_ERR_ARG, "at most %u captured rounds", kMaxGraphRounds);
+    if (rounds != c->graph_rounds) {
+        HIPCHK(hipSetDevice(c->device));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+        if (c->graph) (void)hipGraphDestroy(c->graph);
+        c->graph_exec = nullptr;
+        c->graph = nullptr;
+        c->graph_key.clear();
+    }
+    c->graph_rounds = rounds;
+    return PS5_OK;
+}
+
+int ps5_ctx_get_graph_rounds(ps5_ctx* c)
+{
+    return c ? (int)c->graph_rounds : -1;
 }
 
 int ps5_ctx_set_profile(ps5_ctx* c, int on)

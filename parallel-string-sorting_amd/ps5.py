@@ -23,7 +23,8 @@ EXPORTS = [
     "ps5_sort_strings", "ps5_sort_strings_cache", "ps5_dist_chars_device",
     "ps5_ctx_create", "ps5_ctx_destroy", "ps5_ctx_reserve", "ps5_ctx_stream",
     "ps5_sort_device", "ps5_sort_device_noalloc", "ps5_noalloc_max_strings", "ps5_ctx_set_profile",
-    "ps5_ctx_set_planning", "ps5_ctx_get_planning", "ps5_ctx_kernel_count", "ps5_ctx_kernel_stat",
+    "ps5_ctx_set_planning", "ps5_ctx_get_planning", "ps5_ctx_set_graph_rounds",
+    "ps5_ctx_get_graph_rounds", "ps5_ctx_kernel_count", "ps5_ctx_kernel_stat",
     "ps5_ctx_work_stats",
     "ps5_dev_malloc", "ps5_dev_free", "ps5_memcpy_h2d", "ps5_memcpy_d2h", "ps5_device_synchronize",
     "ps5_dev_copy",
@@ -82,6 +83,8 @@ def load():
     _sig(lib, "ps5_ctx_set_profile", _I, [_P, _I])
     _sig(lib, "ps5_ctx_set_planning", _I, [_P, _I])
     _sig(lib, "ps5_ctx_get_planning", _I, [_P])
+    _sig(lib, "ps5_ctx_set_graph_rounds", _I, [_P, ctypes.c_uint32])
+    _sig(lib, "ps5_ctx_get_graph_rounds", _I, [_P])
     _sig(lib, "ps5_ctx_kernel_count", _I, [_P])
     _sig(lib, "ps5_ctx_kernel_stat", _I, [_P, _I, ctypes.POINTER(ctypes.c_char_p),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_U64),
@@ -404,6 +407,13 @@ class Context:
 
     def get_planning(self):
         return int(load().ps5_ctx_get_planning(self.h))
+
+    def set_graph_rounds(self, rounds):
+        """Capture a sort's first `rounds` rounds as one HIP graph, replayed while the shape repeats (0: off)."""
+        _check(load().ps5_ctx_set_graph_rounds(self.h, int(rounds)), "ps5_ctx_set_graph_rounds")
+
+    def get_graph_rounds(self):
+        return int(load().ps5_ctx_get_graph_rounds(self.h))
 
     def set_profile(self, on):
         _check(load().ps5_ctx_set_profile(self.h, 1 if on else 0), "ps5_ctx_set_profile")

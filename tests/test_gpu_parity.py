@@ -669,3 +669,124 @@ def test_host_lcp_merge_strings(ps5):
     el = [next((h for h in range(min(len(a), len(b))) if a[h] != b[h]), min(len(a), len(b)))
           for a, b in zip(out, out[1:])]
     assert lcp[0] == 42 and list(lcp[1:]) == el
+
+
+# ---------------------------------------------------------------- planning modes
+
+PLAN_MODES = {"host": (0, 0), "device": (1, 0), "graph2": (0, 2), "graph8": (1, 8)}
+
+
+def _plan_workload(ps5, kind):
+    if kind == "random":          # one root, three rounds at most
+        arena, offs = ps5.gen_uniform(0x5EED0101, 4 << 20, 1, 20, b"ACGT")
+        return arena, offs.astype(np.uint64), None
+    if kind == "highlcp":         # windows with long repeats: more rounds than a graph of 2 or 8 holds
+        arena, offs = ps5.workload("highlcp", scale=0.02)
+        return arena, offs.astype(np.uint64), None
+    arena, offs, sf = ps5.gen_batch(0x5EED0102, 300, 1, 20000)
+    return arena, offs, sf
+
+
+def _relabel(arena, variant):
+    """the same strings' shape with other characters (NULs stay NULs)"""
+    if variant == 0:
+        return arena
+    lut = np.arange(256, dtype=np.uint8)
+    lut[1:] = np.random.default_rng(variant).permutation(255).astype(np.uint8) + 1
+    return lut[arena]
+
+
+def _sort_in_modes(ps5, ctxs, arena, offs, sf, bufs):
+    da, do, dp, dl = bufs
+    n = len(offs)
+    da.upload(arena)
+    out = {}
+    for m, c in ctxs.items():
+        dp.upload(np.full(n, 0xFFFFFFFF, np.uint32))
+        dl.upload(np.full(n, 0xFFFFFFFF, np.uint32))
+        if sf is None:
+            do.upload(offs.astype(np.uint32))
+            c.sort_device(da, int(arena.size), do, n, dp, dl)
+        else:
+            do.upload(offs)
+            c.sort_batch_device(da, int(arena.size), do, sf, dp, dl)
+        out[m] = (dp.download(np.uint32, n), dl.download(np.uint32, n), c.work_stats())
+    return out
+
+
+@pytest.mark.parametrize("kind", ["random", "highlcp", "batch"])
+def test_planning_modes_agree(ps5, oracle, kind):
+    """levels planned on the host, on the device, and rounds captured in a
+    HIP graph (2 rounds, then round by round; 8 rounds) give identical
+    results; each graph is replayed for new contents of the same buffers, with
+    the other modes' sorts and synchronisations in between"""
+    arena, offs, sf = _plan_workload(ps5, kind)
+    n = len(offs)
+    say("planning modes", kind, n)
+    bufs = (ps5.DeviceBuffer(arena.size), ps5.DeviceBuffer(n * 8), ps5.DeviceBuffer(n * 4),
+            ps5.DeviceBuffer(n * 4))
+    ctxs = {}
+    try:
+        for m, (planning, rounds) in PLAN_MODES.items():
+            ctxs[m] = ps5.Context(0)
+            ctxs[m].set_planning(planning)
+            ctxs[m].set_graph_rounds(rounds)
+            assert ctxs[m].get_graph_rounds() == rounds
+        for variant in range(3):
+            a = _relabel(arena, variant)
+            out = _sort_in_modes(ps5, ctxs, a, offs, sf, bufs)
+            perm, lcp, _ = out["host"]
+            if sf is None:
+                assert oracle.verify(a, offs, perm, lcp)[0] == 0
+            else:
+                _check_batch(oracle, a, offs, sf, perm, lcp)
+            for m, (p, l, ws) in out.items():
+                assert np.array_equal(p, perm), f"{m} variant {variant}: permutation differs"
+                assert np.array_equal(l[1:], lcp[1:]), f"{m} variant {variant}: lcp differs"
+            if kind == "random":      # every round inside the graph: one synchronisation
+                assert out["graph8"][2]["syncs"] == 1, out["graph8"][2]
+    finally:
+        for c in ctxs.values():
+            c.close()
+        for b in bufs:
+            b.free()
+
+
+def test_graph_levels_fallback(ps5, oracle, monkeypatch):
+    """a captured round that needs more sample-sort levels than the graph
+    holds (forced: one level per round on 4.7M strings) is sorted again with
+    host-planned levels and gives the host-planned result, on every replay"""
+    arena, offs = ps5.gen_uniform(0x5EED0103, 40 << 20, 1, 16, ps5.ALPHA_DIGITS)
+    n = len(offs)
+    say("graph fallback", n)
+    monkeypatch.setenv("PS5_GRAPH_LEVELS", "1")
+    g = ps5.Context(0)
+    monkeypatch.delenv("PS5_GRAPH_LEVELS")
+    h = ps5.Context(0)
+    g.set_graph_rounds(4)
+    bufs = (ps5.DeviceBuffer(arena.size), ps5.DeviceBuffer(n * 4), ps5.DeviceBuffer(n * 4), ps5.DeviceBuffer(n * 4))
+    try:
+        for variant in range(2):
+            a = _relabel(arena, variant)
+            out = _sort_in_modes(ps5, {"host": h, "graph": g}, a, offs.astype(np.uint64), None, bufs)
+            perm, lcp, _ = out["host"]
+            assert oracle.verify(a, offs, perm, lcp)[0] == 0
+            p, l, ws = out["graph"]
+            assert np.array_equal(p, perm) and np.array_equal(l[1:], lcp[1:])
+            assert ws["syncs"] > 1, ws       # the host-planned sort after the graph
+    finally:
+        g.close()
+        h.close()
+        for b in bufs:
+            b.free()
+
+
+def test_graph_rounds_bounds(ps5):
+    c = ps5.Context(0)
+    try:
+        with pytest.raises(ps5.Ps5Error):
+            c.set_graph_rounds(65)
+        c.set_graph_rounds(0)
+        assert c.get_graph_rounds() == 0
+    finally:
+        c.close()
