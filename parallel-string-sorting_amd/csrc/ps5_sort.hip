@@ -1617,18 +1617,38 @@ __global__ __launch_bounds__(kWG) void k_leafbound(LeafLists LL, Bufs B, uint32_
 
 // Batch outputs: input index -> index inside its set (perm[pos] - first of
 // the set holding pos; a set occupies the same positions as its inputs).
+// A workgroup takes kLocTile positions: two lanes find the sets holding its
+// first and last position, and each position searches only between them
+// (no search at all in the common case of a tile inside one set).
+constexpr uint32_t kLocTile = 16 * kWG;
 __global__ __launch_bounds__(kWG) void k_localize(uint32_t* perm, uint32_t n, const uint64_t* set_first,
                                                    uint32_t nsets)
 {
-    const uint32_t pos = blockIdx.x * kWG + tid_x();
-    if (pos >= n) return;
-    uint32_t lo = 0, hi = nsets;   // largest s with set_first[s] <= pos
-    while (lo + 1 < hi) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (set_first[mid] <= pos) lo = mid;
-        else hi = mid;
+    __shared__ uint32_t s_set[2];
+    const uint32_t t0 = blockIdx.x * kLocTile;
+    if (t0 >= n) return;
+    const uint32_t t1 = (uint32_t)min<uint64_t>(n, (uint64_t)t0 + kLocTile);
+    if (tid_x() < 2) {
+        const uint32_t pos = tid_x() == 0 ? t0 : t1 - 1;
+        uint32_t lo = 0, hi = nsets;   // largest s with set_first[s] <= pos
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (set_first[mid] <= pos) lo = mid;
+            else hi = mid;
+        }
+        s_set[tid_x()] = lo;
     }
-    perm[pos] -= (uint32_t)set_first[lo];
+    __syncthreads();
+    const uint32_t s0 = s_set[0], s1 = s_set[1];
+    for (uint32_t pos = t0 + tid_x(); pos < t1; pos += kWG) {
+        uint32_t lo = s0, hi = s1 + 1;
+        while (lo + 1 < hi) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (set_first[mid] <= pos) lo = mid;
+            else hi = mid;
+        }
+        perm[pos] -= (uint32_t)set_first[lo];
+    }
 }
 
 // Distinguishing characters of a sorted order: cache[i] = s_i[lcp[i]] for
@@ -2592,6 +2612,12 @@ constexpr uint32_t kDevTreeWgs = 512, kDevCountWgs = 2048, kDevScanWgs = 1024, k
                    kDevTinyWgs = 8192, kDevRunWgs = 2048, kDevBoundWgs = 1024;
 static_assert(kBlkE0 == kBlkE1 && kBlkE1 == kBlkE2 && kBlkE2 == kBlkE3, "one thread count per 256 keys");
 
+// Leaf grids for nt tiny leaves and nr equal-key chunks: the leaf kernels
+// loop over their lists, and a bounded grid of them is far faster than a
+// workgroup per leaf when leaves number in the millions.
+uint32_t leaf_tiny_grid(uint64_t nt) { return (uint32_t)std::min<uint64_t>(kDevTinyWgs, div_up(nt, kWaves)); }
+uint32_t leaf_run_grid(uint64_t nr) { return (uint32_t)std::min<uint64_t>(kDevRunWgs, nr); }
+
 // Blocks of size class cls and round parity rp: the list entries [first,
 // first + count) (or, with dev_first, from the device counter C_BDONE + cls
 // to the list's count), `grid` workgroups.
@@ -2770,7 +2796,7 @@ int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t*
         if (nlarge == 0 && !c->profile && div_up(nt, kWaves) + nr >= kForkLeafGroups) {
             HIPCHK(hipEventRecord(c->ev_fork, st));
             HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-            launch_leaves(c, B, rp, nsink, depth, div_up(nt, kWaves), nr, nt, nr, c->side);
+            launch_leaves(c, B, rp, nsink, depth, leaf_tiny_grid(nt), leaf_run_grid(nr), nt, nr, c->side);
             HIPCHK(hipEventRecord(c->ev_join, c->side));
             *leaves_forked = true;
         }
@@ -2823,7 +2849,7 @@ int host_round_end(ps5_ctx* c, const Bufs& B, uint32_t round, bool leaves_forked
                 c->h_cnt[C_BLOCK + 6 + rp], nr, (unsigned long long)c->st_levels);
     }
     if (leaves_forked) HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
-    else launch_leaves(c, B, rp, nsink, depth, div_up(nt, kWaves), nr, nt, nr, st);
+    else launch_leaves(c, B, rp, nsink, depth, leaf_tiny_grid(nt), leaf_run_grid(nr), nt, nr, st);
     if (nt + nb + nr && B.lcp) launch_leafbound(c, B, rp, depth, div_up((uint64_t)nt + nb + nr, kWG), nt + nb + nr);
     HIPCHK(hipGetLastError());
     return readback(c, c->w.l_round[np], C_LROUND + np);
@@ -3464,7 +3490,7 @@ int ps5_sort_batch_device(ps5_ctx* c, const uint8_t* d_arena, uint64_t arena_byt
         }
         HIPCHK(hipMemcpyAsync(c->d_sets, set_first, ((uint64_t)nsets + 1) * sizeof(uint64_t), hipMemcpyHostToDevice,
                               c->stream));
-        hipLaunchKernelGGL(k_localize, dim3(div_up(n, kWG)), dim3(kWG), 0, c->stream, d_perm, (uint32_t)n, c->d_sets,
+        hipLaunchKernelGGL(k_localize, dim3(div_up(n, kLocTile)), dim3(kWG), 0, c->stream, d_perm, (uint32_t)n, c->d_sets,
                            nsets);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
