@@ -2618,6 +2618,14 @@ static_assert(kBlkE0 == kBlkE1 && kBlkE1 == kBlkE2 && kBlkE2 == kBlkE3, "one thr
 uint32_t leaf_tiny_grid(uint64_t nt) { return (uint32_t)std::min<uint64_t>(kDevTinyWgs, div_up(nt, kWaves)); }
 uint32_t leaf_run_grid(uint64_t nr) { return (uint32_t)std::min<uint64_t>(kDevRunWgs, nr); }
 
+// Looping block kernels of size class cls: four workgroups per wave slot
+// of the GPU (32 per CU)
+uint32_t block_grid_full(const ps5_ctx* c, int cls)
+{
+    const uint32_t waves = (256u << cls) / (uint32_t)kBlkE0 / 64;
+    return (uint32_t)((uint64_t)c->scatter_wgs * 128 / waves);
+}
+
 // Blocks of size class cls and round parity rp: the list entries [first,
 // first + count) (or, with dev_first, from the device counter C_BDONE + cls
 // to the list's count), `grid` workgroups.
@@ -2764,6 +2772,22 @@ uint32_t dev_levels_for(const ps5_ctx* c, uint64_t m)
         ++L;
     }
     return std::max<uint32_t>(L, 1);
+}
+
+// Largest treebits a device-planned level `level` of a round gives its
+// segments, when the round's segments hold at most m strings: the bound of
+// dev_levels_for, level by level.  A segment above the bound only gets a
+// smaller tree than it would choose, and the next level splits it further.
+int dev_level_kmax(const ps5_ctx* c, uint64_t m, uint32_t level)
+{
+    const auto tb = [c](uint64_t x) {
+        return large_treebits((uint32_t)std::min<uint64_t>(x, 0xFFFFFFFFull), c->leaf_target, (int)c->max_treebits);
+    };
+    for (uint32_t l = 0; l < level; ++l) {
+        const int k = tb(m);
+        m = (m * 4 + (1ull << k) - 1) >> k;
+    }
+    return tb(m);
 }
 
 // The host-planned levels of a round from the large list of counter cur_ci
@@ -2933,13 +2957,14 @@ int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
                 const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
                 const bool rootlvl = round == 0 && level == 0;
                 dev_level(c, B, in_list, in_ci, level == 0 ? nl0 : deeper_large, S, sink, seed,
-                          fused_root && rootlvl, rootlvl, rootlvl ? (int)root_kmax : (int)c->max_treebits,
+                          fused_root && rootlvl, rootlvl, rootlvl ? (int)root_kmax : dev_level_kmax(c, mmax, level),
                           level > 0 ? rp : -1);
                 for (int cls = 0; cls < kBlkClasses; ++cls) {
-                    // enough workgroups for every wave slot of the GPU (32 per CU)
+                    // four workgroups for every wave slot of the GPU (32
+                    // per CU): blocks differ in size up to 2x, and the
+                    // dispatcher's later workgroups even out the load
                     const uint32_t minm = cls == 0 ? kTinyT + 1 : (128u << cls) + 1;
-                    const uint32_t waves = (256u << cls) / (uint32_t)kBlkE0 / 64;
-                    const uint64_t full = (uint64_t)c->scatter_wgs * 32 / waves;
+                    const uint64_t full = block_grid_full(c, cls);
                     const uint64_t known = level == 0 ? c->h_cnt[C_BLOCK + 2 * cls + rp] : 0;
                     const uint32_t g = (uint32_t)std::min<uint64_t>(full, known + S / minm + 1);
                     launch_block_class(c, B, rp, cls, 0, 0, true, g, nsink, depth);
@@ -3014,11 +3039,9 @@ void graph_round(ps5_ctx* c, SortRun& R, uint32_t round, uint32_t L, uint32_t nl
         const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
         const bool rootlvl = round == 0 && level == 0;
         dev_level(c, B, in_list, in_ci, rootlvl ? nl_root : any_large, S, sink, seed, R.fused_root && rootlvl,
-                  rootlvl, rootlvl ? (int)R.root_kmax : (int)c->max_treebits, level > 0 ? rp : -1);
+                  rootlvl, rootlvl ? (int)R.root_kmax : dev_level_kmax(c, R.root_max, level), level > 0 ? rp : -1);
         for (int cls = 0; cls < kBlkClasses; ++cls) {
-            const uint32_t waves = (256u << cls) / (uint32_t)kBlkE0 / 64;
-            const uint32_t g = (uint32_t)((uint64_t)c->scatter_wgs * 32 / waves);
-            launch_block_class(c, B, rp, cls, 0, 0, true, g, nsink, depth);
+            launch_block_class(c, B, rp, cls, 0, 0, true, block_grid_full(c, cls), nsink, depth);
         }
         in_ci = out_ci;
         in_list = c->w.l_lvl[level & 1];
