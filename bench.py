@@ -139,6 +139,27 @@ def bench_batch(ps5, args):
     batched_s = (time.perf_counter() - t) / args.steps
     perm = d_perm.download(np.uint32, n)
     lcp = d_lcp.download(np.uint32, n)
+    # the same batch as one captured HIP graph per call: the first rounds
+    # planned on the device and replayed (the first call captures)
+    ctx.set_planning(1)
+    ctx.set_graph_rounds(args.graph_rounds)
+    ctx.sort_batch_device(d_arena, int(arena.size), d_offs, sf, d_perm, d_lcp)
+    lib.ps5_device_synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.sort_batch_device(d_arena, int(arena.size), d_offs, sf, d_perm, d_lcp)
+    lib.ps5_device_synchronize()
+    graph_s = (time.perf_counter() - t) / args.steps
+    graph_work = ctx.work_stats()
+    graph_same = bool(np.array_equal(d_perm.download(np.uint32, n), perm))
+    gl = d_lcp.download(np.uint32, n)
+    for k in range(args.sets):
+        b, e = int(sf[k]), int(sf[k + 1])
+        if e - b > 1:
+            graph_same &= bool(np.array_equal(gl[b + 1:e], lcp[b + 1:e]))
+    del gl
+    ctx.set_graph_rounds(0)
+    ctx.set_planning(0)
     # plain path: one ps5_sort_device call per set (uint32 offsets local to the set)
     starts = np.array([int(offs[int(sf[k])]) for k in range(args.sets)], np.uint64)
     ends = np.append(starts[1:], np.uint64(arena.size - 16))      # sets are laid out back to back
@@ -189,6 +210,10 @@ def bench_batch(ps5, args):
            "plain_sorts_per_s": round(args.sets / plain_s, 2), "plain_ms": round(plain_s * 1e3, 3),
            "speedup": round(plain_s / batched_s, 2), "batched_chars_gbs": round(arena.size / batched_s / 1e9, 2),
            "identical_to_plain": same, "oracle_checked_sets": (args.sets + 15) // 16, "oracle_failures": bad,
+           "graph": {"rounds_captured": args.graph_rounds, "ms": round(graph_s * 1e3, 3),
+                     "sorts_per_s": round(args.sets / graph_s, 2), "host_syncs_per_sort": graph_work["syncs"],
+                     "identical_to_batched": graph_same,
+                     "note": "levels planned on the device, the first rounds replayed as one captured HIP graph"},
            "tuning": {"leaf_target": tuning[0], "max_treebits": tuning[1], "max_oversample": tuning[2],
                       "autotuned": autotune},
            "config": {"workload": f"{args.sets} sets, strings log-uniform [{args.count_lo}, {args.count_hi}], "
@@ -461,6 +486,8 @@ def main():
     ap.add_argument("--sets", type=int, default=4096)
     ap.add_argument("--count-lo", type=int, default=1000)
     ap.add_argument("--count-hi", type=int, default=2_000_000)
+    ap.add_argument("--graph-rounds", type=int, default=7,
+                    help="rounds of the batch captured in one HIP graph (strings of at most 48 characters: 7)")
     ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--req-lo", type=int, default=10_000)
     ap.add_argument("--req-hi", type=int, default=256 << 20)
