@@ -265,7 +265,10 @@ typedef struct ps5_server ps5_server;
  * free slot takes every waiting request that fits under hbm_budget bytes of
  * staged data (in arrival order; a request larger than the budget runs alone)
  * and at most max_batch_strings strings (0: no limit), and sorts them as one
- * batch; slots run concurrently on the device.  With max_batch_strings > 0
+ * batch; slots run concurrently on the device.  Admission is decided on the
+ * device: submitted requests wait in a ring in mapped host memory, and a
+ * kernel launched by the free slot claims them against a ledger of admitted
+ * and released bytes kept in device memory.  With max_batch_strings > 0
  * every slot allocates its device buffers for such a batch at creation (up to
  * min(hbm_budget / slots, 64 * max_batch_strings) characters), so that the
  * first requests do not wait for allocations.  No counterpart in the
@@ -292,6 +295,10 @@ int ps5_server_stats(ps5_server* srv, double* p50_ms, double* p99_ms, uint64_t* 
                      uint64_t* batches, double* throughput_gbs);
 /* Slots running now, and requests completed with an error so far. */
 int ps5_server_health(ps5_server* srv, uint32_t* live_slots, uint64_t* failed);
+/* The device-side admission ledger: requests admitted so far, and bytes
+ * admitted and not yet released (0 once every request is done).  No
+ * counterpart in the reference. */
+int ps5_server_ledger(ps5_server* srv, uint64_t* admitted, uint64_t* in_use_bytes);
 /* Test hook: the request that will receive id `id` (ids count up from 1 in
  * submission order) fails inside its batch as a device error would, which
  * exercises the failure isolation from the host.  No counterpart in the

@@ -2,13 +2,17 @@
 //
 // Requests (host string sets) are submitted from any thread.  A fixed number
 // of slots each own a sort context: its HIP stream and device work buffers.
-// A free slot takes every waiting request that fits under the HBM budget (in
-// arrival order, at least one) and sorts them together as one batch
-// (ps5_sort_batch_device: every request is its own set).  Slots run
-// concurrently, so requests that arrive while others are on the GPU are
-// admitted as soon as a slot and memory free up.  Each request's result
-// depends only on its own strings.  Requests are copied between their host
-// arrays and the slot's device buffers directly; device buffers only grow.
+// Admission is decided on the device (ps5_admit.hpp): submit appends the
+// request to a ring in mapped host memory, and a free slot launches k_admit,
+// which takes every waiting request that fits under the HBM budget (in
+// arrival order, at least one) from a ledger kept in device memory, and
+// writes the claim to the slot's mailbox.  The slot sorts the claim as one
+// batch (ps5_sort_batch_device: every request is its own set) and reports
+// the bytes it releases.  Slots run concurrently, so requests that arrive
+// while others are on the GPU are admitted as soon as a slot and memory free
+// up.  Each request's result depends only on its own strings.  Requests are
+// copied between their host arrays and the slot's device buffers directly;
+// device buffers only grow.
 //
 // Failure isolation (the role of psstest's fork-per-run status
 // classification, src/psstest.cpp:310-370): a malformed request is rejected
@@ -31,6 +35,7 @@
 #include <vector>
 
 #include "../../include/ps5_mi355x.h"
+#include "ps5_admit.hpp"
 #include "ps5_internal.hpp"
 
 namespace {
@@ -62,11 +67,21 @@ struct ps5_server {
     uint64_t warm_strings = 0;        // per-slot buffers allocated up front
     uint64_t warm_bytes = 0;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
-    std::deque<Request*> queue;
+    std::condition_variable cv_work, cv_done, cv_space;
     std::unordered_map<uint64_t, Request*> reqs;
     uint64_t next_id = 1;
-    uint64_t in_use = 0;              // device bytes held by running batches
+    // admission state (ps5_admit.hpp): the submission ring in mapped host
+    // memory, the ledger in device memory, one claim mailbox per slot
+    ps5::AdmitIn* ain = nullptr;          // host view
+    ps5::AdmitIn* ain_dev = nullptr;      // device view
+    ps5::AdmitSlot* aslot = nullptr;      // host view of kAdmitSlots mailboxes
+    ps5::AdmitSlot* aslot_dev = nullptr;
+    ps5::AdmitLedger* ledger = nullptr;   // device memory
+    std::vector<Request*> by_seq;         // the request of sequence number q at q % kAdmitRing
+    uint64_t published = 0;               // requests in the ring (its tail)
+    uint64_t claimed = 0;                 // requests claimed by slots, as far as the host has seen
+    uint64_t release_epoch = 0;           // batches that released their bytes
+    uint64_t in_use_seen = 0;             // bytes in use after the latest claim (from the device)
     bool stop = false;
     uint32_t live_slots = 0;          // slots that started and have not stopped
     uint32_t starting = 0;            // slots still setting up
@@ -157,10 +172,10 @@ void fail_all_locked(ps5_server* s)
     s->dead = true;
     const auto now = Clock::now();
     const std::string why = "no serving slot could start: " + s->slot_error;
-    while (!s->queue.empty()) {
-        complete_locked(s, s->queue.front(), PS5_ERR_HIP, why, now, now);
-        s->queue.pop_front();
-    }
+    for (uint64_t q = s->claimed; q < s->published; ++q)
+        complete_locked(s, s->by_seq[q % ps5::kAdmitRing], PS5_ERR_HIP, why, now, now);
+    s->claimed = s->published;
+    s->cv_space.notify_all();
 }
 
 // Stage the requests of `batch` straight from their host arrays into the
@@ -234,10 +249,10 @@ int sort_batch(ps5_ctx* ctx, SlotBuffers& D, const std::vector<Request*>& batch,
     return PS5_OK;
 }
 
-// One slot: admit a batch (every waiting request that fits the budget, in
-// arrival order), sort it; when the batch fails, run its requests again one
-// by one so that only a request that fails alone reports an error.
-void run_slot(ps5_server* s)
+// One slot: claim a batch on the device (every waiting request that fits the
+// budget, in arrival order), sort it; when the batch fails, run its requests
+// again one by one so that only a request that fails alone reports an error.
+void run_slot(ps5_server* s, uint32_t slot)
 {
     ps5_ctx* ctx = nullptr;
     const int rc0 = ps5_ctx_create(&ctx, s->device);
@@ -256,6 +271,8 @@ void run_slot(ps5_server* s)
         s->cv_work.notify_all();
         return;
     }
+    hipStream_t st = (hipStream_t)ps5_ctx_stream(ctx);
+    volatile ps5::AdmitSlot* box = s->aslot + slot;
     SlotBuffers D;
     // warm start: buffers for one batch of the configured size, so that the
     // first requests do not pay for device allocations
@@ -265,32 +282,52 @@ void run_slot(ps5_server* s)
     }
     std::vector<uint64_t> set_first, set_base;
     for (;;) {
-        std::vector<Request*> batch;
-        uint64_t need = 0, nstr = 0;
+        uint64_t epoch;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] {
-                return s->stop || (!s->queue.empty() &&
-                                   (s->in_use == 0 || s->in_use + s->queue.front()->dev_bytes <= s->budget));
-            });
-            if (s->stop && s->queue.empty()) break;
-            // admit in arrival order while the budget allows (the first one
-            // always when nothing runs, so an oversized request still runs alone)
-            while (!s->queue.empty()) {
-                Request* r = s->queue.front();
-                const bool fits = s->in_use + need + r->dev_bytes <= s->budget;
-                if (!(fits || (batch.empty() && s->in_use == 0))) break;
-                if (!batch.empty() && nstr + r->n > s->max_batch) break;
-                batch.push_back(r);
-                need += r->dev_bytes;
-                nstr += r->n;
-                s->queue.pop_front();
-            }
-            s->in_use += need;
+            s->cv_work.wait(lk, [&] { return s->stop || s->published > s->claimed; });
+            if (s->published == s->claimed) break;     // stopping, and nothing waits
+            epoch = s->release_epoch;
         }
+        // the claim, decided on the device
+        int rc = ps5::admit_claim(s->ain_dev, s->ledger, s->aslot_dev + slot, st);
+        if (rc == PS5_OK && hipStreamSynchronize(st) != hipSuccess) rc = PS5_ERR_HIP;
+        if (rc != PS5_OK) {
+            // the slot cannot claim: it stops, as a slot that could not start
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->slot_error = ps5_last_error();
+            s->live_slots--;
+            if (s->live_slots == 0 && s->starting == 0) fail_all_locked(s);
+            s->cv_done.notify_all();
+            s->cv_work.notify_all();
+            D.release();
+            ps5_ctx_destroy(ctx);
+            return;
+        }
+        if (box->busy) {
+            std::this_thread::yield();
+            continue;
+        }
+        const uint64_t first = box->first, count = box->count, bytes = box->bytes, in_use = box->in_use;
+        if (count == 0) {
+            // nothing waits (another slot claimed it), or the next request
+            // does not fit until a running batch releases its bytes
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->release_epoch != epoch || s->claimed >= s->published; });
+            continue;
+        }
+        std::vector<Request*> batch;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (uint64_t q = first; q < first + count; ++q) batch.push_back(s->by_seq[q % ps5::kAdmitRing]);
+            s->claimed = std::max(s->claimed, first + count);
+            s->in_use_seen = in_use;
+        }
+        s->cv_work.notify_all();
+        s->cv_space.notify_all();
         const auto t_start = Clock::now();
         std::string why;
-        const int rc = sort_batch(ctx, D, batch, set_first, set_base, &why);
+        rc = sort_batch(ctx, D, batch, set_first, set_base, &why);
         std::vector<int> rcs(batch.size(), rc);
         std::vector<std::string> whys(batch.size(), why);
         if (rc != PS5_OK && batch.size() > 1) {
@@ -304,7 +341,9 @@ void run_slot(ps5_server* s)
         {
             std::lock_guard<std::mutex> lk(s->mu);
             for (size_t i = 0; i < batch.size(); ++i) complete_locked(s, batch[i], rcs[i], whys[i], t_start, t_done);
-            s->in_use -= need;
+            // the batch's bytes go back to the budget (read by the next claim)
+            __atomic_fetch_add(&s->ain->released, bytes, __ATOMIC_RELEASE);
+            s->release_epoch++;
             s->batches++;
         }
         s->cv_done.notify_all();
@@ -316,6 +355,16 @@ void run_slot(ps5_server* s)
         std::lock_guard<std::mutex> lk(s->mu);
         s->live_slots--;
     }
+}
+
+void free_admission(ps5_server* s)
+{
+    if (s->ain) (void)hipHostFree(s->ain);
+    if (s->aslot) (void)hipHostFree(s->aslot);
+    if (s->ledger) (void)hipFree(s->ledger);
+    s->ain = nullptr;
+    s->aslot = nullptr;
+    s->ledger = nullptr;
 }
 
 }  // namespace
@@ -330,12 +379,39 @@ int ps5_server_create(ps5_server** out, int device, uint64_t hbm_budget, uint32_
     s->device = device;
     s->budget = hbm_budget;
     s->max_batch = max_batch_strings ? max_batch_strings : (1ull << 31);
+    // admission state: ring and mailboxes in mapped host memory, the ledger
+    // in device memory (zeroed before any slot can claim)
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&s->ain), sizeof(ps5::AdmitIn),
+                          hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&s->ain_dev), s->ain, 0);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void**>(&s->aslot), ps5::kAdmitSlots * sizeof(ps5::AdmitSlot),
+                          hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&s->aslot_dev), s->aslot, 0);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->ledger), sizeof(ps5::AdmitLedger));
+    if (e == hipSuccess) e = hipMemset(s->ledger, 0, sizeof(ps5::AdmitLedger));
+    if (e != hipSuccess) {
+        // no device to admit on: the server starts without slots, and every
+        // request completes with this error (as when no slot can start)
+        free_admission(s);
+        s->slot_error = std::string("admission state: ") + hipGetErrorString(e);
+        s->dead = true;
+        *out = s;
+        return PS5_OK;
+    }
+    memset(s->ain, 0, sizeof(ps5::AdmitIn));
+    memset(s->aslot, 0, ps5::kAdmitSlots * sizeof(ps5::AdmitSlot));
+    s->ain->budget = hbm_budget;
+    s->ain->max_batch = s->max_batch;
+    s->by_seq.assign(ps5::kAdmitRing, nullptr);
     if (max_batch_strings) {    // each slot starts with buffers for a full batch
         s->warm_strings = max_batch_strings;
         s->warm_bytes = std::min<uint64_t>(hbm_budget / slots, max_batch_strings * 64);
     }
     s->starting = slots;
-    for (uint32_t i = 0; i < slots; ++i) s->workers.emplace_back(run_slot, s);
+    for (uint32_t i = 0; i < slots; ++i) s->workers.emplace_back(run_slot, s, i);
     *out = s;
     return PS5_OK;
 }
@@ -350,6 +426,7 @@ void ps5_server_destroy(ps5_server* s)
     s->cv_work.notify_all();
     for (auto& t : s->workers) t.join();
     for (auto& kv : s->reqs) delete kv.second;
+    free_admission(s);
     delete s;
 }
 
@@ -387,14 +464,23 @@ uint64_t ps5_server_submit(ps5_server* s, const uint8_t* arena, uint64_t arena_b
     r->dev_bytes = request_bytes(arena_bytes, n);
     r->t_submit = Clock::now();
     {
-        std::lock_guard<std::mutex> lk(s->mu);
+        std::unique_lock<std::mutex> lk(s->mu);
         r->id = s->next_id++;
         r->inject_fault = s->fault_ids.erase(r->id) != 0;
         s->reqs[r->id] = r;
         if (!s->any) { s->t_first = r->t_submit; s->any = true; }
-        if (s->dead) complete_locked(s, r, PS5_ERR_HIP, "no serving slot could start: " + s->slot_error,
-                                     r->t_submit, Clock::now());
-        else s->queue.push_back(r);
+        // the ring holds kAdmitRing requests that no slot has claimed yet
+        s->cv_space.wait(lk, [&] { return s->dead || s->published - s->claimed < ps5::kAdmitRing; });
+        if (s->dead) {
+            complete_locked(s, r, PS5_ERR_HIP, "no serving slot could start: " + s->slot_error, r->t_submit,
+                            Clock::now());
+        } else {
+            // the entry first, then the tail that makes it visible to k_admit
+            const uint64_t q = s->published++;
+            s->by_seq[q % ps5::kAdmitRing] = r;
+            s->ain->ring[q % ps5::kAdmitRing] = ps5::AdmitEntry{ r->dev_bytes, r->n };
+            __atomic_store_n(&s->ain->tail, q + 1, __ATOMIC_RELEASE);
+        }
     }
     s->cv_work.notify_all();
     s->cv_done.notify_all();
@@ -442,6 +528,19 @@ int ps5_server_health(ps5_server* s, uint32_t* live_slots, uint64_t* failed)
     std::lock_guard<std::mutex> lk(s->mu);
     if (live_slots) *live_slots = s->live_slots;
     if (failed) *failed = s->failed;
+    return PS5_OK;
+}
+
+int ps5_server_ledger(ps5_server* s, uint64_t* admitted, uint64_t* in_use_bytes)
+{
+    if (!s) return ps5::set_error(PS5_ERR_ARG, "ps5_server_ledger: server is NULL");
+    if (!s->ledger) return ps5::set_error(PS5_ERR_HIP, "ps5_server_ledger: no admission state (%s)", s->slot_error.c_str());
+    ps5::AdmitLedger L;
+    const hipError_t e = hipMemcpy(&L, s->ledger, sizeof(L), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return ps5::set_error(PS5_ERR_HIP, "ps5_server_ledger: %s", hipGetErrorString(e));
+    const uint64_t released = __atomic_load_n(&s->ain->released, __ATOMIC_ACQUIRE);
+    if (admitted) *admitted = L.head;
+    if (in_use_bytes) *in_use_bytes = L.admitted - released;
     return PS5_OK;
 }
 

@@ -32,7 +32,7 @@ EXPORTS = [
     "ps5_gen_uniform", "ps5_gen_windows", "ps5_gen_reference_random", "ps5_gen_prefixed",
     "ps5_sort_batch_device", "ps5_gen_batch_plan", "ps5_gen_batch_fill",
     "ps5_server_create", "ps5_server_destroy", "ps5_server_submit", "ps5_server_wait", "ps5_server_stats",
-    "ps5_server_health", "ps5_server_inject_fault",
+    "ps5_server_health", "ps5_server_ledger", "ps5_server_inject_fault",
     "ps5_ctx_set_tuning", "ps5_ctx_get_tuning", "ps5_autotune_device", "ps5_autotune_batch_device",
     "ps5_merge_device", "ps5_lcp_merge_strings",
 ]
@@ -111,6 +111,7 @@ def load():
     _sig(lib, "ps5_server_wait", _I, [_P, _U64, ctypes.POINTER(ctypes.c_double)])
     _sig(lib, "ps5_server_health", _I, [_P, ctypes.POINTER(_U32), ctypes.POINTER(_U64)])
     _sig(lib, "ps5_server_inject_fault", _I, [_P, _U64])
+    _sig(lib, "ps5_server_ledger", _I, [_P, ctypes.POINTER(_U64), ctypes.POINTER(_U64)])
     _sig(lib, "ps5_server_stats", _I, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(_U64), ctypes.POINTER(_U64), ctypes.POINTER(ctypes.c_double)])
     _sig(lib, "ps5_ctx_set_tuning", _I, [_P, _U32, _U32, _U32])
@@ -469,6 +470,12 @@ class Server:
         live, failed = _U32(), _U64()
         _check(load().ps5_server_health(self.h, ctypes.byref(live), ctypes.byref(failed)), "ps5_server_health")
         return {"live_slots": live.value, "failed": failed.value}
+
+    def ledger(self):
+        """the device-side admission ledger: requests admitted, bytes in use"""
+        adm, used = _U64(), _U64()
+        _check(load().ps5_server_ledger(self.h, ctypes.byref(adm), ctypes.byref(used)), "ps5_server_ledger")
+        return {"admitted": adm.value, "in_use_bytes": used.value}
 
     def inject_fault(self, rid):
         _check(load().ps5_server_inject_fault(self.h, rid), "ps5_server_inject_fault")

@@ -545,10 +545,13 @@ def test_serve_mix_equals_alone(ps5, ctx, oracle, budget_mb, slots, max_batch):
         ids = [srv.submit(a, o) for a, o in reqs]
         mixed = [srv.wait(i) for i in ids]
         st = srv.stats()
+        ledger = srv.ledger()
     finally:
         srv.close()
-    say("serve stats", st)
+    say("serve stats", st, "ledger", ledger)
     assert st["completed"] == len(reqs)
+    # admission ran on the device: every request admitted, every byte released
+    assert ledger == {"admitted": len(reqs), "in_use_bytes": 0}
     for (a, o), (pa, la), (pm, lm, ms) in zip(reqs, alone, mixed):
         assert np.array_equal(pa, pm)
         assert np.array_equal(la[1:], lm[1:])
@@ -581,10 +584,12 @@ def test_serve_failure_isolation(ps5, ctx, oracle):
                 assert rid == bad_id and "injected fault" in str(e)
                 out.append(None)
         health = srv.health()
+        ledger = srv.ledger()
     finally:
         srv.close()
-    say("health", health)
+    say("health", health, "ledger", ledger)
     assert health == {"live_slots": 2, "failed": 1}
+    assert ledger == {"admitted": len(reqs), "in_use_bytes": 0}
     for k, ((a, o), (pa, la), res) in enumerate(zip(reqs, alone, out)):
         if ids[k] == bad_id:
             assert res is None
