@@ -243,11 +243,31 @@ int ps5_sort_device_timed(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_b
 int ps5_ctx_set_tuning(ps5_ctx* ctx, uint32_t leaf_target, uint32_t max_treebits, uint32_t max_oversample);
 int ps5_ctx_get_tuning(ps5_ctx* ctx, uint32_t* leaf_target, uint32_t* max_treebits, uint32_t* max_oversample);
 
+/* Tile shapes and key width of a context's sorts (they never change the
+ * output).  No counterpart in the reference (its tiles are per-thread
+ * buckets and its key cache is one uint64, bingmann-sample_sort.hpp:49):
+ *   chunk_max      keys per wave chunk of a multi-workgroup level, in
+ *                  [2048, 9216] (default 9216): the unit of stable ranks
+ *                  and of the scatter's LDS reorder;
+ *   chunk_max_root the same in a level of roots, in [2048, 12032] (default
+ *                  12032; the root scatter stages 2-byte positions);
+ *   min_tiles      tiles a level is spread over at least, its chunks
+ *                  shrinking (not below 2048 keys) when it is small, in
+ *                  [1, 65536] (default 512);
+ *   leaf_keys      keys (8 characters each) a leaf pass fetches per string
+ *                  at a time, in [1, 8] (default 8). */
+int ps5_ctx_set_shape(ps5_ctx* ctx, uint32_t chunk_max, uint32_t chunk_max_root, uint32_t min_tiles,
+                      uint32_t leaf_keys);
+int ps5_ctx_get_shape(ps5_ctx* ctx, uint32_t* chunk_max, uint32_t* chunk_max_root, uint32_t* min_tiles,
+                      uint32_t* leaf_keys);
+
 /* Autotuner: sorts the given input (arguments as ps5_sort_device /
  * ps5_sort_batch_device, outputs written as by them) under candidate
- * settings -- coordinate descent over the three parameters, each candidate
- * timed as the best of two sorts -- keeps the fastest in the context and
- * returns it and its time.  No counterpart in the reference. */
+ * settings -- coordinate descent over the three tuning parameters, the three
+ * tile shapes and the key width, each candidate timed as the best of two
+ * sorts -- keeps the fastest in the context and returns its tuning
+ * parameters and its time (ps5_ctx_get_shape reads the chosen shape).  No
+ * counterpart in the reference. */
 int ps5_autotune_device(ps5_ctx* ctx, const uint8_t* d_arena, uint64_t arena_bytes,
                         const uint32_t* d_offs, uint64_t n, uint32_t* d_perm, uint32_t* d_lcp,
                         uint32_t* leaf_target, uint32_t* max_treebits, uint32_t* max_oversample,

@@ -64,18 +64,19 @@ __device__ __forceinline__ key64 load_key(const uint8_t* arena, uint64_t alen, u
 // past the string's NUL hold whatever follows it; callers use a key only
 // while every key before it was NUL-free.
 template <int KB>
-__device__ __forceinline__ void load_keys(const uint8_t* arena, uint64_t alen, uint64_t off, key64 (&k)[KB])
+__device__ __forceinline__ void load_keys(const uint8_t* arena, uint64_t alen, uint64_t off, key64 (&k)[KB],
+                                          int nk = KB)
 {
     const uint64_t a8 = off & ~7ull;
-    if (a8 + 8 * (KB + 1) > alen) {
+    if (a8 + 8 * (uint64_t)(nk + 1) > alen) {
 #pragma unroll
-        for (int i = 0; i < KB; ++i) k[i] = load_key(arena, alen, off + 8 * i);
+        for (int i = 0; i < KB; ++i) k[i] = i < nk ? load_key(arena, alen, off + 8 * i) : 0ull;
         return;
     }
     const g_u64* p = (const g_u64*)(arena + a8);
     uint64_t w[KB + 1];
 #pragma unroll
-    for (int i = 0; i <= KB; ++i) w[i] = p[i];
+    for (int i = 0; i <= KB; ++i) w[i] = i <= nk ? p[i] : 0ull;
     const uint32_t sh = (uint32_t)(off & 7) * 8;
 #pragma unroll
     for (int i = 0; i < KB; ++i) {

@@ -141,6 +141,7 @@ struct Bufs {
     uint32_t* bktrank;
     const uint32_t* pending;   // non-null and non-zero: large segments remain, the round's leaves wait
     uint32_t pending_err;      // ... and that is an error (ERR_LEVELS: a captured graph cannot continue the round)
+    uint32_t leaf_keys;        // keys (8 characters each) a leaf pass fetches per string at a time, 1..8
 };
 
 }  // namespace ps5

@@ -112,6 +112,7 @@ struct PlanArgs {
     unsigned char* plan;     // header, descriptors, tile table (tiles at tiles_off)
     uint64_t tiles_off, tiles_cap, hist_cap, trees_cap;
     uint32_t leaf_target, max_treebits, max_oversample;
+    uint32_t chunk_max, chunk_max_root, min_tiles;    // tile shapes (ps5_ctx_set_shape)
     uint32_t kmax;           // bound of the count launch (treebits are clamped to it)
     int done_rp;             // >= 0: mark the blocks of round parity done_rp listed so far as sorted
 };
@@ -150,8 +151,8 @@ __global__ __launch_bounds__(kBigWG) void k_plan(PlanArgs A, uint32_t* cnt)
     atomicAdd(&s_tot, tot);
     if (!ident) atomicAnd(&s_ident, 0u);
     __syncthreads();
-    const uint64_t cmax = s_ident ? kChunkMaxIdent : kChunkMax;
-    const uint64_t climit = min(cmax, max((uint64_t)kChunkMin, (s_tot + kWaves * kMinTiles - 1) / (kWaves * kMinTiles)));
+    const uint64_t cmax = s_ident ? A.chunk_max_root : A.chunk_max;
+    const uint64_t climit = min(cmax, max((uint64_t)kChunkMin, (s_tot + kWaves * A.min_tiles - 1) / (kWaves * A.min_tiles)));
     // descriptors: tile and histogram bases are exclusive scans over the
     // segments (blocks of kBigWG with carries); pad holds the tile base
     uint32_t tcarry = 0;
@@ -895,6 +896,8 @@ __device__ __forceinline__ void wave_lds_sync()
 // the position this lane's slot stands for (consecutive within a run), and
 // every lane of a run passes the run's first slot as run0.  A single leaf is
 // run0 = 0, pos = begin + lane.
+// NK: keys fetched per string at a time (1..8), or 0 for B.leaf_keys.
+template <int NK>
 __device__ void wave_finish_leaf(const Bufs& B, uint32_t pos, uint32_t m, key64 k, uint32_t id, uint32_t cur,
                                  bool write_bkey, uint32_t run0, key64* s_k, uint32_t* s_i, uint32_t* s_r)
 {
@@ -911,6 +914,8 @@ __device__ void wave_finish_leaf(const Bufs& B, uint32_t pos, uint32_t m, key64 
     bool first_pass = write_bkey;
     constexpr int kLeafKeys = 8;
     key64 kbuf[kLeafKeys] = {};     // string bid's next keys (depths cur+8, cur+16, ...)
+    // keys fetched per string at a time (the tuned key width of leaf passes)
+    const int nk = NK ? NK : (int)min(max(B.leaf_keys, 1u), (uint32_t)kLeafKeys);
     uint32_t bid = 0xFFFFFFFFu;
     int nbuf = 0;
     for (;;) {
@@ -967,29 +972,30 @@ __device__ void wave_finish_leaf(const Bufs& B, uint32_t pos, uint32_t m, key64 
         wave_lds_sync();
         if (act) s_r[lane] = live ? run : 0xFFFFFFFFu;
         // the next 8 characters of every string still in a run: fetched
-        // kLeafKeys keys at a time, so deep shared prefixes cost one memory
-        // round trip per 8 * kLeafKeys characters.  A lane's buffer belongs
+        // nk keys at a time, so deep shared prefixes cost one memory round
+        // trip per 8 * nk characters.  A lane's buffer belongs
         // to the string it was loaded for (bid); when a re-sort moved any
         // string to another lane, the wave loads afresh.
         first_pass = false;
         cur += 8;
         if (nbuf == 0 || __ballot(live && id != bid) != 0) {     // (uniform)
             if (live) {
-                if (id < B.n) load_keys<kLeafKeys>(B.arena, B.alen, str_off(B, id) + cur, kbuf);
+                if (id < B.n) load_keys<kLeafKeys>(B.arena, B.alen, str_off(B, id) + cur, kbuf, nk);
                 else { flag_error(B.cnt, ERR_BOUNDS); for (int i = 0; i < kLeafKeys; ++i) kbuf[i] = 0; }
                 bid = id;
             }
-            nbuf = kLeafKeys;
+            nbuf = nk;
             // depths where, in every run, all strings agree with the run's
             // first one and none ends decide nothing: skip them together
-            // (identical strings then cost one pass per 56 characters)
-            int skip = kLeafKeys - 1;
+            // (identical strings then cost one pass per 8 * (nk - 1) characters)
+            int skip = nk - 1;
 #pragma unroll
             for (int i = 0; i < kLeafKeys - 1; ++i) {
+                if (i >= nk - 1) break;     // (uniform)
                 const uint32_t lo = __shfl((uint32_t)kbuf[i], (int)run, 64);
                 const uint32_t hi = __shfl((uint32_t)(kbuf[i] >> 32), (int)run, 64);
                 const bool same = kbuf[i] == (((uint64_t)hi << 32) | lo) && (kbuf[i] & 0xFFull) != 0;
-                if (skip == kLeafKeys - 1 && __ballot(live && !same) != 0) skip = i;     // (uniform)
+                if (skip == nk - 1 && __ballot(live && !same) != 0) skip = i;     // (uniform)
             }
             for (int s = 0; s < skip; ++s) {     // (uniform)
 #pragma unroll
@@ -1007,6 +1013,7 @@ __device__ void wave_finish_leaf(const Bufs& B, uint32_t pos, uint32_t m, key64 
 }
 
 // One wave per tiny leaf (at most 64 strings), finished completely.
+template <int NK>
 __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint32_t* cnt_items, uint32_t cap,
                                                    Bufs B, uint32_t depth)
 {
@@ -1037,7 +1044,7 @@ __global__ __launch_bounds__(kWG) void k_leaf_tiny(const Item* items, const uint
             k = src ? B.key1[L.begin + lane] : B.key0[L.begin + lane];
             id = (L.flags & F_IDENT) ? L.begin + lane : src ? B.idx1[L.begin + lane] : B.idx0[L.begin + lane];
         }
-        wave_finish_leaf(B, L.begin + lane, m, k, id, depth, true, 0u, s_k[w], s_i[w], s_r[w]);
+        wave_finish_leaf<NK>(B, L.begin + lane, m, k, id, depth, true, 0u, s_k[w], s_i[w], s_r[w]);
     }
 }
 
@@ -1075,6 +1082,7 @@ constexpr int kBlkE0 = 4, kBlkE1 = 4, kBlkE2 = 4, kBlkE3 = 4;
 // packed into batches of at most 64 strings, and a wave finishes all runs of
 // a batch in one pass (duplicates come in many short runs).  The batch
 // table lives in s_k after the waves' leaf arrays.
+template <int NK>
 __device__ __forceinline__ void block_finish_runs(const Bufs& B, key64* s_k, const uint32_t* s_runs,
                                                   uint32_t nruns, uint32_t beg, uint32_t depth, uint32_t nwaves)
 {
@@ -1125,7 +1133,7 @@ __device__ __forceinline__ void block_finish_runs(const Bufs& B, key64* s_k, con
             kk = B.key0[pos];
             id = B.idx0[pos];
         }
-        wave_finish_leaf(B, pos, total, kk, id, depth + 8, false, slot0, wk, wi, wr);
+        wave_finish_leaf<NK>(B, pos, total, kk, id, depth + 8, false, slot0, wk, wi, wr);
     }
 }
 
@@ -1427,7 +1435,7 @@ struct BlockLds {
 // both give the stable (key, position) order.  Finishing a sorted position:
 // LCP to its neighbour, identical strings done, runs of equal keys without a
 // NUL handed to round r+1.
-template <uint32_t MAXM, int E>
+template <uint32_t MAXM, int E, int NK>
 __device__ __forceinline__ void block_one(const Item L, const Bufs& B, const Sink& nxt, uint32_t depth,
                                           BlockLds<MAXM, E>& S)
 {
@@ -1505,14 +1513,14 @@ __device__ __forceinline__ void block_one(const Item L, const Bufs& B, const Sin
     }
     __syncthreads();     // the runs' keys and indices written above are visible to the whole group
     const uint32_t nruns = min(s_nruns, kMaxRuns);
-    if (nruns != 0) block_finish_runs(B, s_k, s_runs, nruns, beg, depth, T / 64);     // (uniform)
+    if (nruns != 0) block_finish_runs<NK>(B, s_k, s_runs, nruns, beg, depth, T / 64);     // (uniform)
 }
 
 // Blocks [first, last) of a list (one per workgroup), or with LOOP (levels
 // planned on the device) from *p_first to the list's count, each workgroup
 // looping over them; a count over the list's capacity has been flagged and
 // is clamped.
-template <uint32_t MAXM, int E, bool LOOP>
+template <uint32_t MAXM, int E, bool LOOP, int NK>
 __global__ __launch_bounds__(MAXM / E) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_block(const Item* items, uint32_t first, uint32_t last,
                                                      const uint32_t* p_first, const uint32_t* cnt_items, uint32_t cap,
                                                      Bufs B, Sink nxt, uint32_t depth)
@@ -1521,13 +1529,13 @@ __global__ __launch_bounds__(MAXM / E) __attribute__((amdgpu_waves_per_eu(8, 8))
     if (!LOOP) {
         const uint32_t idx = first + blockIdx.x;
         if (idx >= min(last, cap)) return;
-        block_one<MAXM, E>(items[idx], B, nxt, depth, S);
+        block_one<MAXM, E, NK>(items[idx], B, nxt, depth, S);
         return;
     }
     const uint32_t lo = *p_first, hi = min(*cnt_items, cap);
     for (uint32_t idx = lo + blockIdx.x; idx < hi; idx += gridDim.x) {
         __syncthreads();     // the previous block's LDS is no longer read
-        block_one<MAXM, E>(items[idx], B, nxt, depth, S);
+        block_one<MAXM, E, NK>(items[idx], B, nxt, depth, S);
     }
 }
 
@@ -2074,6 +2082,11 @@ struct ps5_ctx {
     uint32_t leaf_target = 768;      // strings per block the multi-workgroup levels aim for
     uint32_t max_treebits = kMaxTreebits;   // splitter-tree depth limit of one level
     uint32_t max_oversample = 32;    // sample keys per splitter (limited by kMaxSamples)
+    // tile shapes of the multi-workgroup levels and the leaf passes' key width
+    uint32_t chunk_max = kChunkMax;          // keys per wave chunk, ordinary levels
+    uint32_t chunk_max_root = kChunkMaxIdent;    // the same in a level of roots
+    uint32_t min_tiles = kMinTiles;          // a small level's chunks shrink to give this many tiles
+    uint32_t leaf_keys = 8;                  // keys a leaf pass fetches per string at a time
     uint64_t seed = 0x5EED5EEDull;
     bool dev_levels = false;         // plan levels on the device: no host synchronisation inside a round
     // a sort's first rounds as one captured HIP graph (0: off), and the
@@ -2468,8 +2481,8 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
         total += c->w.h_large[i].size;
         all_ident = all_ident && (c->w.h_large[i].flags & F_IDENT);
     }
-    const uint64_t cmax = all_ident ? kChunkMaxIdent : kChunkMax;
-    const uint64_t climit = std::min<uint64_t>(cmax, std::max<uint64_t>(kChunkMin, div_up(total, kWaves * kMinTiles)));
+    const uint64_t cmax = all_ident ? c->chunk_max_root : c->chunk_max;
+    const uint64_t climit = std::min<uint64_t>(cmax, std::max<uint64_t>(kChunkMin, div_up(total, kWaves * c->min_tiles)));
     for (uint32_t i = 0; i < nlarge; ++i) {
         const Item seg = c->w.h_large[i];
         const uint32_t T = div_up(seg.size, climit * kWaves);
@@ -2639,9 +2652,16 @@ void launch_block_class(ps5_ctx* c, const Bufs& B, int rp, int cls, uint32_t fir
     const uint32_t* pf = dev_first ? c->d_cnt + C_BDONE + cls : nullptr;
     const uint32_t cap = c->w.cap_block;
     Launch L(c, K_BLOCK, 0);
-#define PS5_BLOCK_LAUNCH(M, E, LOOP)                                                                              \
-    hipLaunchKernelGGL((k_block<M, E, LOOP>), dim3(grid), dim3(M / E), 0, st, list, first, last, pf, cnt, cap, B, \
-                       nsink, depth)
+    // the default key width (8 keys) has its own instances; others share one
+#define PS5_BLOCK_LAUNCH(M, E, LOOP)                                                                               \
+    do {                                                                                                           \
+        if (c->leaf_keys == 8)                                                                                     \
+            hipLaunchKernelGGL((k_block<M, E, LOOP, 8>), dim3(grid), dim3(M / E), 0, st, list, first, last, pf,    \
+                               cnt, cap, B, nsink, depth);                                                         \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_block<M, E, LOOP, 0>), dim3(grid), dim3(M / E), 0, st, list, first, last, pf,    \
+                               cnt, cap, B, nsink, depth);                                                         \
+    } while (0)
     if (dev_first) {
         if (cls == 0) PS5_BLOCK_LAUNCH(256, kBlkE0, true);
         else if (cls == 1) PS5_BLOCK_LAUNCH(512, kBlkE1, true);
@@ -2664,8 +2684,12 @@ void launch_leaves(ps5_ctx* c, const Bufs& B, int rp, const Sink& nsink, uint32_
     if (g_tiny) {
         {
             Launch L(c, K_LEAF_TINY, 0);
-            hipLaunchKernelGGL(k_leaf_tiny, dim3(g_tiny), dim3(kWG), 0, s, c->w.l_tiny[rp], c->d_cnt + C_TINY + rp,
-                               c->w.cap_tiny, B, depth);
+            if (c->leaf_keys == 8)
+                hipLaunchKernelGGL(k_leaf_tiny<8>, dim3(g_tiny), dim3(kWG), 0, s, c->w.l_tiny[rp],
+                                   c->d_cnt + C_TINY + rp, c->w.cap_tiny, B, depth);
+            else
+                hipLaunchKernelGGL(k_leaf_tiny<0>, dim3(g_tiny), dim3(kWG), 0, s, c->w.l_tiny[rp],
+                                   c->d_cnt + C_TINY + rp, c->w.cap_tiny, B, depth);
         }
         sum_sizes(c, c->w.l_tiny[rp], 0, c->d_cnt + C_TINY + rp, (uint32_t)n_tiny, S_TINY);
     }
@@ -2719,6 +2743,9 @@ void dev_level(ps5_ctx* c, const Bufs& B, const Item* list, int ci, uint32_t bou
     A.leaf_target = c->leaf_target;
     A.max_treebits = c->max_treebits;
     A.max_oversample = c->max_oversample;
+    A.chunk_max = c->chunk_max;
+    A.chunk_max_root = c->chunk_max_root;
+    A.min_tiles = c->min_tiles;
     A.kmax = (uint32_t)kmax;
     A.done_rp = done_rp;
     {
@@ -3078,6 +3105,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     B.bktrank = c->w.bktrank;
     B.pending = nullptr;
     B.pending_err = 0;
+    B.leaf_keys = c->leaf_keys;
 
     // One root larger than a block: its keys are extracted by the first
     // classification pass itself, and the host knows the root without asking.
@@ -3135,7 +3163,8 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                                       (uint64_t)(uintptr_t)lcp, nsets, (uint64_t)(uintptr_t)c->w.key[0], c->w.cap,
                                       (uint64_t)(uintptr_t)c->w.trees, c->w.trees_cap, c->leaf_target,
                                       c->max_treebits, c->max_oversample, c->graph_rounds, c->seed,
-                                      c->w_gen, c->w.fixed ? 1u : 0u, c->graph_levels };
+                                      c->w_gen, c->w.fixed ? 1u : 0u, c->graph_levels, c->chunk_max,
+                                      c->chunk_max_root, c->min_tiles, c->leaf_keys };
         for (uint32_t s = 0; set_first && s <= nsets; ++s) key.push_back(set_first[s]);
         if ((rc = grow_level(c, c->w.cap_large, c->w.cap + c->w.cap / 4 + 64))) return rc;
         if (!c->graph_exec || key != c->graph_key) {
@@ -3557,6 +3586,33 @@ int ps5_ctx_set_tuning(ps5_ctx* c, uint32_t leaf_target, uint32_t max_treebits, 
     c->leaf_target = leaf_target;
     c->max_treebits = max_treebits;
     c->max_oversample = max_oversample;
+    return PS5_OK;
+}
+
+int ps5_ctx_set_shape(ps5_ctx* c, uint32_t chunk_max, uint32_t chunk_max_root, uint32_t min_tiles, uint32_t leaf_keys)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
+    if (chunk_max < (uint32_t)kChunkMin || chunk_max > (uint32_t)kChunkMax)
+        return set_err(PS5_ERR_ARG, "chunk_max must be in [%d, %d]", kChunkMin, kChunkMax);
+    if (chunk_max_root < (uint32_t)kChunkMin || chunk_max_root > (uint32_t)kChunkMaxIdent)
+        return set_err(PS5_ERR_ARG, "chunk_max_root must be in [%d, %d]", kChunkMin, kChunkMaxIdent);
+    if (min_tiles < 1 || min_tiles > 65536) return set_err(PS5_ERR_ARG, "min_tiles must be in [1, 65536]");
+    if (leaf_keys < 1 || leaf_keys > 8) return set_err(PS5_ERR_ARG, "leaf_keys must be in [1, 8]");
+    c->chunk_max = chunk_max;
+    c->chunk_max_root = chunk_max_root;
+    c->min_tiles = min_tiles;
+    c->leaf_keys = leaf_keys;
+    return PS5_OK;
+}
+
+int ps5_ctx_get_shape(ps5_ctx* c, uint32_t* chunk_max, uint32_t* chunk_max_root, uint32_t* min_tiles,
+                      uint32_t* leaf_keys)
+{
+    if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
+    if (chunk_max) *chunk_max = c->chunk_max;
+    if (chunk_max_root) *chunk_max_root = c->chunk_max_root;
+    if (min_tiles) *min_tiles = c->min_tiles;
+    if (leaf_keys) *leaf_keys = c->leaf_keys;
     return PS5_OK;
 }
 

@@ -33,7 +33,8 @@ EXPORTS = [
     "ps5_sort_batch_device", "ps5_gen_batch_plan", "ps5_gen_batch_fill",
     "ps5_server_create", "ps5_server_destroy", "ps5_server_submit", "ps5_server_wait", "ps5_server_stats",
     "ps5_server_health", "ps5_server_ledger", "ps5_server_inject_fault",
-    "ps5_ctx_set_tuning", "ps5_ctx_get_tuning", "ps5_autotune_device", "ps5_autotune_batch_device",
+    "ps5_ctx_set_tuning", "ps5_ctx_get_tuning", "ps5_ctx_set_shape", "ps5_ctx_get_shape",
+    "ps5_autotune_device", "ps5_autotune_batch_device",
     "ps5_merge_device", "ps5_lcp_merge_strings",
 ]
 
@@ -116,6 +117,8 @@ def load():
                                        ctypes.POINTER(_U64), ctypes.POINTER(_U64), ctypes.POINTER(ctypes.c_double)])
     _sig(lib, "ps5_ctx_set_tuning", _I, [_P, _U32, _U32, _U32])
     _sig(lib, "ps5_ctx_get_tuning", _I, [_P] + [ctypes.POINTER(_U32)] * 3)
+    _sig(lib, "ps5_ctx_set_shape", _I, [_P, _U32, _U32, _U32, _U32])
+    _sig(lib, "ps5_ctx_get_shape", _I, [_P] + [ctypes.POINTER(_U32)] * 4)
     _sig(lib, "ps5_autotune_device", _I, [_P, _P, _U64, _P, _U64, _P, _P] + [ctypes.POINTER(_U32)] * 3
          + [ctypes.POINTER(ctypes.c_double)])
     _sig(lib, "ps5_autotune_batch_device", _I, [_P, _P, _U64, _P, _P, _U32, _P, _P] + [ctypes.POINTER(_U32)] * 3
@@ -381,6 +384,15 @@ class Context:
     def get_tuning(self):
         v = [_U32() for _ in range(3)]
         _check(load().ps5_ctx_get_tuning(self.h, *[ctypes.byref(x) for x in v]), "ps5_ctx_get_tuning")
+        return tuple(x.value for x in v)
+
+    def set_shape(self, chunk_max=9216, chunk_max_root=12032, min_tiles=512, leaf_keys=8):
+        _check(load().ps5_ctx_set_shape(self.h, chunk_max, chunk_max_root, min_tiles, leaf_keys), "ps5_ctx_set_shape")
+
+    def get_shape(self):
+        """(chunk_max, chunk_max_root, min_tiles, leaf_keys)"""
+        v = [_U32() for _ in range(4)]
+        _check(load().ps5_ctx_get_shape(self.h, *[ctypes.byref(x) for x in v]), "ps5_ctx_get_shape")
         return tuple(x.value for x in v)
 
     def autotune_device(self, d_arena, arena_bytes, d_offs, n, d_perm, d_lcp=None):

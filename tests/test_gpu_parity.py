@@ -350,6 +350,48 @@ def test_tuning_does_not_change_output(ps5, oracle, tuning):
         c.close()
 
 
+@pytest.mark.parametrize("shape", [(2048, 2048, 1, 1), (4096, 6144, 2048, 2), (9216, 12032, 512, 4),
+                                   (6144, 9216, 65536, 8)])
+@pytest.mark.parametrize("planning", [0, 1])
+def test_shape_does_not_change_output(ps5, oracle, shape, planning):
+    """tile shapes (chunk limits, the smallest level's tile count) and the
+    leaf passes' key width, with host- and device-planned levels, sort to the
+    same exact output: random ASCII, short binary strings with many
+    duplicates, and windows with long shared prefixes"""
+    say("shape", shape, "planning", planning)
+    c = ps5.Context(0)
+    try:
+        c.set_shape(*shape)
+        c.set_planning(planning)
+        assert c.get_shape() == shape
+        arena, offs = ps5.workload("ascii2gb", scale=1 / 256)
+        perm, lcp = c.sort(arena, offs)
+        op, ol = oracle.ps5_sort(arena, offs)
+        assert np.array_equal(perm, op) and np.array_equal(lcp[1:], ol[1:])
+        strs = random_strings(50000, 0, 10, b"ab", 31)
+        a2, o2 = pack(strs)
+        p2, l2 = c.sort(a2, o2)
+        bp, bl = oracle.brute_sort(a2, o2)
+        assert np.array_equal(p2, bp) and np.array_equal(l2[1:], bl[1:])
+        a3, o3 = ps5.workload("highlcp", scale=0.01)
+        p3, l3 = c.sort(a3, o3)
+        assert oracle.verify(a3, o3, p3, l3)[0] == 0
+    finally:
+        c.close()
+
+
+def test_shape_rejects_bad_values(ps5):
+    c = ps5.Context(0)
+    try:
+        for bad in [(2047, 12032, 512, 8), (9217, 12032, 512, 8), (9216, 12033, 512, 8), (9216, 12032, 0, 8),
+                    (9216, 12032, 512, 0), (9216, 12032, 512, 9)]:
+            with pytest.raises(ps5.Ps5Error):
+                c.set_shape(*bad)
+        assert c.get_shape() == (9216, 12032, 512, 8)
+    finally:
+        c.close()
+
+
 def test_tuning_rejects_bad_values(ps5):
     c = ps5.Context(0)
     try:
