@@ -106,6 +106,12 @@ def cpu_baseline(ps5, workload, arena, offs, budget_s):
             "output_verified": bool(verified), "host": hi}
 
 
+def shape_of(ctx):
+    """the context's tile shapes and leaf key width"""
+    cm, cr, mt, lk = ctx.get_shape()
+    return {"chunk_max": cm, "chunk_max_root": cr, "min_tiles": mt, "leaf_keys": lk}
+
+
 def bench_batch(ps5, args):
     """Batched pass over many independent sets vs one device sort per set."""
     lib = ps5.load()
@@ -215,7 +221,7 @@ def bench_batch(ps5, args):
                      "identical_to_batched": graph_same,
                      "note": "levels planned on the device, the first rounds replayed as one captured HIP graph"},
            "tuning": {"leaf_target": tuning[0], "max_treebits": tuning[1], "max_oversample": tuning[2],
-                      "autotuned": autotune},
+                      **shape_of(ctx), "autotuned": autotune},
            "config": {"workload": f"{args.sets} sets, strings log-uniform [{args.count_lo}, {args.count_hi}], "
                                   "length U[4,48], alphabet ACGT/digits/bytes 33..126, seed 0x5EED0004"}}
     return out
@@ -644,7 +650,7 @@ def measure_plain(ps5, args, workload, steps, warmup, kernels):
            "config": {"workload": WORKLOADS[workload], "strings": n, "arena_bytes": chars,
                       "output": "permutation + LCP array", "device_resident": True,
                       "tuning": {"leaf_target": tuning[0], "max_treebits": tuning[1], "max_oversample": tuning[2],
-                                 "autotuned": bool(args.autotune)}}}
+                                 **shape_of(ctx), "autotuned": bool(args.autotune)}}}
     if kernels:
         res["roofline"] = kernel_roofline_of(ctx, args, d_arena, alen, d_offs, n, d_perm, d_lcp, workload,
                                              float(np.median(step_ms)))
