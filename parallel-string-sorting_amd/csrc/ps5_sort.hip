@@ -1073,7 +1073,16 @@ __device__ __forceinline__ void thread_sort(key64 (&k)[E], uint32_t (&p)[E])
     }
 }
 
-constexpr uint32_t kBlockRuns = 64;     // short runs a block finishes itself
+constexpr uint32_t kBlockRuns = 64;     // short runs a block finishes itself (per batch of runs)
+
+// Long runs (> kTinyT strings) a block in tail mode keeps to sort itself at
+// their next depth.  Pending runs are disjoint intervals of the block, so
+// at most kBlkT / (kTinyT + 1) = 31 wait at once.
+constexpr uint32_t kTailStack = 32;
+struct TailStack {
+    uint32_t pos[kTailStack], len[kTailStack], depth[kTailStack];
+    uint32_t n;
+};
 // keys per thread of the four block classes (256, 512, 1024, 2048 keys)
 constexpr int kBlkE0 = 4, kBlkE1 = 4, kBlkE2 = 4, kBlkE3 = 4;
 
@@ -1415,12 +1424,12 @@ __device__ __forceinline__ bool block_radix_sort(const key64* __restrict__ K, ui
 }
 
 // The block sort's LDS (one workgroup).
-template <uint32_t MAXM, int E>
+template <uint32_t MAXM, int E, bool TAIL>
 struct BlockLds {
     static constexpr uint32_t T = MAXM / E;
     key64 k[MAXM + 1];     // + 1: a merge step reads the slot after the one it takes
     uint16_t p[MAXM];
-    uint32_t runs[kBlockRuns];        // (start << 16) | length
+    uint32_t runs[TAIL ? MAXM / 2 : kBlockRuns];      // (start << 16) | length; in tail mode every short run
     uint32_t nruns;
     uint32_t cnt[MAXM + 2];           // radix partition: group counts, then starts
     alignas(8) uint32_t tmp[4 * (T / 64) + 4];
@@ -1435,14 +1444,21 @@ struct BlockLds {
 // both give the stable (key, position) order.  Finishing a sorted position:
 // LCP to its neighbour, identical strings done, runs of equal keys without a
 // NUL handed to round r+1.
-template <uint32_t MAXM, int E, int NK>
+//
+// TAIL (a round whose blocks are its last work): a run longer than kTinyT
+// is pushed on the workgroup's stack ts at depth + 8 instead of going to the
+// next round (k_block sorts it there), and every short run is finished here.
+// top: a block of the round's lists (false: a run of one, sorted deeper),
+// whose first and last keys at this depth go to bkey for k_leafbound.
+template <uint32_t MAXM, int E, int NK, bool TAIL>
 __device__ __forceinline__ void block_one(const Item L, const Bufs& B, const Sink& nxt, uint32_t depth,
-                                          BlockLds<MAXM, E>& S)
+                                          BlockLds<MAXM, E, TAIL>& S, TailStack* ts, bool top)
 {
     constexpr uint32_t T = MAXM / E;
     static_assert(T % 64 == 0 && MAXM <= 65536, "whole waves; positions fit 16 bits");
-    constexpr uint32_t kMaxRuns = kBlockRuns;    // short equal runs finished here (more go to the next round)
-    static_assert(MAXM * 8 >= 1024 * (T / 64) + 4 * (2 * kMaxRuns + 2),
+    // short equal runs finished here (more go to the next round; all of them in tail mode)
+    constexpr uint32_t kMaxRuns = TAIL ? MAXM / 2 : kBlockRuns;
+    static_assert(MAXM * 8 >= 1024 * (T / 64) + 4 * (2 * kBlockRuns + 2),
                   "the per-wave leaf arrays (1 KB each) and the run batches reuse s_k");
     key64* s_k = S.k;
     uint16_t* s_p = S.p;
@@ -1486,7 +1502,7 @@ __device__ __forceinline__ void block_one(const Item L, const Bufs& B, const Sin
             eqp = i > 0 && pk == kk;
             eqn = i + 1 < m && s_k[i + 1] == kk;
             finish_position(B, beg + i, ids[e], kk, i > 0, pk, !eqp && !eqn, depth);
-            if (i == 0 || i == m - 1) B.bkey[beg + i] = kk;
+            if (top && (i == 0 || i == m - 1)) B.bkey[beg + i] = kk;
         }
         // a run of equal keys without a NUL continues at depth + 8 (its keys
         // there are in place, finish_position fetched them): a run of at most
@@ -1508,34 +1524,74 @@ __device__ __forceinline__ void block_one(const Item L, const Bufs& B, const Sin
         const uint32_t slot = small ? atomicAdd(&s_nruns, 1u) : kMaxRuns;
         const bool keep = slot < kMaxRuns;
         if (keep) s_runs[slot] = (i << 16) | rl;
-        Item ni = { beg + i, rl, beg + i, 0u };
-        wave_append_segment(nr && !keep, ni, nxt);
+        if (TAIL) {
+            if (nr && !small) {
+                const uint32_t t = atomicAdd(&ts->n, 1u);
+                if (t < kTailStack) {
+                    ts->pos[t] = beg + i;
+                    ts->len[t] = rl;
+                    ts->depth[t] = depth + 8;
+                } else {
+                    flag_error(B.cnt, ERR_BOUNDS);     // (pending runs are disjoint: cannot happen)
+                }
+            }
+        } else {
+            Item ni = { beg + i, rl, beg + i, 0u };
+            wave_append_segment(nr && !keep, ni, nxt);
+        }
     }
     __syncthreads();     // the runs' keys and indices written above are visible to the whole group
     const uint32_t nruns = min(s_nruns, kMaxRuns);
-    if (nruns != 0) block_finish_runs<NK>(B, s_k, s_runs, nruns, beg, depth, T / 64);     // (uniform)
+    for (uint32_t r0 = 0; r0 < nruns; r0 += kBlockRuns) {     // (uniform) batches of at most kBlockRuns runs
+        block_finish_runs<NK>(B, s_k, s_runs + r0, min(kBlockRuns, nruns - r0), beg, depth, T / 64);
+        if (TAIL) __syncthreads();     // the next batch reuses the batch table
+    }
 }
 
 // Blocks [first, last) of a list (one per workgroup), or with LOOP (levels
 // planned on the device) from *p_first to the list's count, each workgroup
 // looping over them; a count over the list's capacity has been flagged and
 // is clamped.
-template <uint32_t MAXM, int E, bool LOOP, int NK>
+//
+// TAIL (host-planned, the round's last work: no large segments and no
+// equal-key chunks are left): the workgroup finishes its block completely,
+// sorting each long run again at its next depth until every string is
+// decided, so the sort needs no further round.
+template <uint32_t MAXM, int E, bool LOOP, int NK, bool TAIL>
 __global__ __launch_bounds__(MAXM / E) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_block(const Item* items, uint32_t first, uint32_t last,
                                                      const uint32_t* p_first, const uint32_t* cnt_items, uint32_t cap,
                                                      Bufs B, Sink nxt, uint32_t depth)
 {
-    __shared__ BlockLds<MAXM, E> S;
-    if (!LOOP) {
+    __shared__ BlockLds<MAXM, E, TAIL> S;
+    if constexpr (TAIL) {
+        static_assert(!LOOP, "tail mode is host-planned");
+        __shared__ TailStack ts;
         const uint32_t idx = first + blockIdx.x;
         if (idx >= min(last, cap)) return;
-        block_one<MAXM, E, NK>(items[idx], B, nxt, depth, S);
-        return;
-    }
-    const uint32_t lo = *p_first, hi = min(*cnt_items, cap);
-    for (uint32_t idx = lo + blockIdx.x; idx < hi; idx += gridDim.x) {
-        __syncthreads();     // the previous block's LDS is no longer read
-        block_one<MAXM, E, NK>(items[idx], B, nxt, depth, S);
+        if (tid_x() == 0) ts.n = 0;
+        // (block_one's barriers order this before its pushes)
+        Item cur = items[idx];
+        uint32_t d = depth;
+        for (bool top = true;; top = false) {     // one call site, so that block_one stays inline
+            block_one<MAXM, E, NK, true>(cur, B, nxt, d, S, &ts, top);
+            __syncthreads();     // the last pass's pushes are visible, its LDS is no longer read
+            const uint32_t n = min(ts.n, kTailStack);
+            if (n == 0) break;     // (uniform)
+            cur = Item{ ts.pos[n - 1], ts.len[n - 1], ts.pos[n - 1], 0u };
+            d = ts.depth[n - 1];
+            __syncthreads();     // every thread has read the top entry
+            if (tid_x() == 0) ts.n = n - 1;
+        }
+    } else if constexpr (!LOOP) {
+        const uint32_t idx = first + blockIdx.x;
+        if (idx >= min(last, cap)) return;
+        block_one<MAXM, E, NK, false>(items[idx], B, nxt, depth, S, nullptr, true);
+    } else {
+        const uint32_t lo = *p_first, hi = min(*cnt_items, cap);
+        for (uint32_t idx = lo + blockIdx.x; idx < hi; idx += gridDim.x) {
+            __syncthreads();     // the previous block's LDS is no longer read
+            block_one<MAXM, E, NK, false>(items[idx], B, nxt, depth, S, nullptr, true);
+        }
     }
 }
 
@@ -2096,6 +2152,7 @@ struct ps5_ctx {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<uint64_t> graph_key;
     uint64_t w_gen = 0;              // bumped whenever a work buffer is allocated again
+    bool tail_blocks = true;         // a round's last blocks finish every string (PS5_TAIL_BLOCKS=0: off)
     uint32_t graph_levels = 0;       // non-zero: fewer levels per captured round (PS5_GRAPH_LEVELS; tests
                                      // use it to reach the host-planned fallback)
     // profiling
@@ -2643,7 +2700,7 @@ uint32_t block_grid_full(const ps5_ctx* c, int cls)
 // first + count) (or, with dev_first, from the device counter C_BDONE + cls
 // to the list's count), `grid` workgroups.
 void launch_block_class(ps5_ctx* c, const Bufs& B, int rp, int cls, uint32_t first, uint32_t count, bool dev_first,
-                        uint32_t grid, const Sink& nsink, uint32_t depth)
+                        uint32_t grid, const Sink& nsink, uint32_t depth, bool tail = false)
 {
     const uint32_t last = first + count;
     hipStream_t st = c->stream;
@@ -2653,25 +2710,30 @@ void launch_block_class(ps5_ctx* c, const Bufs& B, int rp, int cls, uint32_t fir
     const uint32_t cap = c->w.cap_block;
     Launch L(c, K_BLOCK, 0);
     // the default key width (8 keys) has its own instances; others share one
-#define PS5_BLOCK_LAUNCH(M, E, LOOP)                                                                               \
+#define PS5_BLOCK_LAUNCH(M, E, LOOP, TAIL)                                                                         \
     do {                                                                                                           \
         if (c->leaf_keys == 8)                                                                                     \
-            hipLaunchKernelGGL((k_block<M, E, LOOP, 8>), dim3(grid), dim3(M / E), 0, st, list, first, last, pf,    \
-                               cnt, cap, B, nsink, depth);                                                         \
+            hipLaunchKernelGGL((k_block<M, E, LOOP, 8, TAIL>), dim3(grid), dim3(M / E), 0, st, list, first, last,  \
+                               pf, cnt, cap, B, nsink, depth);                                                     \
         else                                                                                                       \
-            hipLaunchKernelGGL((k_block<M, E, LOOP, 0>), dim3(grid), dim3(M / E), 0, st, list, first, last, pf,    \
-                               cnt, cap, B, nsink, depth);                                                         \
+            hipLaunchKernelGGL((k_block<M, E, LOOP, 0, TAIL>), dim3(grid), dim3(M / E), 0, st, list, first, last,  \
+                               pf, cnt, cap, B, nsink, depth);                                                     \
     } while (0)
     if (dev_first) {
-        if (cls == 0) PS5_BLOCK_LAUNCH(256, kBlkE0, true);
-        else if (cls == 1) PS5_BLOCK_LAUNCH(512, kBlkE1, true);
-        else if (cls == 2) PS5_BLOCK_LAUNCH(1024, kBlkE2, true);
-        else PS5_BLOCK_LAUNCH(kBlkT, kBlkE3, true);
+        if (cls == 0) PS5_BLOCK_LAUNCH(256, kBlkE0, true, false);
+        else if (cls == 1) PS5_BLOCK_LAUNCH(512, kBlkE1, true, false);
+        else if (cls == 2) PS5_BLOCK_LAUNCH(1024, kBlkE2, true, false);
+        else PS5_BLOCK_LAUNCH(kBlkT, kBlkE3, true, false);
+    } else if (tail) {
+        if (cls == 0) PS5_BLOCK_LAUNCH(256, kBlkE0, false, true);
+        else if (cls == 1) PS5_BLOCK_LAUNCH(512, kBlkE1, false, true);
+        else if (cls == 2) PS5_BLOCK_LAUNCH(1024, kBlkE2, false, true);
+        else PS5_BLOCK_LAUNCH(kBlkT, kBlkE3, false, true);
     } else {
-        if (cls == 0) PS5_BLOCK_LAUNCH(256, kBlkE0, false);
-        else if (cls == 1) PS5_BLOCK_LAUNCH(512, kBlkE1, false);
-        else if (cls == 2) PS5_BLOCK_LAUNCH(1024, kBlkE2, false);
-        else PS5_BLOCK_LAUNCH(kBlkT, kBlkE3, false);
+        if (cls == 0) PS5_BLOCK_LAUNCH(256, kBlkE0, false, false);
+        else if (cls == 1) PS5_BLOCK_LAUNCH(512, kBlkE1, false, false);
+        else if (cls == 2) PS5_BLOCK_LAUNCH(1024, kBlkE2, false, false);
+        else PS5_BLOCK_LAUNCH(kBlkT, kBlkE3, false, false);
     }
 #undef PS5_BLOCK_LAUNCH
 }
@@ -2865,9 +2927,13 @@ int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t*
         const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
         if (nlarge && (rc = large_step(c, B, nlarge, sink, seed, fused_root && round == 0 && level == 0)))
             return rc;
+        // a level without large segments whose round lists no equal-key
+        // chunks holds the round's last work: its blocks finish every
+        // string themselves (tail mode) and no further round is needed
+        const bool tail = c->tail_blocks && nlarge == 0 && c->h_cnt[C_RUN + rp] == 0;
         for (int cls = 0; cls < kBlkClasses; ++cls) {
             if (!nblk[cls]) continue;
-            launch_block_class(c, B, rp, cls, blk_done[cls], nblk[cls], false, nblk[cls], nsink, depth);
+            launch_block_class(c, B, rp, cls, blk_done[cls], nblk[cls], false, nblk[cls], nsink, depth, tail);
             sum_sizes(c, c->w.l_block[rp] + (uint64_t)cls * c->w.cap_block, blk_done[cls],
                       c->d_cnt + C_BLOCK + 2 * cls + rp, blk_done[cls] + nblk[cls], S_BLOCK);
             blk_done[cls] += nblk[cls];
@@ -3266,6 +3332,7 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     if (const char* dl = getenv("PS5_DEV_LEVELS")) c->dev_levels = atoi(dl) != 0;
     if (const char* gr = getenv("PS5_GRAPH_ROUNDS")) c->graph_rounds = std::min<uint32_t>(atoi(gr), kMaxGraphRounds);
     if (const char* gl = getenv("PS5_GRAPH_LEVELS")) c->graph_levels = (uint32_t)std::max(atoi(gl), 0);
+    if (const char* tb = getenv("PS5_TAIL_BLOCKS")) c->tail_blocks = atoi(tb) != 0;
     int ncu = 0;
     hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
     c->scatter_wgs = std::max<uint32_t>(kXcds, (uint32_t)std::max(ncu, 0) / kXcds * kXcds);

@@ -132,6 +132,25 @@ def test_duplicate_groups_in_blocks(ctx, oracle, copies):
     assert np.array_equal(perm, op) and np.array_equal(lcp[1:], ol[1:])
 
 
+@pytest.mark.parametrize("groups,seed", [(12, 5), (400, 6)])
+def test_tail_blocks_deep_runs(ctx, oracle, groups, seed):
+    """groups of 65..300 strings sharing prefixes of 20..600 characters, with
+    short differing tails and exact duplicates: in the round whose blocks are
+    its last work, each block sorts its long runs again at every deeper depth
+    itself (one round in all for the small set, which is a single block)"""
+    rng = np.random.default_rng(seed)
+    strs = []
+    for g in range(groups):
+        prefix = bytes(rng.integers(97, 99, size=int(rng.integers(20, 600))).astype(np.uint8))
+        tails = [bytes(rng.integers(97, 100, size=int(rng.integers(0, 12))).astype(np.uint8))
+                 for _ in range(int(rng.integers(65, 300)))]
+        strs += [prefix + t for t in tails] + [prefix] * int(rng.integers(0, 3))
+    rng.shuffle(strs)
+    say("tail blocks", groups, len(strs))
+    arena, offs = pack(strs)
+    check_against_brute(ctx, oracle, arena, offs)
+
+
 def shaped_set(seed):
     """A seeded random shape: string count (log-uniform in 30..300 000),
     alphabet (1 to 255 symbols, high bytes included), length range, a common
