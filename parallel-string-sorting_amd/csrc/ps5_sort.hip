@@ -1476,17 +1476,34 @@ __device__ __forceinline__ void block_one(const Item L, const Bufs& B, const Sin
     const uint32_t* I = src ? B.idx1 : B.idx0;
     const bool ident = L.flags & F_IDENT;
     if (tid == 0) s_nruns = 0;
+    // the block's input indices are loaded beside its keys and kept in
+    // registers through the sort, then staged in s_cnt (free after either
+    // sort), so that sorted positions read them from LDS instead of waiting
+    // for a second global gather
+    constexpr int kPerThread = MAXM / T;
+    uint32_t idv[kPerThread];
+    if (!ident) {
+#pragma unroll
+        for (int e = 0; e < kPerThread; ++e) idv[e] = I[beg + min(tid + (uint32_t)e * T, m - 1)];
+    }
     if (!block_radix_sort<MAXM, E>(K, beg, m, s_k, s_p, s_cnt, s_tmp))
         block_merge_sort<MAXM, E>(m, s_k, s_p);
-    // the input index of every sorted position, read where it lies before
-    // finish_position may overwrite those slots (continuing runs store theirs)
-    constexpr int kPerThread = MAXM / T;
+    if (!ident) {     // (uniform)
+#pragma unroll
+        for (int e = 0; e < kPerThread; ++e) {
+            const uint32_t i = tid + (uint32_t)e * T;
+            if (i < m) s_cnt[i] = idv[e];
+        }
+        __syncthreads();
+    }
+    // the input index of every sorted position, read before finish_position
+    // may overwrite its slot in idx0 (continuing runs store theirs)
     uint32_t ids[kPerThread];
 #pragma unroll
     for (int e = 0; e < kPerThread; ++e) {
         const uint32_t i = tid + (uint32_t)e * T;
         ids[e] = 0;
-        if (i < m) ids[e] = ident ? beg + s_p[i] : I[beg + s_p[i]];
+        if (i < m) ids[e] = ident ? beg + s_p[i] : s_cnt[s_p[i]];
     }
     __syncthreads();
 #pragma unroll
