@@ -1239,19 +1239,44 @@ __device__ __forceinline__ void block_merge_sort(uint32_t m, key64* s_k, uint16_
 constexpr uint32_t kRadixMaxGroup = 128;
 constexpr uint32_t kRadixMeanGroup = 16;
 
+// Exclusive block scan of a, the block totals of a and b and the block
+// maximum of b (one uint32 each per thread, NW waves), with one barrier.  The
+// caller writes s_tmp again only after another barrier.
 template <int NW>
-__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* s_tmp)
+__device__ __forceinline__ uint32_t block_scan_sum_max(uint32_t a, uint32_t b, uint32_t* tot_a, uint32_t* tot_b,
+                                                       uint32_t* max_b, uint32_t* s_tmp)
 {
+    const uint32_t lane = lane_id(), w = wave_id();
+    uint32_t x = a, sb = b, mb = b;
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, d, 64));
-    __syncthreads();
-    if (lane_id() == 0) s_tmp[wave_id()] = v;
-    __syncthreads();
-    uint32_t r = 0;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
 #pragma unroll
-    for (int i = 0; i < NW; ++i) r = max(r, s_tmp[i]);
+    for (int d = 32; d >= 1; d >>= 1) {
+        sb += (uint32_t)__shfl_xor((int)sb, d, 64);
+        mb = max(mb, (uint32_t)__shfl_xor((int)mb, d, 64));
+    }
+    if (lane == 63) {
+        s_tmp[w] = x;
+        s_tmp[NW + w] = sb;
+        s_tmp[2 * NW + w] = mb;
+    }
     __syncthreads();
-    return r;
+    uint32_t wpre = 0, ta = 0, tb = 0, mx = 0;
+#pragma unroll 1
+    for (int i = 0; i < NW; ++i) {     // (not unrolled: few registers for the 8-wave class)
+        const uint32_t t = s_tmp[i];
+        wpre += (i < (int)w) ? t : 0u;
+        ta += t;
+        tb += s_tmp[NW + i];
+        mx = max(mx, s_tmp[2 * NW + i]);
+    }
+    *tot_a = ta;
+    *tot_b = tb;
+    *max_b = mx;
+    return wpre + x - a;
 }
 
 __device__ __forceinline__ key64 shfl_xor64(key64 v, int d)
@@ -1308,7 +1333,7 @@ __device__ __forceinline__ bool block_radix_sort(const key64* __restrict__ K, ui
         kmin = min(kmin, s_mm[i]);
         kmax = max(kmax, s_mm[NW + i]);
     }
-    __syncthreads();
+    // (s_tmp is written again only after the counting barrier below)
     const key64 range = kmax - kmin;
     if ((range >> 48) != 0) {     // (uniform) offsets too wide to pack: merge sort
 #pragma unroll
@@ -1339,10 +1364,9 @@ __device__ __forceinline__ bool block_radix_sort(const key64* __restrict__ K, ui
         sum += c;
         sq += c * c;
     }
-    uint32_t tot, tsq;
-    const uint32_t over = block_max_u32<NW>(sq, s_tmp);
-    uint32_t run = block_excl_scan<NW>(sum, &tot, s_tmp);
-    (void)block_excl_scan<NW>(sq, &tsq, s_tmp);     // (all counts are read: starts may overwrite them)
+    uint32_t tot, tsq, over;
+    // (all counts are read before the scan's barrier: starts may overwrite them)
+    uint32_t run = block_scan_sum_max<NW>(sum, sq, &tot, &tsq, &over, s_tmp);
 #pragma unroll
     for (int j = 0; j < E; ++j) {
         const uint32_t c = s_cnt[tid * E + j];
