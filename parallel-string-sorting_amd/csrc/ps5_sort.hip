@@ -2194,6 +2194,8 @@ struct ps5_ctx {
     std::vector<uint64_t> graph_key;
     uint64_t w_gen = 0;              // bumped whenever a work buffer is allocated again
     bool tail_blocks = true;         // a round's last blocks finish every string (PS5_TAIL_BLOCKS=0: off)
+    bool chain_levels = true;        // a host-planned level's successor is planned on the device, without a
+                                     // read-back between them (PS5_CHAIN_LEVELS=0: off)
     uint32_t graph_levels = 0;       // non-zero: fewer levels per captured round (PS5_GRAPH_LEVELS; tests
                                      // use it to reach the host-planned fallback)
     // profiling
@@ -2715,7 +2717,8 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
 }
 
 constexpr uint32_t kMaxGraphRounds = 64;      // rounds one captured graph may hold
-constexpr uint32_t kForkLeafGroups = 8192;   // leaf workgroups worth a side stream
+constexpr uint32_t kForkLeafGroups = 8192;
+constexpr uint64_t kChainMin = 1u << 22;       // strings of a level whose successor is chained on the device   // leaf workgroups worth a side stream
 
 // Largest grids of the device-planned levels: their kernels loop over the
 // work the device lists, so these bound the workgroups of a launch.
@@ -2983,6 +2986,36 @@ int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t*
         // blocks only append to the next round's lists: after a level
         // without large segments this round has no more levels
         if (nlarge == 0) break;
+        uint64_t S = 0, mmax = 0;     // this level's strings and largest segment
+        for (uint32_t i = 0; c->chain_levels && i < nlarge; ++i) {
+            S += c->w.h_large[i].size;
+            mmax = std::max<uint64_t>(mmax, c->w.h_large[i].size);
+        }
+        if (c->chain_levels && S >= kChainMin) {
+            // the next level planned on the device at once, without reading
+            // this level's large list back: its segments hold at most this
+            // level's S strings and are no larger than its largest segment
+            // (below kChainMin strings the bounded grids of a device level
+            // cost more than the read-back they save)
+            const int nn = (nx_ci - C_LLVL) ^ 1;     // the level list the device level fills
+            Item* nn_large = c->w.l_lvl[nn];
+            const int nn_ci = C_LLVL + nn;
+            {
+                Launch L(c, K_MISC, 0);
+                hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, 1u << nn_ci);
+            }
+            ++level;
+            c->st_levels++;
+            const uint64_t seed2 = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
+            const int kmax = large_treebits((uint32_t)std::min<uint64_t>(mmax, 0xFFFFFFFFull), c->leaf_target,
+                                            (int)c->max_treebits);
+            dev_level(c, B, nx_large, nx_ci, (uint32_t)std::min<uint64_t>(c->w.cap_large, S / (kBlkT + 1) + 1), S,
+                      make_sink(c, nn_large, nn_ci, rp), seed2, false, false, kmax, -1);
+            HIPCHK(hipGetLastError());
+            if ((rc = readback(c, nn_large, nn_ci))) return rc;
+            cur_ci = nn_ci;
+            continue;
+        }
         if ((rc = readback(c, nx_large, nx_ci))) return rc;
         cur_ci = nx_ci;
     }
@@ -3243,7 +3276,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                                                                                (int)c->max_treebits));
         }
     }
-    if (dev && (rc = grow_level(c, c->w.cap_large, c->w.cap + c->w.cap / 4 + 64))) return rc;
+    if ((dev || c->chain_levels) && (rc = grow_level(c, c->w.cap_large, c->w.cap + c->w.cap / 4 + 64))) return rc;
     if (fused_root) {
         for (int i = 0; i < C_NCNT; ++i) c->h_cnt[i] = 0;
         c->h_cnt[C_LROUND] = 1;
@@ -3374,6 +3407,7 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     if (const char* gr = getenv("PS5_GRAPH_ROUNDS")) c->graph_rounds = std::min<uint32_t>(atoi(gr), kMaxGraphRounds);
     if (const char* gl = getenv("PS5_GRAPH_LEVELS")) c->graph_levels = (uint32_t)std::max(atoi(gl), 0);
     if (const char* tb = getenv("PS5_TAIL_BLOCKS")) c->tail_blocks = atoi(tb) != 0;
+    if (const char* cl = getenv("PS5_CHAIN_LEVELS")) c->chain_levels = atoi(cl) != 0;
     int ncu = 0;
     hipError_t e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
     c->scatter_wgs = std::max<uint32_t>(kXcds, (uint32_t)std::max(ncu, 0) / kXcds * kXcds);
