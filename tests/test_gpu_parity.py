@@ -847,6 +847,30 @@ def test_graph_levels_fallback(ps5, oracle, monkeypatch):
             b.free()
 
 
+def test_chained_levels_match(ps5, oracle, monkeypatch):
+    """a level of at least 4M strings has its successor planned on the
+    device without a read-back (chained); with chaining switched off the
+    output is the same, and both are exact"""
+    arena, offs = ps5.workload("ascii2gb", scale=1 / 12)
+    say("chained levels", len(offs))
+    on = ps5.Context(0)
+    monkeypatch.setenv("PS5_CHAIN_LEVELS", "0")
+    off = ps5.Context(0)
+    monkeypatch.delenv("PS5_CHAIN_LEVELS")
+    try:
+        p1, l1 = on.sort(arena, offs)
+        w1 = on.work_stats()
+        p0, l0 = off.sort(arena, offs)
+        w0 = off.work_stats()
+    finally:
+        on.close()
+        off.close()
+    say("work", w1, w0)
+    assert w1["syncs"] < w0["syncs"]
+    assert np.array_equal(p1, p0) and np.array_equal(l1[1:], l0[1:])
+    assert oracle.verify(arena, offs, p1, l1)[0] == 0
+
+
 def test_graph_rounds_bounds(ps5):
     c = ps5.Context(0)
     try:
