@@ -81,7 +81,8 @@ enum : int {
     C_RUN = 14,    // [2]
     C_ERR = 16,    // error flags
     C_BDONE = 17,  // [4] blocks of the current round already sorted, per size class (device-driven levels)
-    C_NCNT = 21
+    C_RUNCONT = 21,  // [2] equal-key leaves whose strings continue into the next round (round parity)
+    C_NCNT = 23
 };
 constexpr uint32_t ERR_CAP = 1u, ERR_LCP = 2u, ERR_BOUNDS = 4u;
 constexpr uint32_t ERR_LEVELS = 8u;          // a captured round had more levels than it was built with
@@ -122,6 +123,7 @@ struct Sink {            // where new items are appended
     Item* large; Item* block; Item* tiny; Item* run;
     uint32_t* cnt;       // counter base
     int c_large, c_block, c_tiny, c_run;     // counter indices (c_block: of size class 0)
+    int c_runcont;       // counter of equal-key leaves that continue (their key holds no NUL)
     uint32_t cap_large, cap_block, cap_tiny, cap_run;   // cap_block: per size class;
                          // class c lives at block + c * cap_block, counter c_block + 2 * c
 };

@@ -766,12 +766,15 @@ __global__ __launch_bounds__(kBigWG) void k_scatter(Plan P, Bufs B, const uint32
 // seg.size.  Tiny buckets become tiny leaves, other equal buckets (odd b)
 // equal-key leaves in chunks, other less-than buckets blocks or large
 // segments of the next level (distribute_finished, .hpp:1492-1584).
+//
+// spl: the segment's sorted splitters; equal bucket b holds the key spl[b >> 1],
+// and one whose key holds no NUL continues into the next round (counted).
 __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart, uint32_t stride, uint32_t base,
-                              const Sink& s, uint32_t* s_tmp)
+                              const key64* spl, const Sink& s, uint32_t* s_tmp)
 {
     constexpr int kPer = (kMaxBkt + kWG - 1) / kWG;   // 8
     const uint32_t dstbuf = (seg.flags & F_BUFB) ^ F_BUFB;
-    uint32_t c_t = 0, c_l = 0, c_r = 0, c_b[kBlkClasses] = {};
+    uint32_t c_t = 0, c_l = 0, c_r = 0, c_b[kBlkClasses] = {}, c_cont = 0;
     uint32_t st[kPer], sz[kPer];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
@@ -785,7 +788,10 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
             const uint32_t m = sz[j];
             if (m == 0) continue;
             if (m <= kTinyT) c_t++;
-            else if (b & 1) c_r += (m + kRunChunk - 1) / kRunChunk;
+            else if (b & 1) {
+                c_r += (m + kRunChunk - 1) / kRunChunk;
+                if ((spl[b >> 1] & 0xFFull) != 0) c_cont++;
+            }
             else if (m <= kBlkT) c_b[block_class(m)]++;
             else c_l++;
         }
@@ -796,6 +802,7 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
     uint32_t o_r = block_excl_scan<kWaves>(c_r, &tr, s_tmp);
 #pragma unroll
     for (int c = 0; c < kBlkClasses; ++c) o_b[c] = block_excl_scan<kWaves>(c_b[c], &tb[c], s_tmp);
+    if (c_cont) atomicAdd(&s.cnt[s.c_runcont], c_cont);
     if (tid_x() == 0) {
         s_tmp[8] = tt ? atomicAdd(&s.cnt[s.c_tiny], tt) : 0;
         s_tmp[9] = tl ? atomicAdd(&s.cnt[s.c_large], tl) : 0;
@@ -837,14 +844,15 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
     }
 }
 
-__global__ __launch_bounds__(kWG) void k_children(Plan P, const uint32_t* hscan, Sink s)
+__global__ __launch_bounds__(kWG) void k_children(Plan P, const uint32_t* hscan, const key64* trees, Sink s)
 {
     __shared__ uint32_t s_tmp[16];
     const uint32_t nlarge = P.hdr->nlarge;
     for (uint32_t si = blockIdx.x; si < nlarge; si += gridDim.x) {
         const LargeDesc D = P.desc[si];
         const uint32_t* h = hscan + D.hist_base;
-        emit_children(D.seg, (2 << D.k) - 1, h, D.T * kWaves, h[0], s, s_tmp);
+        emit_children(D.seg, (2 << D.k) - 1, h, D.T * kWaves, h[0], trees + (uint64_t)si * kTreeSlots + 1024, s,
+                      s_tmp);
         __syncthreads();     // s_tmp is reused by the next segment
     }
 }
@@ -1615,9 +1623,10 @@ __global__ __launch_bounds__(MAXM / E) __attribute__((amdgpu_waves_per_eu(8, 8))
         uint32_t d = depth;
         for (bool top = true;; top = false) {     // one call site, so that block_one stays inline
             block_one<MAXM, E, NK, true>(cur, B, nxt, d, S, &ts, top);
-            __syncthreads();     // the last pass's pushes are visible, its LDS is no longer read
+            // (the pushes precede a barrier in block_one's finish pass)
             const uint32_t n = min(ts.n, kTailStack);
             if (n == 0) break;     // (uniform)
+            __syncthreads();     // the last pass's LDS is no longer read
             cur = Item{ ts.pos[n - 1], ts.len[n - 1], ts.pos[n - 1], 0u };
             d = ts.depth[n - 1];
             __syncthreads();     // every thread has read the top entry
@@ -2488,6 +2497,7 @@ Sink make_sink(ps5_ctx* c, Item* large, int c_large, int rp)
     s.c_block = C_BLOCK + rp;
     s.c_tiny = C_TINY + rp;
     s.c_run = C_RUN + rp;
+    s.c_runcont = C_RUNCONT + rp;
     s.cap_large = c->w.cap_large;
     s.cap_block = c->w.cap_block;
     s.cap_tiny = c->w.cap_tiny;
@@ -2645,7 +2655,7 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     }
     {
         Launch L(c, K_CHILDREN, hl * 4 / kWaves);
-        hipLaunchKernelGGL(k_children, dim3(nlarge), dim3(kWG), 0, st, P, c->w.hist, sink);
+        hipLaunchKernelGGL(k_children, dim3(nlarge), dim3(kWG), 0, st, P, c->w.hist, c->w.trees, sink);
     }
     c->st_segments += nlarge;
     return PS5_OK;
@@ -2888,7 +2898,8 @@ void dev_level(ps5_ctx* c, const Bufs& B, const Item* list, int ci, uint32_t bou
     }
     {
         Launch L(c, K_CHILDREN, 0);
-        hipLaunchKernelGGL(k_children, dim3(std::min(gl, kDevChildWgs)), dim3(kWG), 0, st, P, c->w.hist, sink);
+        hipLaunchKernelGGL(k_children, dim3(std::min(gl, kDevChildWgs)), dim3(kWG), 0, st, P, c->w.hist, c->w.trees,
+                           sink);
     }
 }
 
@@ -2971,10 +2982,11 @@ int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t*
         const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
         if (nlarge && (rc = large_step(c, B, nlarge, sink, seed, fused_root && round == 0 && level == 0)))
             return rc;
-        // a level without large segments whose round lists no equal-key
-        // chunks holds the round's last work: its blocks finish every
-        // string themselves (tail mode) and no further round is needed
-        const bool tail = c->tail_blocks && nlarge == 0 && c->h_cnt[C_RUN + rp] == 0;
+        // a level without large segments, in a round whose equal-key
+        // leaves all end here (their keys hold a NUL), holds the round's
+        // last work: its blocks finish every string themselves (tail mode)
+        // and no further round is needed
+        const bool tail = c->tail_blocks && nlarge == 0 && c->h_cnt[C_RUNCONT + rp] == 0;
         for (int cls = 0; cls < kBlkClasses; ++cls) {
             if (!nblk[cls]) continue;
             launch_block_class(c, B, rp, cls, blk_done[cls], nblk[cls], false, nblk[cls], nsink, depth, tail);
@@ -3078,6 +3090,7 @@ int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
         Launch L(c, K_MISC, 0);
         uint32_t mask = (1u << (C_LROUND + np)) | (1u << (C_TINY + np)) | (1u << (C_RUN + np));
         for (int cls = 0; cls < kBlkClasses; ++cls) mask |= (1u << (C_BLOCK + 2 * cls + np)) | (1u << (C_BDONE + cls));
+        mask |= 1u << (C_RUNCONT + np);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, mask);
     }
     const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
@@ -3192,6 +3205,7 @@ void graph_round(ps5_ctx* c, SortRun& R, uint32_t round, uint32_t L, uint32_t nl
     if (round > 0) {
         uint32_t mask = (1u << (C_LROUND + np)) | (1u << (C_TINY + np)) | (1u << (C_RUN + np));
         for (int cls = 0; cls < kBlkClasses; ++cls) mask |= (1u << (C_BLOCK + 2 * cls + np)) | (1u << (C_BDONE + cls));
+        mask |= 1u << (C_RUNCONT + np);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, mask);
     }
     const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
