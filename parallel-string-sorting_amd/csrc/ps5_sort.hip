@@ -46,6 +46,22 @@ __global__ __launch_bounds__(kWG) void k_init(Bufs B, uint32_t n)
     B.key0[i] = load_key(B.arena, B.alen, str_off(B, i));
 }
 
+// Round 0 keys of the positions of listed items (the sets of a batch that
+// start as tiny leaves or blocks, when the first count pass extracts the keys
+// of the large ones), one workgroup per item at a time.
+__global__ __launch_bounds__(kWG) void k_init_items(const Item* items, uint32_t nitems, Bufs B)
+{
+    for (uint32_t ii = blockIdx.x; ii < nitems; ii += gridDim.x) {
+        const Item it = items[ii];
+        if (!item_ok(it, B)) {
+            if (tid_x() == 0) flag_error(B.cnt, ERR_BOUNDS);
+            continue;
+        }
+        for (uint32_t p = it.begin + tid_x(); p < it.begin + it.size; p += kWG)
+            B.key0[p] = load_key(B.arena, B.alen, str_off(B, p));
+    }
+}
+
 // Reset counters and place the root item.
 __global__ void k_setup(uint32_t* cnt, Item* list, int c_index, uint32_t n)
 {
@@ -3234,6 +3250,30 @@ void graph_round(ps5_ctx* c, SortRun& R, uint32_t round, uint32_t L, uint32_t nl
     if (B.lcp) launch_leafbound(c, B, rp, depth, (uint32_t)std::min<uint64_t>(kDevBoundWgs, div_up(n, kWG)), n);
 }
 
+// Round 0 keys the first classification pass does not extract: every
+// position's (no fused root), or those of a batch's sets that start as tiny
+// leaves or blocks (their lists and counts as setup_roots left them).
+int init_root_keys(ps5_ctx* c, const Bufs& B, uint32_t n, const uint64_t* set_first, bool fused_root)
+{
+    hipStream_t st = c->stream;
+    if (!fused_root) {
+        Launch L(c, K_INIT, (uint64_t)n * 20);   // offs 4 R + 8 characters R + key 8 W
+        hipLaunchKernelGGL(k_init, dim3(div_up(n, kWG)), dim3(kWG), 0, st, B, n);
+    } else if (set_first) {
+        Launch L(c, K_INIT, 0);
+        const uint32_t nt = c->h_upcnt[C_TINY];
+        if (nt) hipLaunchKernelGGL(k_init_items, dim3(std::min<uint32_t>(nt, 4096)), dim3(kWG), 0, st,
+                                   c->w.l_tiny[0], nt, B);
+        for (int cls = 0; cls < kBlkClasses; ++cls) {
+            const uint32_t nb = c->h_upcnt[C_BLOCK + 2 * cls];
+            if (nb) hipLaunchKernelGGL(k_init_items, dim3(std::min<uint32_t>(nb, 4096)), dim3(kWG), 0, st,
+                                       c->w.l_block[0] + (uint64_t)cls * c->w.cap_block, nb, B);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return PS5_OK;
+}
+
 int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* offs, const uint64_t* offs64,
              uint32_t n, uint32_t* perm, uint32_t* lcp, const uint64_t* set_first, uint32_t nsets)
 {
@@ -3261,18 +3301,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     B.pending_err = 0;
     B.leaf_keys = c->leaf_keys;
 
-    // One root larger than a block: its keys are extracted by the first
-    // classification pass itself, and the host knows the root without asking.
-    const bool fused_root = !set_first && n > kBlkT;
-    if (!fused_root) {
-        Launch L(c, K_INIT, (uint64_t)n * 20);   // offs 4 R + 8 characters R + key 8 W
-        hipLaunchKernelGGL(k_init, dim3(div_up(n, kWG)), dim3(kWG), 0, st, B, n);
-    }
-    if ((rc = setup_roots(c, n, set_first, nsets))) return rc;
-    HIPCHK(hipGetLastError());
-    // device-planned levels: the plan buffers take their largest size now
-    // (the levels' sizes are not known on the host)
-    bool dev = c->dev_levels;
+    // the roots (one, or a batch's sets) and their sizes
     uint32_t root_kmax = 1;                     // largest treebits of a large root
     uint64_t root_max = n;                      // largest root (no segment of any round is larger)
     uint64_t root_sum = n;                      // strings in large roots
@@ -3290,8 +3319,19 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
                                                                                (int)c->max_treebits));
         }
     }
+    // Roots larger than a block have their keys extracted by the first
+    // classification pass itself (its level holds only roots, whose indices
+    // are their positions); k_init covers the rest.  One such root alone is
+    // known to the host without asking.
+    const bool single_root = !set_first && n > kBlkT;
+    const bool fused_root = single_root || (set_first && root_sum > 0);
+    if ((rc = setup_roots(c, n, set_first, nsets))) return rc;
+    if ((rc = init_root_keys(c, B, n, set_first, fused_root))) return rc;
+    // device-planned levels: the plan buffers take their largest size now
+    // (the levels' sizes are not known on the host)
+    bool dev = c->dev_levels;
     if ((dev || c->chain_levels) && (rc = grow_level(c, c->w.cap_large, c->w.cap + c->w.cap / 4 + 64))) return rc;
-    if (fused_root) {
+    if (single_root) {
         for (int i = 0; i < C_NCNT; ++i) c->h_cnt[i] = 0;
         c->h_cnt[C_LROUND] = 1;
         c->w.h_large[0] = Item{ 0u, n, 0u, F_IDENT };
@@ -3348,11 +3388,9 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
         if (c->h_cnt[C_ERR] & ERR_LEVELS) {
             // a round needed more levels than the graph holds: sort again,
             // every level planned on the host
-            if (!fused_root) {
-                hipLaunchKernelGGL(k_init, dim3(div_up(n, kWG)), dim3(kWG), 0, st, B, n);
-            }
             if ((rc = setup_roots(c, n, set_first, nsets))) return rc;
-            if (fused_root) {
+            if ((rc = init_root_keys(c, B, n, set_first, fused_root))) return rc;
+            if (single_root) {
                 for (int i = 0; i < C_NCNT; ++i) c->h_cnt[i] = 0;
                 c->h_cnt[C_LROUND] = 1;
                 c->w.h_large[0] = Item{ 0u, n, 0u, F_IDENT };
