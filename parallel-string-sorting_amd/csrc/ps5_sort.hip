@@ -140,6 +140,7 @@ struct PlanArgs {
 __global__ __launch_bounds__(kBigWG) void k_plan(PlanArgs A, uint32_t* cnt)
 {
     __shared__ uint32_t s_tmp[kBigWaves];
+    __shared__ uint32_t s_tb[kBigWG];        // tile offsets of a block of segments, for its tile table
     __shared__ unsigned long long s_tot;
     __shared__ uint32_t s_ident, s_kmax;
     LevelHdr* hdr = reinterpret_cast<LevelHdr*>(A.plan);
@@ -196,23 +197,29 @@ __global__ __launch_bounds__(kBigWG) void k_plan(PlanArgs A, uint32_t* cnt)
             D.hist_base = (uint32_t)(hcarry + hex);
             desc[i] = D;
         }
+        // this block's entries of the tile table, written by the whole
+        // workgroup (within its capacity; an overflow is flagged below):
+        // tile t of the block belongs to the last segment whose offset is
+        // <= t (every segment has at least one tile)
+        s_tb[tid] = tex;
+        __syncthreads();
+        const uint32_t nseg = min(nl - b0, (uint32_t)kBigWG);
+        for (uint32_t t = tid; t < ttot && tcarry + t < A.tiles_cap; t += kBigWG) {
+            uint32_t lo = 0, hi = nseg;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_tb[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            tiles[tcarry + t] = TileRef{ b0 + lo, t - s_tb[lo] };
+        }
+        __syncthreads();     // s_tb is written again by the next block
         tcarry += ttot;
         hcarry += htot;
     }
     const bool fits = tcarry <= A.tiles_cap && hcarry <= A.hist_cap && hcarry <= 0xFFFFFFFFull && nl <= A.trees_cap;
     if (!fits && tid == 0) flag_error(cnt, ERR_CAP);
     const uint32_t ntiles = fits ? tcarry : 0u;
-    __syncthreads();     // every descriptor is written
-    // tile table: tile t belongs to the last segment whose tile base is <= t
-    for (uint32_t t = tid; t < ntiles; t += kBigWG) {
-        uint32_t lo = 0, hi = nl;            // desc[lo].pad <= t < desc[hi].pad (hi = nl: past the end)
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (desc[mid].pad <= t) lo = mid;
-            else hi = mid;
-        }
-        tiles[t] = TileRef{ lo, t - desc[lo].pad };
-    }
     if (tid == 0)
         *hdr = LevelHdr{ fits ? nl : 0u, ntiles, fits ? (uint32_t)hcarry : 0u, s_kmax, { 0u, 0u, 0u, 0u } };
 }

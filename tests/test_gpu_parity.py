@@ -857,6 +857,8 @@ def test_chained_levels_match(ps5, oracle, monkeypatch):
     monkeypatch.setenv("PS5_CHAIN_LEVELS", "0")
     off = ps5.Context(0)
     monkeypatch.delenv("PS5_CHAIN_LEVELS")
+    on.set_planning(0)     # chaining follows host-planned levels
+    off.set_planning(0)
     try:
         p1, l1 = on.sort(arena, offs)
         w1 = on.work_stats()
