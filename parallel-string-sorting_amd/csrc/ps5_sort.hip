@@ -1777,6 +1777,22 @@ __global__ __launch_bounds__(kWG) void k_localize(uint32_t* perm, uint32_t n, co
     }
     __syncthreads();
     const uint32_t s0 = s_set[0], s1 = s_set[1];
+    if (s0 == s1 && ((uintptr_t)perm & 15) == 0) {     // (uniform) one set: one base, 16-byte accesses
+        const uint32_t base = (uint32_t)set_first[s0];
+        for (uint32_t q = t0 + 4 * tid_x(); q < t1; q += 4 * kWG) {
+            if (q + 4 <= t1) {
+                uint4 v = *reinterpret_cast<const uint4*>(perm + q);
+                v.x -= base;
+                v.y -= base;
+                v.z -= base;
+                v.w -= base;
+                *reinterpret_cast<uint4*>(perm + q) = v;
+            } else {
+                for (uint32_t pos = q; pos < t1; ++pos) perm[pos] -= base;
+            }
+        }
+        return;
+    }
     for (uint32_t pos = t0 + tid_x(); pos < t1; pos += kWG) {
         uint32_t lo = s0, hi = s1 + 1;
         while (lo + 1 < hi) {
