@@ -3042,12 +3042,13 @@ int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t*
             S += c->w.h_large[i].size;
             mmax = std::max<uint64_t>(mmax, c->w.h_large[i].size);
         }
-        if (c->chain_levels && S >= kChainMin) {
+        if (c->chain_levels && !c->profile && S >= kChainMin) {
             // the next level planned on the device at once, without reading
             // this level's large list back: its segments hold at most this
             // level's S strings and are no larger than its largest segment
             // (below kChainMin strings the bounded grids of a device level
-            // cost more than the read-back they save)
+            // cost more than the read-back they save; a profiled sort plans
+            // every level on the host, which knows each launch's bytes)
             const int nn = (nx_ci - C_LLVL) ^ 1;     // the level list the device level fills
             Item* nn_large = c->w.l_lvl[nn];
             const int nn_ci = C_LLVL + nn;
