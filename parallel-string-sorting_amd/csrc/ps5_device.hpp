@@ -153,6 +153,33 @@ __device__ __forceinline__ uint64_t match_bucket(uint32_t b)
     return ((uint64_t)hi << 32) | lo;
 }
 
+// The same lanes for NB-bit ids with fewer vector instructions (all 64 lanes
+// must execute this): every lane ORs its bit (lanebit) into this wave's table
+// of 64-bit lane masks at the id's low TB bits, reads the entry back -- the
+// lanes agreeing on those bits -- and clears it, and ballots on the NB - TB
+// high bits keep the lanes agreeing on the whole id.  tab holds 1 << TB
+// entries, zero on entry and zero again on return.  The OR of the lanes' bits
+// does not depend on the order the LDS applies them in, and a wave's LDS
+// operations complete in issue order, so every read sees all of the ORs.
+template <int NB, int TB>
+__device__ __forceinline__ uint64_t match_bucket_lds(uint32_t b, uint64_t* tab, uint64_t lanebit)
+{
+    static_assert(TB >= 1 && TB <= NB, "table bits within the id");
+    uint64_t* e = tab + (b & ((1u << TB) - 1));
+    __hip_atomic_fetch_or(e, lanebit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    const uint64_t got = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    __hip_atomic_store(e, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    uint32_t lo = (uint32_t)got, hi = (uint32_t)(got >> 32);
+#pragma unroll
+    for (int i = TB; i < NB; ++i) {
+        const uint32_t m = (uint32_t)((int32_t)(b << (31 - i)) >> 31);
+        const uint64_t nbal = ~(uint64_t)__ballot(m != 0u);
+        lo &= (uint32_t)nbal ^ m;
+        hi &= (uint32_t)(nbal >> 32) ^ m;
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ void flag_error(uint32_t* cnt, uint32_t e) { atomicOr(&cnt[C_ERR], e); }
 
 // byte offset of input string i in the arena

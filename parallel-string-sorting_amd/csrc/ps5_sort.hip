@@ -352,12 +352,18 @@ __host__ __device__ constexpr uint32_t count_lds_bytes(int kmax)
     return kWaves * (2u << kmax) * 2u;
 }
 
-// NB: bits of a bucket id in this launch (kmax + 1, at least 8): the ballot
-// matching of ranks costs a few vector instructions per bit.
+// NB: bits of a bucket id in this launch (kmax + 1, at least 8).  Lanes of
+// one bucket are matched through a per-wave LDS table of lane masks on the
+// id's low kPeerBits bits and by ballots on the rest (a few vector
+// instructions per bit); ids of 11 bits use ballots only, since their
+// launch's LDS leaves no room for the tables at full occupancy.
+template <int NB>
+constexpr int kPeerBits = NB <= 10 ? (NB < 8 ? NB : 8) : 0;
+
 template <int NB>
 __device__ __forceinline__ void count_tile(uint32_t ti, const Plan& P, const Bufs& B, const key64* trees,
                                            uint32_t* hist, int from_arena, int kmax, key64* s_tree, key64* s_sorted,
-                                           uint16_t* s_hist_base)
+                                           uint16_t* s_hist_base, uint64_t* s_peer)
 {
     constexpr uint32_t kNoBkt = (1u << NB) - 1;   // id of inactive lanes, above every bucket of the launch
     const uint32_t nb2 = 2u << kmax;               // bucket slots per wave
@@ -379,6 +385,8 @@ __device__ __forceinline__ void count_tile(uint32_t ti, const Plan& P, const Buf
     const uint32_t t = tr.t;
     const uint32_t w = wave_id(), lane = lane_id();
     uint16_t* s_hist = s_hist_base + w * nb2;      // this wave's counts
+    uint64_t* peer_tab = s_peer + (w << kPeerBits<NB>);
+    const uint64_t lanebit = 1ull << lane;
     const uint32_t cb = (t * kWaves + w) * D.chunk;
     const uint32_t cmax = (seg.flags & F_IDENT) ? kChunkMaxIdent : kChunkMax;
     const uint32_t ce = (item_ok(seg, B) && t < D.T && D.chunk <= cmax) ? min(seg.size, cb + D.chunk) : 0u;
@@ -428,7 +436,9 @@ __device__ __forceinline__ void count_tile(uint32_t ti, const Plan& P, const Buf
 #pragma unroll
         for (int j = 0; j < kGroups; ++j) {
             const uint32_t p = g + 64 * j + lane;
-            const uint64_t peers = match_bucket<NB>(bv[j]);
+            uint64_t peers;
+            if constexpr (kPeerBits<NB> > 0) peers = match_bucket_lds<NB, kPeerBits<NB>>(bv[j], peer_tab, lanebit);
+            else peers = match_bucket<NB>(bv[j]);
             const uint32_t r = (uint32_t)__popcll(peers & lanemask_lt());
             if (p < ce) {
                 const uint32_t old = s_hist[bv[j]];
@@ -465,17 +475,20 @@ __global__ __launch_bounds__(kWG) void k_count(Plan P, Bufs B, const key64* tree
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ key64 s_tree[1 << (NB - 1)];        // trees of up to NB - 1 levels
     __shared__ key64 s_sorted[1 << (NB - 1)];
+    __shared__ uint64_t s_peer[kWaves << kPeerBits<NB>];     // lane-mask tables, kept zero between uses
     uint16_t* s_hist_base = reinterpret_cast<uint16_t*>(smem);
+    if constexpr (kPeerBits<NB> > 0)
+        for (int i = tid_x(); i < (kWaves << kPeerBits<NB>); i += kWG) s_peer[i] = 0;   // (a barrier follows in count_tile)
     const uint32_t N = P.hdr->ntiles, per = gridDim.x / kXcds, xcd = blockIdx.x % kXcds;
     const uint32_t thi = (uint32_t)((uint64_t)N * (xcd + 1) / kXcds);
     uint32_t ti = (uint32_t)((uint64_t)N * xcd / kXcds) + blockIdx.x / kXcds;
     if (!LOOP) {
-        if (ti < thi) count_tile<NB>(ti, P, B, trees, hist, from_arena, kmax, s_tree, s_sorted, s_hist_base);
+        if (ti < thi) count_tile<NB>(ti, P, B, trees, hist, from_arena, kmax, s_tree, s_sorted, s_hist_base, s_peer);
         return;
     }
     for (; ti < thi; ti += per) {
         __syncthreads();     // the previous tile's counts are written out
-        count_tile<NB>(ti, P, B, trees, hist, from_arena, kmax, s_tree, s_sorted, s_hist_base);
+        count_tile<NB>(ti, P, B, trees, hist, from_arena, kmax, s_tree, s_sorted, s_hist_base, s_peer);
     }
 }
 
