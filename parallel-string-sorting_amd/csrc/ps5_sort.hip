@@ -1654,7 +1654,9 @@ __global__ __launch_bounds__(MAXM / E) __attribute__((amdgpu_waves_per_eu(8, 8))
         const uint32_t idx = first + blockIdx.x;
         if (idx >= min(last, cap)) return;
         if (tid_x() == 0) ts.n = 0;
-        // (block_one's barriers order this before its pushes)
+        // (every thread reads ts.n after block_one, which returns at once on
+        // a malformed item: the count is stored before a barrier each time)
+        __syncthreads();
         Item cur = items[idx];
         uint32_t d = depth;
         for (bool top = true;; top = false) {     // one call site, so that block_one stays inline
@@ -1662,11 +1664,11 @@ __global__ __launch_bounds__(MAXM / E) __attribute__((amdgpu_waves_per_eu(8, 8))
             // (the pushes precede a barrier in block_one's finish pass)
             const uint32_t n = min(ts.n, kTailStack);
             if (n == 0) break;     // (uniform)
-            __syncthreads();     // the last pass's LDS is no longer read
+            __syncthreads();     // the last pass's LDS is no longer read, every thread has read ts.n
             cur = Item{ ts.pos[n - 1], ts.len[n - 1], ts.pos[n - 1], 0u };
             d = ts.depth[n - 1];
-            __syncthreads();     // every thread has read the top entry
-            if (tid_x() == 0) ts.n = n - 1;
+            if (tid_x() == 0) ts.n = n - 1;     // (pushes into the freed entry come after block_one's barriers)
+            __syncthreads();     // every thread has read the top entry; the count is stored
         }
     } else if constexpr (!LOOP) {
         const uint32_t idx = first + blockIdx.x;
