@@ -139,11 +139,14 @@ uint64_t ps5_noalloc_max_strings(void);
  * grids sized exactly.  1 (device): a kernel plans each level from the
  * device's own lists and every kernel loops over the work it finds, so a
  * round of levels runs without any host synchronisation (one per round, to
- * learn whether another round is needed).  Both give the same results.  No
- * counterpart in the reference (its levels are recursion on the CPU).  The
- * default is 0 unless the environment sets PS5_DEV_LEVELS=1. */
-int ps5_ctx_set_planning(ps5_ctx* ctx, int device);
-/* The current planning mode (0 or 1), -1 for a NULL context. */
+ * learn whether another round is needed).  2 (auto, the default): device
+ * planning for a single sort of fewer than 4M strings, host planning for
+ * larger sorts, batches and profiled sorts.  All modes give the same
+ * results.  No counterpart in the reference (its levels are recursion on
+ * the CPU).  The environment variable PS5_DEV_LEVELS=0/1/2 sets a new
+ * context's mode.  Returns PS5_ERR_ARG for any other mode. */
+int ps5_ctx_set_planning(ps5_ctx* ctx, int mode);
+/* The current planning mode (0, 1 or 2), -1 for a NULL context. */
 int ps5_ctx_get_planning(ps5_ctx* ctx);
 
 /* Capture the first `rounds` rounds of a sort (8 characters of depth each)

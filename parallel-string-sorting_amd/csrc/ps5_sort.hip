@@ -2248,7 +2248,9 @@ struct ps5_ctx {
     uint32_t min_tiles = kMinTiles;          // a small level's chunks shrink to give this many tiles
     uint32_t leaf_keys = 8;                  // keys a leaf pass fetches per string at a time
     uint64_t seed = 0x5EED5EEDull;
-    bool dev_levels = false;         // plan levels on the device: no host synchronisation inside a round
+    int planning = 2;                // 0: levels planned on the host, 1: on the device (no host synchronisation
+                                     // inside a round), 2: on the device for a single sort of fewer than
+                                     // kAutoDevMax strings, else on the host
     // a sort's first rounds as one captured HIP graph (0: off), and the
     // instantiated graph of the last sort, replayed while its inputs match
     uint32_t graph_rounds = 0;
@@ -2781,8 +2783,13 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
 }
 
 constexpr uint32_t kMaxGraphRounds = 64;      // rounds one captured graph may hold
-constexpr uint32_t kForkLeafGroups = 8192;
-constexpr uint64_t kChainMin = 1u << 22;       // strings of a level whose successor is chained on the device   // leaf workgroups worth a side stream
+constexpr uint32_t kForkLeafGroups = 8192;    // leaf workgroups worth a side stream
+constexpr uint64_t kChainMin = 1u << 22;       // strings of a level whose successor is chained on the device
+// Automatic planning plans a single sort of fewer strings on the device: there
+// the host synchronisations of host-planned levels outweigh the device plan's
+// bounded grids (0.24 vs 0.23 ms on 0.5 M strings, 0.32 vs 0.30 on 1.3 M),
+// from about 4 M strings on they no longer do (0.57 vs 0.58 ms on 6.4 M).
+constexpr uint64_t kAutoDevMax = 1u << 22;
 
 // Largest grids of the device-planned levels: their kernels loop over the
 // work the device lists, so these bound the workgroups of a launch.
@@ -2957,16 +2964,31 @@ void dev_level(ps5_ctx* c, const Bufs& B, const Item* list, int ci, uint32_t bou
     }
 }
 
+// Bound on the buckets of a level over segments of at most m strings (its
+// treebits in *k): 4x their mean below 16 sample keys per splitter (large
+// trees; keys repeated between two splitters can add to the spread of the
+// sample spacings, 2.9x has been seen at 8), 1 + 8/sqrt(s) from s = 16 on
+// (3x at 16, 2.5x at 32: small trees).  Below 4x, the bound shrinks with
+// every level of at least 3 splitters.
+uint64_t level_bucket_bound(const ps5_ctx* c, uint64_t m, int* k)
+{
+    *k = large_treebits((uint32_t)std::min<uint64_t>(m, 0xFFFFFFFFull), c->leaf_target, (int)c->max_treebits);
+    const uint32_t nspl = (1u << *k) - 1;
+    const uint32_t s = std::max<uint32_t>(2, std::min<uint32_t>(c->max_oversample, (kMaxSamples / nspl) & ~1u));
+    const uint64_t quarters = s < 16 ? 16 : 4 + (uint64_t)std::ceil(32.0 / std::sqrt((double)s));   // the factor, in quarters
+    return (m * quarters + (4ull << *k) - 1) / (4ull << *k);
+}
+
 // Levels a round needs when a segment may hold m strings, each level's
-// buckets assumed up to 4x their mean size (blocks are sorted by the level
-// that lists them, so no level of blocks alone is needed).
+// buckets bounded by level_bucket_bound (blocks are sorted by the level that
+// lists them, so no level of blocks alone is needed).  A round whose
+// segments outgrow the bound leaves them pending, and the host finishes them.
 uint32_t dev_levels_for(const ps5_ctx* c, uint64_t m)
 {
     uint32_t L = 0;
+    int k;
     while (m > kBlkT && L < 8) {
-        const int k = large_treebits((uint32_t)std::min<uint64_t>(m, 0xFFFFFFFFull), c->leaf_target,
-                                     (int)c->max_treebits);
-        m = (m * 4 + (1ull << k) - 1) >> k;
+        m = level_bucket_bound(c, m, &k);
         ++L;
     }
     return std::max<uint32_t>(L, 1);
@@ -2978,14 +3000,10 @@ uint32_t dev_levels_for(const ps5_ctx* c, uint64_t m)
 // smaller tree than it would choose, and the next level splits it further.
 int dev_level_kmax(const ps5_ctx* c, uint64_t m, uint32_t level)
 {
-    const auto tb = [c](uint64_t x) {
-        return large_treebits((uint32_t)std::min<uint64_t>(x, 0xFFFFFFFFull), c->leaf_target, (int)c->max_treebits);
-    };
-    for (uint32_t l = 0; l < level; ++l) {
-        const int k = tb(m);
-        m = (m * 4 + (1ull << k) - 1) >> k;
-    }
-    return tb(m);
+    int k;
+    for (uint32_t l = 0; l < level; ++l) m = level_bucket_bound(c, m, &k);
+    (void)level_bucket_bound(c, m, &k);
+    return k;
 }
 
 // The host-planned levels of a round from the large list of counter cur_ci
@@ -3368,7 +3386,9 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     if ((rc = init_root_keys(c, B, n, set_first, fused_root))) return rc;
     // device-planned levels: the plan buffers take their largest size now
     // (the levels' sizes are not known on the host)
-    bool dev = c->dev_levels;
+    bool dev = c->planning == 1 ||
+               (c->planning == 2 && !c->profile && !set_first && n < kAutoDevMax);   // (profiled sorts: host, whose
+                                                                                     // launches know their bytes)
     if ((dev || c->chain_levels) && (rc = grow_level(c, c->w.cap_large, c->w.cap + c->w.cap / 4 + 64))) return rc;
     if (single_root) {
         for (int i = 0; i < C_NCNT; ++i) c->h_cnt[i] = 0;
@@ -3494,7 +3514,7 @@ int ps5_ctx_create(ps5_ctx** out, int device)
     ps5_ctx* c = new ps5_ctx;
     c->device = device;
     c->trace = getenv("PS5_TRACE") != nullptr;
-    if (const char* dl = getenv("PS5_DEV_LEVELS")) c->dev_levels = atoi(dl) != 0;
+    if (const char* dl = getenv("PS5_DEV_LEVELS")) c->planning = std::min(std::max(atoi(dl), 0), 2);
     if (const char* gr = getenv("PS5_GRAPH_ROUNDS")) c->graph_rounds = std::min<uint32_t>(atoi(gr), kMaxGraphRounds);
     if (const char* gl = getenv("PS5_GRAPH_LEVELS")) c->graph_levels = (uint32_t)std::max(atoi(gl), 0);
     if (const char* tb = getenv("PS5_TAIL_BLOCKS")) c->tail_blocks = atoi(tb) != 0;
@@ -3858,16 +3878,17 @@ int ps5_ctx_get_tuning(ps5_ctx* c, uint32_t* leaf_target, uint32_t* max_treebits
     return PS5_OK;
 }
 
-int ps5_ctx_set_planning(ps5_ctx* c, int device)
+int ps5_ctx_set_planning(ps5_ctx* c, int mode)
 {
     if (!c) return set_err(PS5_ERR_ARG, "ctx is NULL");
-    c->dev_levels = device != 0;
+    if (mode < 0 || mode > 2) return set_err(PS5_ERR_ARG, "planning mode %d is not 0 (host), 1 (device) or 2 (auto)", mode);
+    c->planning = mode;
     return PS5_OK;
 }
 
 int ps5_ctx_get_planning(ps5_ctx* c)
 {
-    return c ? (c->dev_levels ? 1 : 0) : -1;
+    return c ? c->planning : -1;
 }
 
 int ps5_ctx_set_graph_rounds(ps5_ctx* c, uint32_t rounds)

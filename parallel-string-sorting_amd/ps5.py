@@ -414,9 +414,10 @@ class Context:
                "ps5_autotune_batch_device")
         return tuple(x.value for x in v), ms.value
 
-    def set_planning(self, device):
-        """0: levels planned on the host; 1: on the device (no synchronisation inside a round)."""
-        _check(load().ps5_ctx_set_planning(self.h, int(device)), "ps5_ctx_set_planning")
+    def set_planning(self, mode):
+        """0: levels planned on the host; 1: on the device (no synchronisation inside a round);
+        2 (the default): on the device for a single sort of fewer than 4M strings, else on the host."""
+        _check(load().ps5_ctx_set_planning(self.h, int(mode)), "ps5_ctx_set_planning")
 
     def get_planning(self):
         return int(load().ps5_ctx_get_planning(self.h))

@@ -739,7 +739,7 @@ def test_host_lcp_merge_strings(ps5):
 
 # ---------------------------------------------------------------- planning modes
 
-PLAN_MODES = {"host": (0, 0), "device": (1, 0), "graph2": (0, 2), "graph8": (1, 8)}
+PLAN_MODES = {"host": (0, 0), "device": (1, 0), "auto": (2, 0), "graph2": (0, 2), "graph8": (1, 8)}
 
 
 def _plan_workload(ps5, kind):
@@ -809,6 +809,8 @@ def test_planning_modes_agree(ps5, oracle, kind):
             for m, (p, l, ws) in out.items():
                 assert np.array_equal(p, perm), f"{m} variant {variant}: permutation differs"
                 assert np.array_equal(l[1:], lcp[1:]), f"{m} variant {variant}: lcp differs"
+            # automatic planning: the device's for a single sort of this size, the host's for a batch
+            assert out["auto"][2]["syncs"] == out["host" if sf is not None else "device"][2]["syncs"], kind
             if kind == "random":      # every round inside the graph: one synchronisation
                 assert out["graph8"][2]["syncs"] == 1, out["graph8"][2]
     finally:
@@ -880,5 +882,22 @@ def test_graph_rounds_bounds(ps5):
             c.set_graph_rounds(65)
         c.set_graph_rounds(0)
         assert c.get_graph_rounds() == 0
+    finally:
+        c.close()
+
+
+def test_planning_mode_values(ps5):
+    """modes 0, 1 and 2 are kept; others are rejected and leave the mode as it was"""
+    c = ps5.Context(0)
+    try:
+        env = os.environ.get("PS5_DEV_LEVELS")
+        assert c.get_planning() == (min(max(int(env), 0), 2) if env else 2)
+        for mode in (0, 1, 2):
+            c.set_planning(mode)
+            assert c.get_planning() == mode
+        for bad in (-1, 3):
+            with pytest.raises(ps5.Ps5Error):
+                c.set_planning(bad)
+            assert c.get_planning() == 2
     finally:
         c.close()
