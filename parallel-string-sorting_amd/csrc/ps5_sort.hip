@@ -1347,7 +1347,9 @@ __device__ __forceinline__ bool block_radix_sort(const key64* __restrict__ K, ui
     constexpr int NW = (int)(T / 64);
     constexpr int D = __builtin_ctz(MAXM);      // digit bits: MAXM groups
     static_assert((MAXM & (MAXM - 1)) == 0 && MAXM / T == (uint32_t)E, "a power-of-two block, E counters per thread");
-    const uint32_t tid = tid_x();
+    // (through an opaque move: indices derived from it are recomputed per
+    // block instead of being hoisted out of a tail workgroup's loop)
+    const uint32_t tid = scat_tid();
     const uint32_t Pm = (m + E - 1) / E * E;
     // w: this thread's keys (positions e*T + tid), later their packed values
     // every load is unconditional (clamped index), so all E are in flight at once
@@ -1435,56 +1437,36 @@ __device__ __forceinline__ bool block_radix_sort(const key64* __restrict__ K, ui
         if (i < m) s_v[s_cnt[(uint32_t)(w[e] >> sh)] + slot[e]] = w[e];
     }
     __syncthreads();
-    // sorted places: thread t ranks its E consecutive slots [tE, tE + E) in
-    // batches of RB.  Groups are ordered by their top bits, so every value
-    // before a batch's first group is smaller and every value after its last
-    // group larger: a slot's place is G0 plus the values of [G0, G1) (the
-    // batch's groups) below it, and one pass over [G0, G1) ranks the batch.
-    // Places wait in s_p (free while ranking: positions ride in the values).
-    constexpr int RB = E < 4 ? E : 4;
-    const uint32_t j0 = tid * E;
-#pragma unroll 1
-    for (int b0 = 0; b0 < E; b0 += RB) {
-        const uint32_t jb = j0 + (uint32_t)b0;
-        if (jb >= m) break;
-        const uint32_t jl = min(jb + (uint32_t)RB, m) - 1;
-        const uint32_t G0 = s_cnt[(uint32_t)(s_v[jb] >> sh)];
-        const uint32_t G1 = s_cnt[(uint32_t)(s_v[jl] >> sh) + 1];
-        key64 wb[RB];
-        uint32_t rb[RB];
+    // sorted places: thread t ranks the slots t + e T (consecutive lanes read
+    // consecutive values, free of LDS bank conflicts).  Groups are ordered
+    // by their top bits, so a slot's place is its group's start plus the
+    // group's values below it; value and place stay in registers until every
+    // value has been read.
+    uint32_t place[E];
 #pragma unroll
-        for (int e = 0; e < RB; ++e) {
-            wb[e] = s_v[min(jb + (uint32_t)e, m - 1)];
-            rb[e] = G0;
-        }
-        uint32_t q = G0;
+    for (int e = 0; e < E; ++e) {
+        if ((uint32_t)e * T >= m) break;     // (uniform) no slot of this pass is in the block
+        const uint32_t j = (uint32_t)e * T + tid;
+        const uint32_t jc = min(j, m - 1);
+        const key64 v = s_v[jc];
+        const uint32_t g = (uint32_t)(v >> sh);
+        const uint32_t G1 = s_cnt[g + 1];
+        uint32_t q = s_cnt[g], r = q;
         for (; q + 1 < G1; q += 2) {     // two values per step: their reads overlap
             const key64 va = s_v[q], vb = s_v[q + 1];
-#pragma unroll
-            for (int e = 0; e < RB; ++e) rb[e] += (va < wb[e] ? 1u : 0u) + (vb < wb[e] ? 1u : 0u);
+            r += (va < v ? 1u : 0u) + (vb < v ? 1u : 0u);
         }
-        if (q < G1) {
-            const key64 va = s_v[q];
-#pragma unroll
-            for (int e = 0; e < RB; ++e) rb[e] += va < wb[e] ? 1u : 0u;
-        }
-#pragma unroll
-        for (int e = 0; e < RB; ++e)
-            if (jb + (uint32_t)e < m) s_p[jb + e] = (uint16_t)rb[e];
+        if (q < G1) r += s_v[q] < v ? 1u : 0u;
+        w[e] = v;
+        place[e] = r;
     }
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const uint32_t j = min(j0 + (uint32_t)e, m - 1);
-        w[e] = s_v[j];
-        slot[e] = s_p[j];     // the place
-    }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        if (j0 + (uint32_t)e < m) {
-            s_k[slot[e]] = (w[e] >> 16) + kmin;
-            s_p[slot[e]] = (uint16_t)w[e];
+        if ((uint32_t)e * T >= m) break;     // (uniform)
+        if ((uint32_t)e * T + tid < m) {
+            s_k[place[e]] = (w[e] >> 16) + kmin;
+            s_p[place[e]] = (uint16_t)w[e];
         }
     }
     __syncthreads();
@@ -1538,7 +1520,8 @@ __device__ __forceinline__ void block_one(const Item L, const Bufs& B, const Sin
         if (tid_x() == 0) flag_error(B.cnt, ERR_BOUNDS);
         return;
     }
-    const uint32_t m = L.size, beg = L.begin, tid = tid_x();
+    const uint32_t m = L.size, beg = L.begin;
+    uint32_t tid = scat_tid();     // (opaque: see block_radix_sort)
     const int src = L.flags & F_BUFB;
     const key64* K = src ? B.key1 : B.key0;
     const uint32_t* I = src ? B.idx1 : B.idx0;
@@ -1556,6 +1539,7 @@ __device__ __forceinline__ void block_one(const Item L, const Bufs& B, const Sin
     }
     if (!block_radix_sort<MAXM, E>(K, beg, m, s_k, s_p, s_cnt, s_tmp))
         block_merge_sort<MAXM, E>(m, s_k, s_p);
+    tid = scat_tid();     // (a fresh copy: positions from it are not held through the sort)
     if (!ident) {     // (uniform)
 #pragma unroll
         for (int e = 0; e < kPerThread; ++e) {
