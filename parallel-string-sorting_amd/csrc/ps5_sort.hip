@@ -352,6 +352,22 @@ __host__ __device__ constexpr uint32_t count_lds_bytes(int kmax)
     return kWaves * (2u << kmax) * 2u;
 }
 
+// Load a read-only value that every lane of the wave reads at the same
+// address, through the constant address space (scalar loads).
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T* p)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole dwords");
+    using cword = const __attribute__((address_space(4))) uint32_t;
+    cword* q = (cword*)(uintptr_t)p;
+    uint32_t w[sizeof(T) / 4];
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(T) / 4; ++i) w[i] = q[i];
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
 // NB: bits of a bucket id in this launch (kmax + 1, at least 8).  Lanes of
 // one bucket are matched through a per-wave LDS table of lane masks on the
 // id's low kPeerBits bits and by ballots on the rest (a few vector
@@ -367,8 +383,9 @@ __device__ __forceinline__ void count_tile(uint32_t ti, const Plan& P, const Buf
 {
     constexpr uint32_t kNoBkt = (1u << NB) - 1;   // id of inactive lanes, above every bucket of the launch
     const uint32_t nb2 = 2u << kmax;               // bucket slots per wave
-    const TileRef tr = P.tiles[ti];
-    const LargeDesc D = P.desc[tr.seg];
+    ti = __builtin_amdgcn_readfirstlane(ti);
+    const TileRef tr = load_uniform(P.tiles + ti);     // (read-only here, the same for the workgroup)
+    const LargeDesc D = load_uniform(P.desc + tr.seg);
     const Item seg = D.seg;
     const int k = (int)D.k, nbkt = (2 << k) - 1;
     if (k > kmax || k + 1 > NB) {   // the launch must size LDS and ids for every tree of the level
@@ -644,10 +661,14 @@ __device__ __forceinline__ void scat_fetch(const LargeDesc* __restrict__ desc, c
                                            ScatChunk<ScatLds<IDENT>::kPer>& S)
 {
     const uint32_t tid = scat_tid();
-    const TileRef tr = tiles[c / kWaves];
+    // the chunk's tile and descriptor are read-only in this kernel and the
+    // same for the whole workgroup: scalar loads, counted apart from the
+    // vector loads and stores of the chunks
+    c = __builtin_amdgcn_readfirstlane(c);
+    const TileRef tr = load_uniform(tiles + c / kWaves);
     S.w = c % kWaves;
     S.t = tr.t;
-    S.D = desc[tr.seg];
+    S.D = load_uniform(desc + tr.seg);
     const Item seg = S.D.seg;
     S.cb = (S.t * kWaves + S.w) * S.D.chunk;
     const bool ok = item_ok(seg, B) && S.t < S.D.T && S.D.chunk <= ScatLds<IDENT>::kMax && S.cb < seg.size;
