@@ -288,7 +288,10 @@ typedef struct ps5_server ps5_server;
  * free slot takes every waiting request that fits under hbm_budget bytes of
  * staged data (in arrival order; a request larger than the budget runs alone)
  * and at most max_batch_strings strings (0: no limit), and sorts them as one
- * batch; slots run concurrently on the device.  Admission is decided on the
+ * batch; a request joins a batch after its first only while the batch then
+ * holds at most 2^20 strings, so larger requests run as batches of their own
+ * (batching them would save only the per-sort overhead and hold their
+ * results back for their batch mates).  Slots run concurrently on the device.  Admission is decided on the
  * device: submitted requests wait in a ring in mapped host memory, and a
  * kernel launched by the free slot claims them against a ledger of admitted
  * and released bytes kept in device memory.  With max_batch_strings > 0

@@ -18,6 +18,11 @@ namespace ps5 {
 constexpr uint32_t kAdmitRing = 1u << 16;     // requests submitted and not yet admitted, at most
 constexpr uint32_t kAdmitSlots = 16;
 constexpr uint32_t kAdmitSpin = 1u << 20;     // lock attempts before k_admit gives up (claims nothing)
+// A request joins a claim after its first only while the claim then holds at
+// most this many strings: larger requests run as batches of their own, since
+// batching them saves only the per-sort overhead (about 0.2 ms) and holds
+// their results back until their batch mates are sorted and transferred.
+constexpr uint64_t kJoinMaxStrings = 1u << 20;
 
 struct AdmitEntry {
     uint64_t dev_bytes;      // device memory the request holds while it runs
@@ -29,7 +34,7 @@ struct AdmitIn {
     uint64_t tail;           // requests submitted so far (sequence numbers 0 .. tail-1)
     uint64_t released;       // bytes of finished batches so far
     uint64_t budget;         // HBM bytes that admitted requests may hold at once
-    uint64_t max_batch;      // strings per batch
+    uint64_t max_batch;      // strings a claim may hold when a request joins it after its first
     AdmitEntry ring[kAdmitRing];   // request seq at seq % kAdmitRing
 };
 

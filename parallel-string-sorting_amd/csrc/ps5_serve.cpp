@@ -404,7 +404,7 @@ int ps5_server_create(ps5_server** out, int device, uint64_t hbm_budget, uint32_
     memset(s->ain, 0, sizeof(ps5::AdmitIn));
     memset(s->aslot, 0, ps5::kAdmitSlots * sizeof(ps5::AdmitSlot));
     s->ain->budget = hbm_budget;
-    s->ain->max_batch = s->max_batch;
+    s->ain->max_batch = std::min<uint64_t>(s->max_batch, ps5::kJoinMaxStrings);
     s->by_seq.assign(ps5::kAdmitRing, nullptr);
     if (max_batch_strings) {    // each slot starts with buffers for a full batch
         s->warm_strings = max_batch_strings;

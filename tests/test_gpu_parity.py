@@ -619,6 +619,24 @@ def test_serve_mix_equals_alone(ps5, ctx, oracle, budget_mb, slots, max_batch):
         assert oracle.verify(a, o, pm, lm)[0] == 0
 
 
+def test_serve_large_requests_run_alone(ps5, ctx, oracle):
+    """requests of more than 2^20 strings never share a batch (each is the
+    first and only request of its claim), whatever the batch limit allows"""
+    reqs = [ps5.gen_uniform(4000 + i, 6 << 20, 1, 8, b"ACGT") for i in range(3)]
+    assert all(len(o) > (1 << 20) for _, o in reqs)
+    say("serve large", [len(o) for _, o in reqs])
+    srv = ps5.Server(0, hbm_budget=4 << 30, slots=1, max_batch_strings=16 << 20)
+    try:
+        ids = [srv.submit(a, o) for a, o in reqs]
+        out = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["completed"] == len(reqs) and st["batches"] == len(reqs), st
+    for (a, o), (p, l, ms) in zip(reqs, out):
+        assert oracle.verify(a, o, p, l)[0] == 0
+
+
 def test_serve_failure_isolation(ps5, ctx, oracle):
     """a request whose sort fails inside a batch (a fault planted from the
     host) fails alone: its batch is run again request by request, every other
