@@ -79,14 +79,27 @@ __global__ void k_setup(uint32_t* cnt, Item* list, int c_index, uint32_t n)
 // min(cnt[ci], kPeek) items from byte kStageHead on.
 constexpr uint32_t kPeek = 2048;
 constexpr uint32_t kStageHead = 128;
-static_assert(C_NCNT * 4 <= kStageHead, "counters must fit the stage head");
-__global__ __launch_bounds__(kWG) void k_stage(const uint32_t* cnt, const Item* list, int ci, unsigned char* out)
+constexpr uint32_t kStageSeq = kStageHead - 4;    // the copy's sequence number, written last
+static_assert(C_NCNT * 4 <= kStageSeq, "counters must fit the stage head");
+// Copy the counters and the head of a list into mapped host memory, then
+// publish `seq`: every thread makes its stores visible to the host first, so
+// a host that reads seq (acquire) reads the whole copy.  The stream runs
+// k_stage after everything enqueued before it, so seq also says that all of
+// that work is complete.
+__global__ __launch_bounds__(kWG) void k_stage(const uint32_t* cnt, const Item* list, int ci, unsigned char* out,
+                                               uint32_t seq)
 {
     if (tid_x() < C_NCNT) reinterpret_cast<uint32_t*>(out)[tid_x()] = cnt[tid_x()];
-    if (!list) return;
-    const uint32_t nl = min(cnt[ci], kPeek);
-    Item* dst = reinterpret_cast<Item*>(out + kStageHead);
-    for (uint32_t i = tid_x(); i < nl; i += kWG) dst[i] = list[i];
+    if (list) {
+        const uint32_t nl = min(cnt[ci], kPeek);
+        Item* dst = reinterpret_cast<Item*>(out + kStageHead);
+        for (uint32_t i = tid_x(); i < nl; i += kWG) dst[i] = list[i];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (tid_x() == 0)
+        __hip_atomic_store(reinterpret_cast<uint32_t*>(out + kStageSeq), seq, __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // copy 8-byte words of a level plan from mapped host memory to the device:
@@ -2233,6 +2246,7 @@ struct ps5_ctx {
     // host memory (no copy engine on the readback path)
     unsigned char* h_stage = nullptr;    // pinned, mapped, coherent
     unsigned char* h_stage_dev = nullptr;    // its device address
+    uint32_t stage_seq = 0;                  // sequence number of the last staged copy
     uint32_t* h_upcnt = nullptr;     // pinned: counters uploaded with batch roots
     Item* h_roots = nullptr;         // pinned: root items of a batch
     uint64_t roots_cap = 0;
@@ -2570,9 +2584,27 @@ Sink make_sink(ps5_ctx* c, Item* large, int c_large, int rp)
 int readback(ps5_ctx* c, const Item* list, int ci)
 {
     // one staged copy holds the counters and the head of the list
-    hipLaunchKernelGGL(k_stage, dim3(1), dim3(kWG), 0, c->stream, c->d_cnt, list, ci, c->h_stage_dev);
+    const uint32_t seq = ++c->stage_seq ? c->stage_seq : ++c->stage_seq;     // (never 0)
+    hipLaunchKernelGGL(k_stage, dim3(1), dim3(kWG), 0, c->stream, c->d_cnt, list, ci, c->h_stage_dev, seq);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
+    // wait for the copy's sequence number rather than for the stream to go
+    // idle: the host sees it as soon as the GPU writes it (the stream is
+    // queried now and then, so that a failed stream ends the wait)
+    {
+        const uint32_t* f = reinterpret_cast<const uint32_t*>(c->h_stage + kStageSeq);
+        for (uint32_t spin = 1;; ++spin) {
+            if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == seq) break;
+            if ((spin & 1023) == 0) {
+                const hipError_t e = hipStreamQuery(c->stream);
+                if (e == hipSuccess) {
+                    if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == seq) break;
+                    return set_err(PS5_ERR_INTERNAL, "the staged copy's sequence number never arrived");
+                }
+                if (e != hipErrorNotReady) return set_err(PS5_ERR_HIP, "readback: %s", hipGetErrorString(e));
+            }
+            __builtin_ia32_pause();
+        }
+    }
     c->st_syncs++;
     memcpy(c->h_cnt, c->h_stage, C_NCNT * sizeof(uint32_t));
     if (c->h_cnt[C_ERR] & ERR_CAP) return set_err(PS5_ERR_CAPACITY, "work list capacity exceeded");
@@ -3436,7 +3468,7 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             for (uint32_t r = 0; r < c->graph_rounds; ++r) graph_round(c, R, r, L, nl_root);
             hipLaunchKernelGGL(k_stage, dim3(1), dim3(kWG), 0, st, c->d_cnt, c->w.l_round[c->graph_rounds & 1],
-                               C_LROUND + (int)(c->graph_rounds & 1), c->h_stage_dev);
+                               C_LROUND + (int)(c->graph_rounds & 1), c->h_stage_dev, 0u);
             hipGraph_t g = nullptr;
             const hipError_t ec = hipStreamEndCapture(st, &g);
             if (ec != hipSuccess) return set_err(PS5_ERR_HIP, "graph capture failed: %s", hipGetErrorString(ec));
