@@ -165,7 +165,7 @@ def bench_batch(ps5, args):
             graph_same &= bool(np.array_equal(gl[b + 1:e], lcp[b + 1:e]))
     del gl
     ctx.set_graph_rounds(0)
-    ctx.set_planning(2)     # back to the default (automatic) for the plain per-set sorts
+    ctx.set_planning(0)     # back to the default (host planning) for the plain per-set sorts
     # plain path: one ps5_sort_device call per set (uint32 offsets local to the set)
     starts = np.array([int(offs[int(sf[k])]) for k in range(args.sets)], np.uint64)
     ends = np.append(starts[1:], np.uint64(arena.size - 16))      # sets are laid out back to back

@@ -139,9 +139,9 @@ uint64_t ps5_noalloc_max_strings(void);
  * grids sized exactly.  1 (device): a kernel plans each level from the
  * device's own lists and every kernel loops over the work it finds, so a
  * round of levels runs without any host synchronisation (one per round, to
- * learn whether another round is needed).  2 (auto, the default): device
- * planning for a single sort of fewer than 4M strings, host planning for
- * larger sorts, batches and profiled sorts.  All modes give the same
+ * learn whether another round is needed).  2 (auto): device planning for a
+ * single sort of fewer than 4M strings, host planning for larger sorts,
+ * batches and profiled sorts.  The default is 0.  All modes give the same
  * results.  No counterpart in the reference (its levels are recursion on
  * the CPU).  The environment variable PS5_DEV_LEVELS=0/1/2 sets a new
  * context's mode.  Returns PS5_ERR_ARG for any other mode. */

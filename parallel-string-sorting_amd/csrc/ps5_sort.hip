@@ -2267,7 +2267,7 @@ struct ps5_ctx {
     uint32_t min_tiles = kMinTiles;          // a small level's chunks shrink to give this many tiles
     uint32_t leaf_keys = 8;                  // keys a leaf pass fetches per string at a time
     uint64_t seed = 0x5EED5EEDull;
-    int planning = 2;                // 0: levels planned on the host, 1: on the device (no host synchronisation
+    int planning = 0;                // 0: levels planned on the host, 1: on the device (no host synchronisation
                                      // inside a round), 2: on the device for a single sort of fewer than
                                      // kAutoDevMax strings, else on the host
     // a sort's first rounds as one captured HIP graph (0: off), and the
@@ -2822,10 +2822,12 @@ int setup_roots(ps5_ctx* c, uint32_t n, const uint64_t* set_first, uint32_t nset
 constexpr uint32_t kMaxGraphRounds = 64;      // rounds one captured graph may hold
 constexpr uint32_t kForkLeafGroups = 8192;    // leaf workgroups worth a side stream
 constexpr uint64_t kChainMin = 1u << 22;       // strings of a level whose successor is chained on the device
-// Automatic planning plans a single sort of fewer strings on the device: there
-// the host synchronisations of host-planned levels outweigh the device plan's
-// bounded grids (0.24 vs 0.23 ms on 0.5 M strings, 0.32 vs 0.30 on 1.3 M),
-// from about 4 M strings on they no longer do (0.57 vs 0.58 ms on 6.4 M).
+// Automatic planning plans a single sort of fewer strings on the device.
+// From about 4 M strings on the device plan's bounded grids cost more than
+// the host's synchronisations (0.489 vs 0.50 ms on 4 M strings); below, the
+// two are within a few percent either way, depending on the input (device
+// planning 6-9% faster on 0.13 M digit strings and 0.64 M ASCII strings, 4-7%
+// slower on 0.32 M ASCII strings and on high-LCP windows).
 constexpr uint64_t kAutoDevMax = 1u << 22;
 
 // Largest grids of the device-planned levels: their kernels loop over the

@@ -415,8 +415,8 @@ class Context:
         return tuple(x.value for x in v), ms.value
 
     def set_planning(self, mode):
-        """0: levels planned on the host; 1: on the device (no synchronisation inside a round);
-        2 (the default): on the device for a single sort of fewer than 4M strings, else on the host."""
+        """0 (the default): levels planned on the host; 1: on the device (no synchronisation inside a
+        round); 2: on the device for a single sort of fewer than 4M strings, else on the host."""
         _check(load().ps5_ctx_set_planning(self.h, int(mode)), "ps5_ctx_set_planning")
 
     def get_planning(self):

@@ -909,7 +909,7 @@ def test_planning_mode_values(ps5):
     c = ps5.Context(0)
     try:
         env = os.environ.get("PS5_DEV_LEVELS")
-        assert c.get_planning() == (min(max(int(env), 0), 2) if env else 2)
+        assert c.get_planning() == (min(max(int(env), 0), 2) if env else 0)
         for mode in (0, 1, 2):
             c.set_planning(mode)
             assert c.get_planning() == mode
