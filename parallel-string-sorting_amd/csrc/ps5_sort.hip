@@ -2581,30 +2581,37 @@ Sink make_sink(ps5_ctx* c, Item* large, int c_large, int rp)
 }
 
 // copy the counters (and the large list `list` with counter `ci`) to the host
+// Wait until the staged copy's sequence number reads `seq`, rather than for
+// the stream to go idle: the host sees the number as soon as the GPU writes
+// it.  The stream is queried every 1024 spins, so that a failed stream ends
+// the wait with its error.
+constexpr uint32_t kGraphSeqPending = 0xFFFFFFFFu;     // set by the host before a captured graph's replay
+int wait_stage(ps5_ctx* c, uint32_t seq)
+{
+    const uint32_t* f = reinterpret_cast<const uint32_t*>(c->h_stage + kStageSeq);
+    for (uint32_t spin = 1;; ++spin) {
+        if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == seq) return PS5_OK;
+        if ((spin & 1023) == 0) {
+            const hipError_t e = hipStreamQuery(c->stream);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == seq) return PS5_OK;
+                return set_err(PS5_ERR_INTERNAL, "the staged copy's sequence number never arrived");
+            }
+            if (e != hipErrorNotReady) return set_err(PS5_ERR_HIP, "waiting for a staged copy: %s", hipGetErrorString(e));
+        }
+        __builtin_ia32_pause();
+    }
+}
+
 int readback(ps5_ctx* c, const Item* list, int ci)
 {
     // one staged copy holds the counters and the head of the list
-    const uint32_t seq = ++c->stage_seq ? c->stage_seq : ++c->stage_seq;     // (never 0)
+    uint32_t seq = ++c->stage_seq;
+    if (seq == 0 || seq == kGraphSeqPending) seq = c->stage_seq = 1;     // (values the graph path uses)
     hipLaunchKernelGGL(k_stage, dim3(1), dim3(kWG), 0, c->stream, c->d_cnt, list, ci, c->h_stage_dev, seq);
     HIPCHK(hipGetLastError());
-    // wait for the copy's sequence number rather than for the stream to go
-    // idle: the host sees it as soon as the GPU writes it (the stream is
-    // queried now and then, so that a failed stream ends the wait)
-    {
-        const uint32_t* f = reinterpret_cast<const uint32_t*>(c->h_stage + kStageSeq);
-        for (uint32_t spin = 1;; ++spin) {
-            if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == seq) break;
-            if ((spin & 1023) == 0) {
-                const hipError_t e = hipStreamQuery(c->stream);
-                if (e == hipSuccess) {
-                    if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == seq) break;
-                    return set_err(PS5_ERR_INTERNAL, "the staged copy's sequence number never arrived");
-                }
-                if (e != hipErrorNotReady) return set_err(PS5_ERR_HIP, "readback: %s", hipGetErrorString(e));
-            }
-            __builtin_ia32_pause();
-        }
-    }
+    int rc = wait_stage(c, seq);
+    if (rc) return rc;
     c->st_syncs++;
     memcpy(c->h_cnt, c->h_stage, C_NCNT * sizeof(uint32_t));
     if (c->h_cnt[C_ERR] & ERR_CAP) return set_err(PS5_ERR_CAPACITY, "work list capacity exceeded");
@@ -3478,8 +3485,10 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
             HIPCHK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
             c->graph_key = key;
         }
+        // the graph's k_stage publishes 0 last; the host marks the word first
+        __atomic_store_n(reinterpret_cast<uint32_t*>(c->h_stage + kStageSeq), kGraphSeqPending, __ATOMIC_RELEASE);
         HIPCHK(hipGraphLaunch(c->graph_exec, st));
-        HIPCHK(hipStreamSynchronize(st));
+        if ((rc = wait_stage(c, 0u))) return rc;
         c->st_syncs++;
         c->st_rounds += c->graph_rounds;
         memcpy(c->h_cnt, c->h_stage, C_NCNT * sizeof(uint32_t));
