@@ -3057,7 +3057,7 @@ int dev_level_kmax(const ps5_ctx* c, uint64_t m, uint32_t level)
 // the host, and the blocks listed before a level are sorted in it
 // (blk_done: blocks already sorted per size class).
 int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t* blk_done, bool fused_root,
-                bool* leaves_forked)
+                bool* leaves_forked, uint32_t reset_mask)
 {
     hipStream_t st = c->stream;
     const uint32_t depth = round * 8;
@@ -3072,6 +3072,22 @@ int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t*
             nblk[cls] = c->h_cnt[C_BLOCK + 2 * cls + rp] - blk_done[cls];
             nblk_all += nblk[cls];
         }
+        Item* nx_large = c->w.l_lvl[level & 1];
+        int nx_ci = C_LLVL + (level & 1);
+        if (nx_ci == cur_ci) {      // (a round continued from device levels may start on either list)
+            nx_large = c->w.l_lvl[(level & 1) ^ 1];
+            nx_ci = C_LLVL + ((level & 1) ^ 1);
+        }
+        // counters to clear before anything of this level is enqueued (the
+        // forked leaves included): the caller's (the next round's lists, at
+        // level 0) and, when a large step runs, the next level's list (every
+        // counter starts at zero in round 0)
+        if (nlarge && (round > 0 || level > 0 || cur_ci != C_LROUND + rp)) reset_mask |= 1u << nx_ci;
+        if (reset_mask) {
+            Launch L(c, K_MISC, 0);
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, reset_mask);
+            reset_mask = 0;
+        }
         if (nlarge == 0 && nblk_all == 0) break;
         c->st_levels++;
         // the last level (blocks only): the round's other leaves are all
@@ -3085,16 +3101,6 @@ int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t*
             launch_leaves(c, B, rp, nsink, depth, leaf_tiny_grid(nt), leaf_run_grid(nr), nt, nr, c->side);
             HIPCHK(hipEventRecord(c->ev_join, c->side));
             *leaves_forked = true;
-        }
-        Item* nx_large = c->w.l_lvl[level & 1];
-        int nx_ci = C_LLVL + (level & 1);
-        if (nx_ci == cur_ci) {      // (a round continued from device levels may start on either list)
-            nx_large = c->w.l_lvl[(level & 1) ^ 1];
-            nx_ci = C_LLVL + ((level & 1) ^ 1);
-        }
-        if (round > 0 || level > 0 || cur_ci != C_LROUND + rp) {     // (every counter starts at zero)
-            Launch L(c, K_MISC, 0);
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, 1u << nx_ci);
         }
         const Sink sink = make_sink(c, nx_large, nx_ci, rp);
         const uint64_t seed = c->seed ^ ((uint64_t)round << 40) ^ ((uint64_t)level << 32);
@@ -3205,18 +3211,23 @@ int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
     const uint32_t depth = round * 8;
     const int rp = round & 1, np = rp ^ 1;
     c->st_rounds++;
-    if (round > 0) {   // the lists of round r+1 are empty again (consumed in round r-1; set up empty in round 0)
+    uint32_t round_mask = 0;     // the lists of round r+1 are empty again (consumed in round r-1; set up empty in round 0)
+    if (round > 0) {
+        round_mask = (1u << (C_LROUND + np)) | (1u << (C_TINY + np)) | (1u << (C_RUN + np));
+        for (int cls = 0; cls < kBlkClasses; ++cls)
+            round_mask |= (1u << (C_BLOCK + 2 * cls + np)) | (1u << (C_BDONE + cls));
+        round_mask |= 1u << (C_RUNCONT + np);
+    }
+    if (round_mask && dev) {     // (host-planned levels clear them with their first level's counters)
         Launch L(c, K_MISC, 0);
-        uint32_t mask = (1u << (C_LROUND + np)) | (1u << (C_TINY + np)) | (1u << (C_RUN + np));
-        for (int cls = 0; cls < kBlkClasses; ++cls) mask |= (1u << (C_BLOCK + 2 * cls + np)) | (1u << (C_BDONE + cls));
-        mask |= 1u << (C_RUNCONT + np);
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, mask);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, c->d_cnt, round_mask);
     }
     const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
     uint32_t blk_done[kBlkClasses] = {};
     bool leaves_forked = false;
     if (!dev) {
-        if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, fused_root, &leaves_forked))) return rc;
+        if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, fused_root, &leaves_forked, round_mask)))
+            return rc;
         if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
     } else {
         // every level of the round without a host synchronisation.  The
@@ -3239,7 +3250,7 @@ int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
         }
         const uint32_t L = nl0 ? dev_levels_for(c, mmax) : 0;
         if (L == 0) {
-            if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, false, &leaves_forked))) return rc;
+            if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, false, &leaves_forked, 0u))) return rc;
             if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
         } else {
             int in_ci = C_LROUND + rp;
@@ -3294,7 +3305,7 @@ int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
                 // back and finish the round on the host
                 for (int cls = 0; cls < kBlkClasses; ++cls) blk_done[cls] = c->h_cnt[C_BLOCK + 2 * cls + rp];
                 if ((rc = readback(c, in_list, in_ci))) return rc;
-                if ((rc = host_levels(c, B, round, in_ci, blk_done, false, &leaves_forked))) return rc;
+                if ((rc = host_levels(c, B, round, in_ci, blk_done, false, &leaves_forked, 0u))) return rc;
                 if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
             } else {
                 uint32_t nb = 0;
