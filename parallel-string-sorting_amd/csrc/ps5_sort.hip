@@ -13,6 +13,7 @@
 // pass is a kernel over a device-side work list, and buckets small enough for
 // LDS are finished by one workgroup without another trip through HBM.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -2861,7 +2862,7 @@ uint32_t block_grid_full(const ps5_ctx* c, int cls)
 // first + count) (or, with dev_first, from the device counter C_BDONE + cls
 // to the list's count), `grid` workgroups.
 void launch_block_class(ps5_ctx* c, const Bufs& B, int rp, int cls, uint32_t first, uint32_t count, bool dev_first,
-                        uint32_t grid, const Sink& nsink, uint32_t depth, bool tail = false)
+                        uint32_t grid, const Sink& nsink, uint32_t depth, bool tail = false, int flags = 0)
 {
     const uint32_t last = first + count;
     hipStream_t st = c->stream;
@@ -2874,11 +2875,11 @@ void launch_block_class(ps5_ctx* c, const Bufs& B, int rp, int cls, uint32_t fir
 #define PS5_BLOCK_LAUNCH(M, E, LOOP, TAIL)                                                                         \
     do {                                                                                                           \
         if (c->leaf_keys == 8)                                                                                     \
-            hipLaunchKernelGGL((k_block<M, E, LOOP, 8, TAIL>), dim3(grid), dim3(M / E), 0, st, list, first, last,  \
-                               pf, cnt, cap, B, nsink, depth);                                                     \
+            hipExtLaunchKernelGGL((k_block<M, E, LOOP, 8, TAIL>), dim3(grid), dim3(M / E), 0, st, nullptr, nullptr, \
+                                  flags, list, first, last, pf, cnt, cap, B, nsink, depth);                          \
         else                                                                                                       \
-            hipLaunchKernelGGL((k_block<M, E, LOOP, 0, TAIL>), dim3(grid), dim3(M / E), 0, st, list, first, last,  \
-                               pf, cnt, cap, B, nsink, depth);                                                     \
+            hipExtLaunchKernelGGL((k_block<M, E, LOOP, 0, TAIL>), dim3(grid), dim3(M / E), 0, st, nullptr, nullptr, \
+                                  flags, list, first, last, pf, cnt, cap, B, nsink, depth);                          \
     } while (0)
     if (dev_first) {
         if (cls == 0) PS5_BLOCK_LAUNCH(256, kBlkE0, true, false);
@@ -3111,9 +3112,11 @@ int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t*
         // last work: its blocks finish every string themselves (tail mode)
         // and no further round is needed
         const bool tail = c->tail_blocks && nlarge == 0 && c->h_cnt[C_RUNCONT + rp] == 0;
+        int bflags = 0;     // the first class waits for the level; the others may run beside it
         for (int cls = 0; cls < kBlkClasses; ++cls) {
             if (!nblk[cls]) continue;
-            launch_block_class(c, B, rp, cls, blk_done[cls], nblk[cls], false, nblk[cls], nsink, depth, tail);
+            launch_block_class(c, B, rp, cls, blk_done[cls], nblk[cls], false, nblk[cls], nsink, depth, tail, bflags);
+            if (!c->profile) bflags = hipExtAnyOrderLaunch;
             sum_sizes(c, c->w.l_block[rp] + (uint64_t)cls * c->w.cap_block, blk_done[cls],
                       c->d_cnt + C_BLOCK + 2 * cls + rp, blk_done[cls] + nblk[cls], S_BLOCK);
             blk_done[cls] += nblk[cls];
