@@ -2903,25 +2903,25 @@ void launch_block_class(ps5_ctx* c, const Bufs& B, int rp, int cls, uint32_t fir
 // Tiny leaves and equal-key chunks of round parity rp on stream s (grids of
 // g_tiny and g_run workgroups; 0 skips a list).
 void launch_leaves(ps5_ctx* c, const Bufs& B, int rp, const Sink& nsink, uint32_t depth, uint32_t g_tiny,
-                   uint32_t g_run, uint64_t n_tiny, uint64_t n_run, hipStream_t s)
+                   uint32_t g_run, uint64_t n_tiny, uint64_t n_run, hipStream_t s, int flags = 0)
 {
     if (g_tiny) {
         {
             Launch L(c, K_LEAF_TINY, 0);
             if (c->leaf_keys == 8)
-                hipLaunchKernelGGL(k_leaf_tiny<8>, dim3(g_tiny), dim3(kWG), 0, s, c->w.l_tiny[rp],
-                                   c->d_cnt + C_TINY + rp, c->w.cap_tiny, B, depth);
+                hipExtLaunchKernelGGL(k_leaf_tiny<8>, dim3(g_tiny), dim3(kWG), 0, s, nullptr, nullptr, flags,
+                                      c->w.l_tiny[rp], c->d_cnt + C_TINY + rp, c->w.cap_tiny, B, depth);
             else
-                hipLaunchKernelGGL(k_leaf_tiny<0>, dim3(g_tiny), dim3(kWG), 0, s, c->w.l_tiny[rp],
-                                   c->d_cnt + C_TINY + rp, c->w.cap_tiny, B, depth);
+                hipExtLaunchKernelGGL(k_leaf_tiny<0>, dim3(g_tiny), dim3(kWG), 0, s, nullptr, nullptr, flags,
+                                      c->w.l_tiny[rp], c->d_cnt + C_TINY + rp, c->w.cap_tiny, B, depth);
         }
         sum_sizes(c, c->w.l_tiny[rp], 0, c->d_cnt + C_TINY + rp, (uint32_t)n_tiny, S_TINY);
     }
     if (g_run) {
         {
             Launch L(c, K_LEAF_RUN, 0);
-            hipLaunchKernelGGL(k_leaf_run, dim3(g_run), dim3(kWG), 0, s, c->w.l_run[rp], c->d_cnt + C_RUN + rp,
-                               c->w.cap_run, B, nsink, depth);
+            hipExtLaunchKernelGGL(k_leaf_run, dim3(g_run), dim3(kWG), 0, s, nullptr, nullptr, flags, c->w.l_run[rp],
+                                  c->d_cnt + C_RUN + rp, c->w.cap_run, B, nsink, depth);
         }
         sum_sizes(c, c->w.l_run[rp], 0, c->d_cnt + C_RUN + rp, (uint32_t)n_run, S_RUN);
     }
@@ -3058,8 +3058,9 @@ int dev_level_kmax(const ps5_ctx* c, uint64_t m, uint32_t level)
 // the host, and the blocks listed before a level are sorted in it
 // (blk_done: blocks already sorted per size class).
 int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t* blk_done, bool fused_root,
-                bool* leaves_forked, uint32_t reset_mask)
+                bool* leaves_forked, uint32_t reset_mask, bool* after_blocks)
 {
+    *after_blocks = false;     // the stream's last packets are the last level's block kernels
     hipStream_t st = c->stream;
     const uint32_t depth = round * 8;
     const int rp = round & 1, np = rp ^ 1;
@@ -3117,6 +3118,7 @@ int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t*
             if (!nblk[cls]) continue;
             launch_block_class(c, B, rp, cls, blk_done[cls], nblk[cls], false, nblk[cls], nsink, depth, tail, bflags);
             if (!c->profile) bflags = hipExtAnyOrderLaunch;
+            *after_blocks = nlarge == 0;
             sum_sizes(c, c->w.l_block[rp] + (uint64_t)cls * c->w.cap_block, blk_done[cls],
                       c->d_cnt + C_BLOCK + 2 * cls + rp, blk_done[cls] + nblk[cls], S_BLOCK);
             blk_done[cls] += nblk[cls];
@@ -3164,7 +3166,7 @@ int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t*
 
 // Leaves and leaf boundaries of a round planned on the host (h_cnt holds
 // the round's final counts), then the read-back of the next round's lists.
-int host_round_end(ps5_ctx* c, const Bufs& B, uint32_t round, bool leaves_forked)
+int host_round_end(ps5_ctx* c, const Bufs& B, uint32_t round, bool leaves_forked, bool after_blocks)
 {
     hipStream_t st = c->stream;
     const uint32_t depth = round * 8;
@@ -3180,7 +3182,11 @@ int host_round_end(ps5_ctx* c, const Bufs& B, uint32_t round, bool leaves_forked
                 c->h_cnt[C_BLOCK + 6 + rp], nr, (unsigned long long)c->st_levels);
     }
     if (leaves_forked) HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
-    else launch_leaves(c, B, rp, nsink, depth, leaf_tiny_grid(nt), leaf_run_grid(nr), nt, nr, st);
+    // right after the last level's block kernels (the first of which waited
+    // for everything before it) the leaves may start beside them; the leaf
+    // boundaries below wait for all of them
+    else launch_leaves(c, B, rp, nsink, depth, leaf_tiny_grid(nt), leaf_run_grid(nr), nt, nr, st,
+                       after_blocks && !c->profile ? hipExtAnyOrderLaunch : 0);
     if (nt + nb + nr && B.lcp) launch_leafbound(c, B, rp, depth, div_up((uint64_t)nt + nb + nr, kWG), nt + nb + nr);
     HIPCHK(hipGetLastError());
     return readback(c, c->w.l_round[np], C_LROUND + np);
@@ -3227,11 +3233,12 @@ int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
     }
     const Sink nsink = make_sink(c, c->w.l_round[np], C_LROUND + np, np);
     uint32_t blk_done[kBlkClasses] = {};
-    bool leaves_forked = false;
+    bool leaves_forked = false, after_blocks = false;
     if (!dev) {
-        if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, fused_root, &leaves_forked, round_mask)))
+        if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, fused_root, &leaves_forked, round_mask,
+                              &after_blocks)))
             return rc;
-        if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
+        if ((rc = host_round_end(c, B, round, leaves_forked, after_blocks))) return rc;
     } else {
         // every level of the round without a host synchronisation.  The
         // host knows the round's lists (read back with the previous
@@ -3253,8 +3260,9 @@ int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
         }
         const uint32_t L = nl0 ? dev_levels_for(c, mmax) : 0;
         if (L == 0) {
-            if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, false, &leaves_forked, 0u))) return rc;
-            if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
+            if ((rc = host_levels(c, B, round, C_LROUND + rp, blk_done, false, &leaves_forked, 0u, &after_blocks)))
+                return rc;
+            if ((rc = host_round_end(c, B, round, leaves_forked, after_blocks))) return rc;
         } else {
             int in_ci = C_LROUND + rp;
             const Item* in_list = c->w.l_round[rp];
@@ -3308,8 +3316,9 @@ int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
                 // back and finish the round on the host
                 for (int cls = 0; cls < kBlkClasses; ++cls) blk_done[cls] = c->h_cnt[C_BLOCK + 2 * cls + rp];
                 if ((rc = readback(c, in_list, in_ci))) return rc;
-                if ((rc = host_levels(c, B, round, in_ci, blk_done, false, &leaves_forked, 0u))) return rc;
-                if ((rc = host_round_end(c, B, round, leaves_forked))) return rc;
+                if ((rc = host_levels(c, B, round, in_ci, blk_done, false, &leaves_forked, 0u, &after_blocks)))
+                    return rc;
+                if ((rc = host_round_end(c, B, round, leaves_forked, after_blocks))) return rc;
             } else {
                 uint32_t nb = 0;
                 for (int cls = 0; cls < kBlkClasses; ++cls) nb += c->h_cnt[C_BLOCK + 2 * cls + rp];
