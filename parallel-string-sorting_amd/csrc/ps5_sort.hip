@@ -3280,6 +3280,7 @@ int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
                 dev_level(c, B, in_list, in_ci, level == 0 ? nl0 : deeper_large, S, sink, seed,
                           fused_root && rootlvl, rootlvl, rootlvl ? (int)root_kmax : dev_level_kmax(c, mmax, level),
                           level > 0 ? rp : -1);
+                int bflags = 0;     // the first class waits for the level; the others run beside it
                 for (int cls = 0; cls < kBlkClasses; ++cls) {
                     // four workgroups for every wave slot of the GPU (32
                     // per CU): blocks differ in size up to 2x, and the
@@ -3288,7 +3289,8 @@ int sort_round(ps5_ctx* c, SortRun& R, uint32_t round, bool dev, bool* done)
                     const uint64_t full = block_grid_full(c, cls);
                     const uint64_t known = level == 0 ? c->h_cnt[C_BLOCK + 2 * cls + rp] : 0;
                     const uint32_t g = (uint32_t)std::min<uint64_t>(full, known + S / minm + 1);
-                    launch_block_class(c, B, rp, cls, 0, 0, true, g, nsink, depth);
+                    launch_block_class(c, B, rp, cls, 0, 0, true, g, nsink, depth, false, bflags);
+                    if (!c->profile) bflags = hipExtAnyOrderLaunch;
                 }
                 HIPCHK(hipGetLastError());
                 in_ci = out_ci;
