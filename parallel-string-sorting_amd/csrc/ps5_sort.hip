@@ -3060,7 +3060,7 @@ int dev_level_kmax(const ps5_ctx* c, uint64_t m, uint32_t level)
 int host_levels(ps5_ctx* c, const Bufs& B, uint32_t round, int cur_ci, uint32_t* blk_done, bool fused_root,
                 bool* leaves_forked, uint32_t reset_mask, bool* after_blocks)
 {
-    *after_blocks = false;     // the stream's last packets are the last level's block kernels
+    *after_blocks = false;     // set when the stream's last packets are the last level's block kernels
     hipStream_t st = c->stream;
     const uint32_t depth = round * 8;
     const int rp = round & 1, np = rp ^ 1;
