@@ -257,6 +257,45 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total,
     return wpre + x - v;
 }
 
+// Exclusive block scans of K uint32 per thread at once (NW waves): ex[k] is
+// the sum of v[k] over the threads before this one, tot[k] the block total.
+// Two barriers for all K (s_tmp: NW * K entries).
+template <int NW, int K>
+__device__ __forceinline__ void block_excl_scan_k(const uint32_t (&v)[K], uint32_t (&ex)[K], uint32_t (&tot)[K],
+                                                  uint32_t* s_tmp)
+{
+    const uint32_t lane = lane_id(), w = wave_id();
+    uint32_t x[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] = v[k];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t y = (uint32_t)__shfl_up((int)x[k], d, 64);
+            if (lane >= (uint32_t)d) x[k] += y;
+        }
+    }
+    if (lane == 63) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) s_tmp[w * K + k] = x[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        uint32_t wpre = 0, t = 0;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) {
+            const uint32_t u = s_tmp[i * K + k];
+            wpre += (i < (int)w) ? u : 0u;
+            t += u;
+        }
+        ex[k] = wpre + x[k] - v[k];
+        tot[k] = t;
+    }
+    __syncthreads();
+}
+
 // In-order index of level-order node t of a complete tree with k levels.
 __device__ __forceinline__ int inorder_of(int t, int k)
 {

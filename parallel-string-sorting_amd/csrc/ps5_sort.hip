@@ -840,6 +840,8 @@ __global__ __launch_bounds__(kBigWG) void k_scatter(Plan P, Bufs B, const uint32
 //
 // spl: the segment's sorted splitters; equal bucket b holds the key spl[b >> 1],
 // and one whose key holds no NUL continues into the next round (counted).
+constexpr int kChildScan = 32;     // scan scratch of emit_children (at least kWaves * 7 entries)
+
 __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart, uint32_t stride, uint32_t base,
                               const key64* spl, const Sink& s, uint32_t* s_tmp)
 {
@@ -867,26 +869,32 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
             else c_l++;
         }
     }
-    uint32_t tt, tl, tr, tb[kBlkClasses], o_b[kBlkClasses];
-    uint32_t o_t = block_excl_scan<kWaves>(c_t, &tt, s_tmp);
-    uint32_t o_l = block_excl_scan<kWaves>(c_l, &tl, s_tmp);
-    uint32_t o_r = block_excl_scan<kWaves>(c_r, &tr, s_tmp);
+    // the list offsets of this thread's children: one scan of all seven
+    // counts (tiny, large, equal-key chunks, the block classes), then one
+    // atomic per list for the segment's place in it
+    constexpr int kLists = 3 + kBlkClasses;
+    static_assert(kWaves * kLists <= kChildScan, "scan scratch");
+    uint32_t cv[kLists], ov[kLists], tv[kLists];
+    cv[0] = c_t;
+    cv[1] = c_l;
+    cv[2] = c_r;
 #pragma unroll
-    for (int c = 0; c < kBlkClasses; ++c) o_b[c] = block_excl_scan<kWaves>(c_b[c], &tb[c], s_tmp);
+    for (int c = 0; c < kBlkClasses; ++c) cv[3 + c] = c_b[c];
+    block_excl_scan_k<kWaves, kLists>(cv, ov, tv, s_tmp);
     if (c_cont) atomicAdd(&s.cnt[s.c_runcont], c_cont);
+    uint32_t* s_base = s_tmp + kChildScan;
     if (tid_x() == 0) {
-        s_tmp[8] = tt ? atomicAdd(&s.cnt[s.c_tiny], tt) : 0;
-        s_tmp[9] = tl ? atomicAdd(&s.cnt[s.c_large], tl) : 0;
-        s_tmp[10] = tr ? atomicAdd(&s.cnt[s.c_run], tr) : 0;
+        s_base[0] = tv[0] ? atomicAdd(&s.cnt[s.c_tiny], tv[0]) : 0;
+        s_base[1] = tv[1] ? atomicAdd(&s.cnt[s.c_large], tv[1]) : 0;
+        s_base[2] = tv[2] ? atomicAdd(&s.cnt[s.c_run], tv[2]) : 0;
 #pragma unroll
-        for (int c = 0; c < kBlkClasses; ++c) s_tmp[11 + c] = tb[c] ? atomicAdd(&s.cnt[s.c_block + 2 * c], tb[c]) : 0;
+        for (int c = 0; c < kBlkClasses; ++c)
+            s_base[3 + c] = tv[3 + c] ? atomicAdd(&s.cnt[s.c_block + 2 * c], tv[3 + c]) : 0;
     }
     __syncthreads();
-    o_t += s_tmp[8];
-    o_l += s_tmp[9];
-    o_r += s_tmp[10];
+    uint32_t o_t = ov[0] + s_base[0], o_l = ov[1] + s_base[1], o_r = ov[2] + s_base[2], o_b[kBlkClasses];
 #pragma unroll
-    for (int c = 0; c < kBlkClasses; ++c) o_b[c] += s_tmp[11 + c];
+    for (int c = 0; c < kBlkClasses; ++c) o_b[c] = ov[3 + c] + s_base[3 + c];
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int b = tid_x() * kPer + j;
@@ -917,7 +925,7 @@ __device__ void emit_children(const Item& seg, int nbkt, const uint32_t* bstart,
 
 __global__ __launch_bounds__(kWG) void k_children(Plan P, const uint32_t* hscan, const key64* trees, Sink s)
 {
-    __shared__ uint32_t s_tmp[16];
+    __shared__ uint32_t s_tmp[kChildScan + 8];     // the counts' scan, then the lists' bases
     const uint32_t nlarge = P.hdr->nlarge;
     for (uint32_t si = blockIdx.x; si < nlarge; si += gridDim.x) {
         const LargeDesc D = P.desc[si];
