@@ -106,8 +106,19 @@ __global__ __launch_bounds__(kWG) void k_stage(const uint32_t* cnt, const Item* 
 // copy 8-byte words of a level plan from mapped host memory to the device:
 // the header with the descriptors, then the tile table
 __global__ __launch_bounds__(kWG) void k_fetch(const uint64_t* src, uint64_t* dst, uint64_t words,
-                                               const uint64_t* src2, uint64_t* dst2, uint64_t words2)
+                                               const uint64_t* src2, uint64_t* dst2, uint64_t words2,
+                                               uint32_t* setup_cnt, Item* setup_list, uint32_t setup_n)
 {
+    // a single root's set-up (k_setup), when this fetch is the sort's first
+    // kernel: one launch fewer before the root level
+    if (setup_cnt && blockIdx.x == 0) {
+        if (tid_x() < C_NCNT) setup_cnt[tid_x()] = 0;
+        __syncthreads();
+        if (tid_x() == 0) {
+            setup_list[0] = Item{ 0u, setup_n, 0u, F_IDENT };
+            setup_cnt[C_LROUND] = 1;
+        }
+    }
     const uint64_t stride = (uint64_t)gridDim.x * kWG;
     for (uint64_t i = (uint64_t)blockIdx.x * kWG + tid_x(); i < words; i += stride) dst[i] = src[i];
     for (uint64_t i = (uint64_t)blockIdx.x * kWG + tid_x(); i < words2; i += stride) dst2[i] = src2[i];
@@ -2256,6 +2267,7 @@ struct ps5_ctx {
     unsigned char* h_stage = nullptr;    // pinned, mapped, coherent
     unsigned char* h_stage_dev = nullptr;    // its device address
     uint32_t stage_seq = 0;                  // sequence number of the last staged copy
+    uint32_t pending_root = 0;               // strings of a single root whose set-up the next plan fetch does
     uint32_t* h_upcnt = nullptr;     // pinned: counters uploaded with batch roots
     Item* h_roots = nullptr;         // pinned: root items of a batch
     uint64_t roots_cap = 0;
@@ -2732,7 +2744,9 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
                            dim3(kWG), 0, st, reinterpret_cast<const uint64_t*>(c->w.h_plan_dev),
                            reinterpret_cast<uint64_t*>(c->w.d_plan), words,
                            reinterpret_cast<const uint64_t*>(c->w.h_plan_dev + toff),
-                           reinterpret_cast<uint64_t*>(c->w.d_plan + toff), words2);
+                           reinterpret_cast<uint64_t*>(c->w.d_plan + toff), words2,
+                           c->pending_root ? c->d_cnt : nullptr, c->w.l_round[0], c->pending_root);
+        c->pending_root = 0;
     }
     const Plan P = plan_view(c->w.d_plan, c->w.cap);
     const uint32_t hl32 = (uint32_t)hl;
@@ -3461,13 +3475,16 @@ int run_sort(ps5_ctx* c, const uint8_t* arena, uint64_t alen, const uint32_t* of
     // known to the host without asking.
     const bool single_root = !set_first && n > kBlkT;
     const bool fused_root = single_root || (set_first && root_sum > 0);
-    if ((rc = setup_roots(c, n, set_first, nsets))) return rc;
-    if ((rc = init_root_keys(c, B, n, set_first, fused_root))) return rc;
-    // device-planned levels: the plan buffers take their largest size now
-    // (the levels' sizes are not known on the host)
     bool dev = c->planning == 1 ||
                (c->planning == 2 && !c->profile && !set_first && n < kAutoDevMax);   // (profiled sorts: host, whose
                                                                                      // launches know their bytes)
+    // a single root planned on the host: the root level's plan fetch, the
+    // sort's first kernel, also sets the counters and the root item up
+    c->pending_root = single_root && !dev && !(c->graph_rounds && !c->profile) ? n : 0u;
+    if (!c->pending_root && (rc = setup_roots(c, n, set_first, nsets))) return rc;
+    if ((rc = init_root_keys(c, B, n, set_first, fused_root))) return rc;
+    // device-planned levels: the plan buffers take their largest size now
+    // (the levels' sizes are not known on the host)
     if ((dev || c->chain_levels) && (rc = grow_level(c, c->w.cap_large, c->w.cap + c->w.cap / 4 + 64))) return rc;
     if (single_root) {
         for (int i = 0; i < C_NCNT; ++i) c->h_cnt[i] = 0;
