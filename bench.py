@@ -452,11 +452,12 @@ def rocprof_summary(d):
             lines.append(f"| {k} | {m['SQ_INSTS_VALU'] / 1e6:.1f}M | {m['SQ_INSTS_SALU'] / 1e6:.1f}M | "
                          f"{m['SQ_INSTS_LDS'] / 1e6:.1f}M | {m['SQ_LDS_BANK_CONFLICT'] / 1e6:.1f}M | "
                          f"{m['SQ_WAIT_ANY'] / wc:.2f} | {m['SQ_WAIT_INST_LDS'] / wc:.2f} |")
-    # dispatch timeline of the last complete sort (from one k_tree at level 0 to the next)
+    # dispatch timeline of the last complete sort: a sort starts with k_setup,
+    # or with k_root_fetch (a single root planned on the host)
     tr = sorted(glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True))
     if tr:
         rows = sorted(csv.DictReader(open(tr[0])), key=lambda r: int(r["Start_Timestamp"]))
-        starts = [i for i, r in enumerate(rows) if _kernel_base(r["Kernel_Name"]) == "k_setup"]
+        starts = [i for i, r in enumerate(rows) if _kernel_base(r["Kernel_Name"]) in ("k_setup", "k_root_fetch")]
         if len(starts) >= 2:
             seq = rows[starts[-2]:starts[-1]]
             t0 = int(seq[0]["Start_Timestamp"])

@@ -105,23 +105,36 @@ __global__ __launch_bounds__(kWG) void k_stage(const uint32_t* cnt, const Item* 
 
 // copy 8-byte words of a level plan from mapped host memory to the device:
 // the header with the descriptors, then the tile table
-__global__ __launch_bounds__(kWG) void k_fetch(const uint64_t* src, uint64_t* dst, uint64_t words,
-                                               const uint64_t* src2, uint64_t* dst2, uint64_t words2,
-                                               uint32_t* setup_cnt, Item* setup_list, uint32_t setup_n)
+__device__ __forceinline__ void fetch_plan(const uint64_t* src, uint64_t* dst, uint64_t words, const uint64_t* src2,
+                                           uint64_t* dst2, uint64_t words2)
 {
-    // a single root's set-up (k_setup), when this fetch is the sort's first
-    // kernel: one launch fewer before the root level
-    if (setup_cnt && blockIdx.x == 0) {
-        if (tid_x() < C_NCNT) setup_cnt[tid_x()] = 0;
-        __syncthreads();
-        if (tid_x() == 0) {
-            setup_list[0] = Item{ 0u, setup_n, 0u, F_IDENT };
-            setup_cnt[C_LROUND] = 1;
-        }
-    }
     const uint64_t stride = (uint64_t)gridDim.x * kWG;
     for (uint64_t i = (uint64_t)blockIdx.x * kWG + tid_x(); i < words; i += stride) dst[i] = src[i];
     for (uint64_t i = (uint64_t)blockIdx.x * kWG + tid_x(); i < words2; i += stride) dst2[i] = src2[i];
+}
+
+__global__ __launch_bounds__(kWG) void k_fetch(const uint64_t* src, uint64_t* dst, uint64_t words,
+                                               const uint64_t* src2, uint64_t* dst2, uint64_t words2)
+{
+    fetch_plan(src, dst, words, src2, dst2, words2);
+}
+
+// The same for the root level of a single root planned on the host, the
+// sort's first kernel: its first workgroup also does k_setup's work
+// (counters, the root item), one launch fewer before the root level.
+__global__ __launch_bounds__(kWG) void k_root_fetch(const uint64_t* src, uint64_t* dst, uint64_t words,
+                                                    const uint64_t* src2, uint64_t* dst2, uint64_t words2,
+                                                    uint32_t* cnt, Item* list, uint32_t n)
+{
+    if (blockIdx.x == 0) {
+        if (tid_x() < C_NCNT) cnt[tid_x()] = 0;
+        __syncthreads();
+        if (tid_x() == 0) {
+            list[0] = Item{ 0u, n, 0u, F_IDENT };
+            cnt[C_LROUND] = 1;
+        }
+    }
+    fetch_plan(src, dst, words, src2, dst2, words2);
 }
 
 // zero the counters whose bit is set in `mask`
@@ -2740,12 +2753,16 @@ int large_step(ps5_ctx* c, const Bufs& B, uint32_t nlarge, const Sink& sink, uin
     {   // the plan: pulled from mapped host memory by a kernel (no copy engine)
         const uint64_t words = (sizeof(LevelHdr) + nlarge * sizeof(LargeDesc)) / 8, words2 = ntiles * sizeof(TileRef) / 8;
         const uint64_t toff = plan_tiles_off(c->w.cap);
-        hipLaunchKernelGGL(k_fetch, dim3((uint32_t)std::min<uint64_t>(div_up(std::max(words, words2), kWG), 1024)),
-                           dim3(kWG), 0, st, reinterpret_cast<const uint64_t*>(c->w.h_plan_dev),
-                           reinterpret_cast<uint64_t*>(c->w.d_plan), words,
-                           reinterpret_cast<const uint64_t*>(c->w.h_plan_dev + toff),
-                           reinterpret_cast<uint64_t*>(c->w.d_plan + toff), words2,
-                           c->pending_root ? c->d_cnt : nullptr, c->w.l_round[0], c->pending_root);
+        const dim3 g((uint32_t)std::min<uint64_t>(div_up(std::max(words, words2), kWG), 1024));
+        const uint64_t* s1 = reinterpret_cast<const uint64_t*>(c->w.h_plan_dev);
+        const uint64_t* s2 = reinterpret_cast<const uint64_t*>(c->w.h_plan_dev + toff);
+        uint64_t* d1 = reinterpret_cast<uint64_t*>(c->w.d_plan);
+        uint64_t* d2 = reinterpret_cast<uint64_t*>(c->w.d_plan + toff);
+        if (c->pending_root)
+            hipLaunchKernelGGL(k_root_fetch, g, dim3(kWG), 0, st, s1, d1, words, s2, d2, words2, c->d_cnt,
+                               c->w.l_round[0], c->pending_root);
+        else
+            hipLaunchKernelGGL(k_fetch, g, dim3(kWG), 0, st, s1, d1, words, s2, d2, words2);
         c->pending_root = 0;
     }
     const Plan P = plan_view(c->w.d_plan, c->w.cap);
